@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark: Owl's conv2d layer forward + backward_input + backward_kernel on
+MI355X, BF16 MFMA path against the plain FP32 path.
+
+Workload (the VGG-style layer): x [64,56,56,64] ~ N(0,1), kernel [3,3,64,128]
+He-normal (std sqrt(2/576)), stride 1, SAME; output gradient [64,56,56,128]
+~ N(0,1); then max_pool2d 2x2 stride 2 VALID forward (with argmax) and
+backward on the FP32 layer output.  A "step" is the three conv passes
+(2 * 3 * 29.595 GFLOP = 88.785 GFLOP) over resident HBM buffers.
+
+Prints one JSON line (rank 0, N = 1).  ``value`` is the BF16 step's TFLOP/s.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+B, H, W, CI, CO, K = 64, 56, 56, 64, 128, 3
+FLOP_PASS = 2.0 * B * H * W * CO * K * K * CI  # 29.595 GFLOP
+PEAK = {"bf16": 2.5e15, "f16": 2.5e15, "f32": 157.3e12}
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 oracle error check")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from owl_amd import ndarray as nd
+    from owl_amd.shapes import SAME, VALID
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(1234)
+    x32 = torch.randn(B, H, W, CI, device=dev, generator=g)
+    w32 = torch.randn(K, K, CI, CO, device=dev, generator=g) * (2.0 / (K * K * CI)) ** 0.5
+    dy32 = torch.randn(B, H, W, CO, device=dev, generator=g)
+    xb, wb, dyb = x32.bfloat16(), w32.bfloat16(), dy32.bfloat16()
+
+    def step(x, w, dy, out_w):
+        y = nd.conv2d(x, w, [1, 1], SAME)
+        dx = nd.conv2d_backward_input(x, w, [1, 1], dy)
+        dw = nd.conv2d_backward_kernel(x, w, [1, 1], dy, out_dtype=out_w)
+        return y, dx, dw
+
+    def timed(x, w, dy, out_w, steps, warmup):
+        for _ in range(warmup):
+            step(x, w, dy, out_w)
+        # per-pass HIP events on the stream the library launches on
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            e = ev[i]
+            e[0].record()
+            nd.conv2d(x, w, [1, 1], SAME)
+            e[1].record()
+            nd.conv2d_backward_input(x, w, [1, 1], dy)
+            e[2].record()
+            nd.conv2d_backward_kernel(x, w, [1, 1], dy, out_dtype=out_w)
+            e[3].record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        per = [statistics.mean(ev[i][j].elapsed_time(ev[i][j + 1]) for i in range(steps)) for j in range(3)]
+        return wall, per
+
+    wall_b, per_b = timed(xb, wb, dyb, torch.float32, args.steps, args.warmup)
+    wall_f, per_f = timed(x32, w32, dy32, torch.float32, args.steps, args.warmup)
+    tf_b = 3 * FLOP_PASS * args.steps / wall_b / 1e12
+    tf_f = 3 * FLOP_PASS * args.steps / wall_f / 1e12
+
+    # pooling on the FP32 layer output
+    y32 = nd.conv2d(x32, w32, [1, 1], SAME)
+    gp = torch.randn(B, H // 2, W // 2, CO, device=dev, generator=g)
+    for _ in range(3):
+        nd.max_pool2d_argmax(y32, [2, 2], [2, 2], VALID)
+        nd.max_pool2d_backward(VALID, y32, [2, 2], [2, 2], gp)
+    torch.cuda.synchronize()
+    pe = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    reps = 10
+    pe[0].record()
+    for _ in range(reps):
+        nd.max_pool2d(y32, [2, 2], [2, 2], VALID)
+    pe[1].record()
+    for _ in range(reps):
+        nd.max_pool2d_argmax(y32, [2, 2], [2, 2], VALID)
+    pe[2].record()
+    for _ in range(reps):
+        nd.max_pool2d_backward(VALID, y32, [2, 2], [2, 2], gp)
+    pe[3].record()
+    torch.cuda.synchronize()
+    ny, no = B * H * W * CO * 4, B * (H // 2) * (W // 2) * CO * 4
+    t_pf, t_pa, t_pb = (pe[i].elapsed_time(pe[i + 1]) / reps * 1e-3 for i in range(3))
+    pool = {"fwd_GBps": round((ny + no) / t_pf / 1e9, 1),
+            "argmax_GBps": round((ny + no + 2 * no) / t_pa / 1e9, 1),
+            "bwd_GBps": round((ny + no + ny) / t_pb / 1e9, 1),
+            "fwd_frac_hbm": round((ny + no) / t_pf / HBM_PEAK, 3),
+            "bwd_frac_hbm": round((ny + no + ny) / t_pb / HBM_PEAK, 3)}
+
+    # errors against the FP64 oracle on a 2-image slice (conv is per-image for
+    # fwd / bwd_input; bwd_kernel is re-run on the slice)
+    errs = None
+    if not args.no_check:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_ndarray as on
+        n = 2
+        xs64 = xb[:n].double().cpu().numpy()
+        ws64 = wb.double().cpu().numpy()
+        gs64 = dyb[:n].double().cpu().numpy()
+        yb, dxb, _ = step(xb, wb, dyb, torch.float32)
+        dwb = nd.conv2d_backward_kernel(xb[:n].contiguous(), wb, [1, 1], dyb[:n].contiguous(), out_dtype=torch.float32)
+        x64u = x32[:n].double().cpu().numpy()
+        w64u = w32.double().cpu().numpy()
+        g64u = dy32[:n].double().cpu().numpy()
+        ry = on.conv2d(x64u, w64u, [1, 1], SAME)
+        rdx = on.conv2d_backward_input(x64u, w64u, [1, 1], g64u)
+        rdw = on.conv2d_backward_kernel(x64u, w64u, [1, 1], g64u)
+        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        yf, dxf, _ = step(x32, w32, dy32, torch.float32)
+        dwf = nd.conv2d_backward_kernel(x32[:n].contiguous(), w32, [1, 1], dy32[:n].contiguous())
+        errs = {
+            "bf16_rel_l2_vs_fp64": {"fwd": rel(yb[:n].double().cpu().numpy(), ry),
+                                    "bwd_input": rel(dxb[:n].double().cpu().numpy(), rdx),
+                                    "bwd_kernel": rel(dwb.double().cpu().numpy(), rdw)},
+            "fp32_rel_l2_vs_fp64": {"fwd": rel(yf[:n].double().cpu().numpy(), ry),
+                                    "bwd_input": rel(dxf[:n].double().cpu().numpy(), rdx),
+                                    "bwd_kernel": rel(dwf.double().cpu().numpy(), rdw)},
+            "slice": f"first {n} images (bwd_kernel recomputed on the slice)",
+        }
+        del xs64, ws64, gs64
+
+    # roofline of the dominant BF16 kernel
+    names = ["fwd", "bwd_input", "bwd_kernel"]
+    j = int(np.argmax(per_b))
+    roof = {"kernel": f"conv_igemm_kernel<{j},bf16> ({names[j]})", "bound": "mfma",
+            "achieved": round(FLOP_PASS / (per_b[j] * 1e-3) / 1e12, 2), "peak": PEAK["bf16"] / 1e12,
+            "unit": "TFLOP/s", "traffic": None}
+    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+
+    out = {
+        "metric": "conv fwd+bwd TFLOP/s & % MFMA peak, BF16 vs FP32 plain, at rel-err <=1e-2",
+        "value": round(tf_b, 2),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_b / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded N(0,1) activations, He-normal kernel)",
+        "config": {"workload": "Owl conv2d VGG layer fwd+bwd_input+bwd_kernel, x[64,56,56,64] "
+                               "w[3,3,64,128] stride 1 SAME, BF16 in/out (dw FP32), FP32 accumulate",
+                   "gflop_per_step": round(3 * FLOP_PASS / 1e9, 3)},
+        "bf16_frac_of_peak": round(tf_b * 1e12 / PEAK["bf16"], 4),
+        "fp32": {"value": round(tf_f, 2), "ms_per_step": round(wall_f / args.steps * 1e3, 4),
+                 "frac_of_peak": round(tf_f * 1e12 / PEAK["f32"], 4)},
+        "bf16_over_fp32": round(tf_b / tf_f, 2),
+        "per_pass_ms": {"bf16": dict(zip(names, [round(v, 4) for v in per_b])),
+                        "fp32": dict(zip(names, [round(v, 4) for v in per_f]))},
+        "errors": errs,
+        "pool_max2x2s2": pool,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline():
+    """Owl's CPU algorithm (im2col + OpenBLAS SGEMM, restated in oracle/) on a
+    16-image slice of the same layer, all host threads, median of 5."""
+    import oracle
+    rng = np.random.default_rng(0)
+    n = 16
+    x = rng.standard_normal([n, H, W, CI]).astype(np.float32)
+    w = (rng.standard_normal([K, K, CI, CO]) * (2.0 / (K * K * CI)) ** 0.5).astype(np.float32)
+    dy = rng.standard_normal([n, H, W, CO]).astype(np.float32)
+    g = oracle.conv_geom([n, H, W, CI], [K, K, CI, CO], [H, W], [1, 1], [1, 1], [1, 1])
+    lib, _, path = oracle.openblas_sgemm()
+    threads = None
+    for sym in ("scipy_openblas_get_num_threads", "openblas_get_num_threads"):
+        f = getattr(lib, sym, None)
+        if f is not None:
+            threads = int(f())
+            break
+    def run():
+        oracle.conv_fwd_owl_f32(g, x, w)
+        oracle.conv_bwd_input_owl_f32(g, w, dy)
+        oracle.conv_bwd_kernel_owl_f32(g, x, dy)
+    run()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    t = statistics.median(ts)
+    flop = 3 * FLOP_PASS * n / B
+    return {"value": round(flop / t / 1e12, 4), "unit": "TFLOP/s", "cores": threads or os.cpu_count(),
+            "kind": "port", "sample": f"{n}-image slice of the layer, fwd+bwd_input+bwd_kernel, FP32 "
+                                      f"im2col + OpenBLAS sgemm ({os.path.basename(path)}), median of 5",
+            "seconds_per_sample": round(t, 3)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,248 @@
+/*
+ * owl_amd.h -- C ABI of the MI355X (gfx950) implementation of Owl's
+ * convolution and pooling hot path.
+ *
+ * Two layers, both plain C (pointers, 64-bit sizes, status codes):
+ *
+ * 1. Owl drop-in entry points, one per C stub that Owl's OCaml externals bind
+ *    (src/owl/core/owl_ndarray_conv.ml, owl_ndarray_pool.ml).  Each takes the
+ *    reference stub's arguments in the reference's order, with host (Bigarray)
+ *    data pointers, and is synchronous: on return the output host buffer is
+ *    complete, as Owl's CPU code guarantees.  The only difference from the
+ *    reference is the error path: a status code instead of exit(1)
+ *    (owl_ndarray_conv_impl.h:299-300); the OCaml shim turns it into
+ *    caml_failwith.  Naming: stub_<dtype>_ndarray_<op>_native  ->
+ *    owl_amd_<dtype>_ndarray_<op>.
+ *
+ * 2. Device entry points (owl_amd_conv_*, owl_amd_pool_*) on device pointers
+ *    and a caller stream, used by the drop-ins and by callers that keep
+ *    tensors resident in HBM.  They enqueue work and return without waiting.
+ *
+ * Layouts are Owl's: input [B][H][W][D][Ci] (Owl calls H "cols", W "rows",
+ * D "dpts"), kernel [kh][kw][kd][Ci][Co], output [B][Ho][Wo][Do][Co]; a 2-D
+ * call is the 3-D one with D = kd = Do = 1.
+ */
+#ifndef OWL_AMD_H
+#define OWL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  OWL_AMD_OK = 0,
+  OWL_AMD_ERR_ARG = 1,         /* invalid sizes / pointers */
+  OWL_AMD_ERR_HIP = 2,         /* HIP runtime error */
+  OWL_AMD_ERR_UNSUPPORTED = 3, /* dtype / variant not provided */
+  OWL_AMD_ERR_NOMEM = 4        /* device allocation failed */
+};
+
+typedef enum {
+  OWL_AMD_F32 = 0,
+  OWL_AMD_F64 = 1,
+  OWL_AMD_BF16 = 2,
+  OWL_AMD_F16 = 3,
+  OWL_AMD_FP8E4M3 = 4
+} owl_amd_dtype;
+
+typedef struct {
+  int64_t B, H, W, D, Ci; /* input  [B][H][W][D][Ci] */
+  int64_t kh, kw, kd, Co; /* kernel [kh][kw][kd][Ci][Co] */
+  int64_t Ho, Wo, Do;     /* output [B][Ho][Wo][Do][Co] */
+  int64_t sh, sw, sd;     /* strides */
+  int64_t rh, rw, rd;     /* dilation rates (Owl: *_in_stride) */
+  int64_t ph, pw, pd;     /* leading pads */
+} owl_amd_conv_desc;
+
+typedef struct {
+  int64_t B, H, W, D, C;
+  int64_t kh, kw, kd; /* window */
+  int64_t Ho, Wo, Do;
+  int64_t sh, sw, sd;
+  int64_t ph, pw, pd;
+} owl_amd_pool_desc;
+
+/* ---------------------------------------------------------------- library */
+int owl_amd_init(int device);
+const char *owl_amd_last_error(void);
+/* Number of device buffers and bytes cached by the drop-in layer. */
+int owl_amd_cache_stats(int64_t *n_buffers, int64_t *bytes);
+void owl_amd_cache_release(void);
+
+/* ---------------------------------------------------------------- device API */
+/* Forward convolution y = x (*) w.  Replaces the arithmetic of
+ * owl_ndarray_conv_impl.h:246-453 (spatial), :1211-1319 (cuboid_im2col),
+ * :3097-3190 / :3417-3533 (dilated).  in_t: F32, F64, BF16, F16 (inputs and
+ * kernel); out_t: F32 or in_t. */
+int owl_amd_conv_fwd(const owl_amd_conv_desc *d, int in_t, int out_t,
+                     const void *x, const void *w, void *y, void *stream);
+
+/* dx = backward_input(w, dy): owl_ndarray_conv_impl.h:465-664, :1456-1554,
+ * :3317-3406, :3678-3784.  Gather formulation, no atomics. */
+int owl_amd_conv_bwd_input(const owl_amd_conv_desc *d, int in_t, int out_t,
+                           const void *w, const void *dy, void *dx, void *stream);
+
+/* dw = backward_kernel(x, dy): owl_ndarray_conv_impl.h:675-888, :1331-1444,
+ * :3203-3306, :3545-3666.  Split-K over B*Ho*Wo*Do with FP32 partials and a
+ * deterministic compensated (Neumaier) reduction.  workspace may be NULL, in
+ * which case the library uses its own cached device buffer. */
+size_t owl_amd_conv_bwd_kernel_workspace(const owl_amd_conv_desc *d, int in_t);
+int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int out_t,
+                            const void *x, const void *dy, void *dw,
+                            void *workspace, size_t ws_bytes, void *stream);
+
+/* Pooling (owl_ndarray_pool_impl.h): results bit-identical to Owl's C. */
+int owl_amd_pool_fwd(const owl_amd_pool_desc *d, int t, int is_max,
+                     const void *x, void *y, void *stream);
+int owl_amd_pool_argmax(const owl_amd_pool_desc *d, int t, const void *x,
+                        void *y, int64_t *argmax, void *stream);
+int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max,
+                     const void *x, const void *dy, void *dx, void *stream);
+
+/* ---------------------------------------------------------------- Owl drop-ins */
+/* Conv 2-D: replaces stub_<T>_ndarray_conv_spatial_native
+ * (owl_ndarray_conv_impl.h:246-253; external owl_ndarray_conv.ml:13-32). */
+#define OWL_AMD_CONV2D_FWD_ARGS(T)                                                  \
+  T *input, T *kernel, T *output, int64_t batches, int64_t input_cols,              \
+      int64_t input_rows, int64_t in_channel, int64_t kernel_cols,                  \
+      int64_t kernel_rows, int64_t output_cols, int64_t output_rows,                \
+      int64_t out_channel, int64_t row_stride, int64_t col_stride, int64_t padding, \
+      int64_t row_in_stride, int64_t col_in_stride
+/* Conv 2-D backward: owl_ndarray_conv_impl.h:465-471 / :675-681 (no padding
+ * argument; input is written by backward_input, kernel by backward_kernel). */
+#define OWL_AMD_CONV2D_BWD_ARGS(T)                                                  \
+  T *input, T *kernel, T *output, int64_t batches, int64_t input_cols,              \
+      int64_t input_rows, int64_t in_channel, int64_t kernel_cols,                  \
+      int64_t kernel_rows, int64_t output_cols, int64_t output_rows,                \
+      int64_t out_channel, int64_t row_stride, int64_t col_stride,                  \
+      int64_t row_in_stride, int64_t col_in_stride
+/* Conv 3-D: owl_ndarray_conv_impl.h:1211-1220 (forward), :1331-1338 (bwd). */
+#define OWL_AMD_CONV3D_FWD_ARGS(T)                                                  \
+  T *input, T *kernel, T *output, int64_t batches, int64_t input_cols,              \
+      int64_t input_rows, int64_t input_dpts, int64_t in_channel,                   \
+      int64_t kernel_cols, int64_t kernel_rows, int64_t kernel_dpts,                \
+      int64_t output_cols, int64_t output_rows, int64_t output_dpts,                \
+      int64_t out_channel, int64_t dpt_stride, int64_t row_stride,                  \
+      int64_t col_stride, int64_t padding
+#define OWL_AMD_CONV3D_BWD_ARGS(T)                                                  \
+  T *input, T *kernel, T *output, int64_t batches, int64_t input_cols,              \
+      int64_t input_rows, int64_t input_dpts, int64_t in_channel,                   \
+      int64_t kernel_cols, int64_t kernel_rows, int64_t kernel_dpts,                \
+      int64_t output_cols, int64_t output_rows, int64_t output_dpts,                \
+      int64_t out_channel, int64_t dpt_stride, int64_t row_stride,                  \
+      int64_t col_stride
+/* Dilated 3-D: owl_ndarray_conv_impl.h:3417-3427 (forward), :3545-3553 (bwd). */
+#define OWL_AMD_DCONV3D_FWD_ARGS(T)                                                 \
+  T *input, T *kernel, T *output, int64_t batches, int64_t input_cols,              \
+      int64_t input_rows, int64_t input_dpts, int64_t in_channel,                   \
+      int64_t kernel_cols, int64_t kernel_rows, int64_t kernel_dpts,                \
+      int64_t output_cols, int64_t output_rows, int64_t output_dpts,                \
+      int64_t out_channel, int64_t dpt_stride, int64_t row_stride,                  \
+      int64_t col_stride, int64_t dpt_in_stride, int64_t row_in_stride,             \
+      int64_t col_in_stride, int64_t padding
+#define OWL_AMD_DCONV3D_BWD_ARGS(T)                                                 \
+  T *input, T *kernel, T *output, int64_t batches, int64_t input_cols,              \
+      int64_t input_rows, int64_t input_dpts, int64_t in_channel,                   \
+      int64_t kernel_cols, int64_t kernel_rows, int64_t kernel_dpts,                \
+      int64_t output_cols, int64_t output_rows, int64_t output_dpts,                \
+      int64_t out_channel, int64_t dpt_stride, int64_t row_stride,                  \
+      int64_t col_stride, int64_t dpt_in_stride, int64_t row_in_stride,             \
+      int64_t col_in_stride
+/* Pool 2-D forward: owl_ndarray_pool_impl.h:9-15. */
+#define OWL_AMD_POOL2D_FWD_ARGS(T)                                                  \
+  T *input, T *output, int64_t batches, int64_t input_cols, int64_t input_rows,     \
+      int64_t in_channel, int64_t kernel_cols, int64_t kernel_rows,                 \
+      int64_t output_cols, int64_t output_rows, int64_t row_stride,                 \
+      int64_t col_stride, int64_t padding, int64_t row_in_stride,                   \
+      int64_t col_in_stride
+/* Pool 2-D backward: owl_ndarray_pool_impl.h:102-108 (pad_rows = W pad,
+ * pad_cols = H pad, from calc_conv2d_padding). */
+#define OWL_AMD_POOL2D_BWD_ARGS(T)                                                  \
+  T *input, T *output_back, T *input_back, int64_t batches, int64_t input_cols,     \
+      int64_t input_rows, int64_t in_channel, int64_t kernel_cols,                  \
+      int64_t kernel_rows, int64_t output_cols, int64_t output_rows,                \
+      int64_t row_stride, int64_t col_stride, int64_t pad_rows, int64_t pad_cols
+/* Max pool 2-D with argmax: owl_ndarray_pool_impl.h:459-465. */
+#define OWL_AMD_POOL2D_ARG_ARGS(T)                                                  \
+  T *input, T *output, int64_t *argmax, int64_t batches, int64_t input_cols,        \
+      int64_t input_rows, int64_t in_channel, int64_t kernel_cols,                  \
+      int64_t kernel_rows, int64_t output_cols, int64_t output_rows,                \
+      int64_t row_stride, int64_t col_stride, int64_t pad_rows, int64_t pad_cols
+/* Pool 3-D forward: owl_ndarray_pool_impl.h:212-219 (strides dpt,row,col). */
+#define OWL_AMD_POOL3D_FWD_ARGS(T)                                                  \
+  T *input, T *output, int64_t batches, int64_t input_cols, int64_t input_rows,     \
+      int64_t input_dpts, int64_t in_channel, int64_t kernel_cols,                  \
+      int64_t kernel_rows, int64_t kernel_dpts, int64_t output_cols,                \
+      int64_t output_rows, int64_t output_dpts, int64_t dpt_stride,                 \
+      int64_t row_stride, int64_t col_stride, int64_t padding
+/* Pool 3-D backward: owl_ndarray_pool_impl.h:324-331 (strides col,row,dpt). */
+#define OWL_AMD_POOL3D_BWD_ARGS(T)                                                  \
+  T *input, T *output_back, T *input_back, int64_t batches, int64_t input_cols,     \
+      int64_t input_rows, int64_t input_dpts, int64_t in_channel,                   \
+      int64_t kernel_cols, int64_t kernel_rows, int64_t kernel_dpts,                \
+      int64_t output_cols, int64_t output_rows, int64_t output_dpts,                \
+      int64_t col_stride, int64_t row_stride, int64_t dpt_stride, int64_t padding
+
+#define OWL_AMD_DECLARE_DROPINS(N, T)                                                   \
+  /* owl_ndarray_conv_impl.h:246 spatial, :903 spatial_im2col, :1570 mec, :2440 naive */ \
+  int owl_amd_##N##_ndarray_conv_spatial(OWL_AMD_CONV2D_FWD_ARGS(T));                    \
+  int owl_amd_##N##_ndarray_conv_spatial_im2col(OWL_AMD_CONV2D_FWD_ARGS(T));             \
+  int owl_amd_##N##_ndarray_conv_spatial_mec(OWL_AMD_CONV2D_FWD_ARGS(T));                \
+  int owl_amd_##N##_ndarray_conv_spatial_naive(OWL_AMD_CONV2D_FWD_ARGS(T));              \
+  /* :465 / :1116 / :1828 / :2642 */                                                    \
+  int owl_amd_##N##_ndarray_conv_spatial_backward_input(OWL_AMD_CONV2D_BWD_ARGS(T));     \
+  int owl_amd_##N##_ndarray_conv_spatial_backward_input_im2col(OWL_AMD_CONV2D_BWD_ARGS(T)); \
+  int owl_amd_##N##_ndarray_conv_spatial_backward_input_mec(OWL_AMD_CONV2D_BWD_ARGS(T)); \
+  int owl_amd_##N##_ndarray_conv_spatial_backward_input_naive(OWL_AMD_CONV2D_BWD_ARGS(T)); \
+  /* :675 / :1006 / :1699 / :2546 */                                                    \
+  int owl_amd_##N##_ndarray_conv_spatial_backward_kernel(OWL_AMD_CONV2D_BWD_ARGS(T));    \
+  int owl_amd_##N##_ndarray_conv_spatial_backward_kernel_im2col(OWL_AMD_CONV2D_BWD_ARGS(T)); \
+  int owl_amd_##N##_ndarray_conv_spatial_backward_kernel_mec(OWL_AMD_CONV2D_BWD_ARGS(T)); \
+  int owl_amd_##N##_ndarray_conv_spatial_backward_kernel_naive(OWL_AMD_CONV2D_BWD_ARGS(T)); \
+  /* :1211 cuboid_im2col, :1957 mec, :2735 naive */                                     \
+  int owl_amd_##N##_ndarray_conv_cuboid_im2col(OWL_AMD_CONV3D_FWD_ARGS(T));              \
+  int owl_amd_##N##_ndarray_conv_cuboid_mec(OWL_AMD_CONV3D_FWD_ARGS(T));                 \
+  int owl_amd_##N##_ndarray_conv_cuboid_naive(OWL_AMD_CONV3D_FWD_ARGS(T));               \
+  /* :1456 / :2278 / :2977 */                                                           \
+  int owl_amd_##N##_ndarray_conv_cuboid_backward_input_im2col(OWL_AMD_CONV3D_BWD_ARGS(T)); \
+  int owl_amd_##N##_ndarray_conv_cuboid_backward_input_mec(OWL_AMD_CONV3D_BWD_ARGS(T));  \
+  int owl_amd_##N##_ndarray_conv_cuboid_backward_input_naive(OWL_AMD_CONV3D_BWD_ARGS(T)); \
+  /* :1331 / :2118 / :2860 */                                                           \
+  int owl_amd_##N##_ndarray_conv_cuboid_backward_kernel_im2col(OWL_AMD_CONV3D_BWD_ARGS(T)); \
+  int owl_amd_##N##_ndarray_conv_cuboid_backward_kernel_mec(OWL_AMD_CONV3D_BWD_ARGS(T)); \
+  int owl_amd_##N##_ndarray_conv_cuboid_backward_kernel_naive(OWL_AMD_CONV3D_BWD_ARGS(T)); \
+  /* :3097 / :3317 / :3203 dilated spatial */                                           \
+  int owl_amd_##N##_ndarray_conv_dilated_spatial_im2col(OWL_AMD_CONV2D_FWD_ARGS(T));     \
+  int owl_amd_##N##_ndarray_conv_dilated_spatial_backward_input_im2col(OWL_AMD_CONV2D_BWD_ARGS(T)); \
+  int owl_amd_##N##_ndarray_conv_dilated_spatial_backward_kernel_im2col(OWL_AMD_CONV2D_BWD_ARGS(T)); \
+  /* :3417 / :3678 / :3545 dilated cuboid */                                            \
+  int owl_amd_##N##_ndarray_conv_dilated_cuboid_im2col(OWL_AMD_DCONV3D_FWD_ARGS(T));     \
+  int owl_amd_##N##_ndarray_conv_dilated_cuboid_backward_input_im2col(OWL_AMD_DCONV3D_BWD_ARGS(T)); \
+  int owl_amd_##N##_ndarray_conv_dilated_cuboid_backward_kernel_im2col(OWL_AMD_DCONV3D_BWD_ARGS(T)); \
+  /* owl_ndarray_pool_impl.h:9 spatial, :102 spatial_backward, :212 cuboid,          */ \
+  /* :324 cuboid_backward, :459 spatial_arg; macro sets owl_ndarray_pool_stub.c:16-51 */ \
+  int owl_amd_##N##_ndarray_maxpool_spatial(OWL_AMD_POOL2D_FWD_ARGS(T));                 \
+  int owl_amd_##N##_ndarray_avgpool_spatial(OWL_AMD_POOL2D_FWD_ARGS(T));                 \
+  int owl_amd_##N##_ndarray_maxpool_spatial_backward(OWL_AMD_POOL2D_BWD_ARGS(T));        \
+  int owl_amd_##N##_ndarray_avgpool_spatial_backward(OWL_AMD_POOL2D_BWD_ARGS(T));        \
+  int owl_amd_##N##_ndarray_maxpool_spatial_arg(OWL_AMD_POOL2D_ARG_ARGS(T));             \
+  int owl_amd_##N##_ndarray_maxpool_cuboid(OWL_AMD_POOL3D_FWD_ARGS(T));                  \
+  int owl_amd_##N##_ndarray_avgpool_cuboid(OWL_AMD_POOL3D_FWD_ARGS(T));                  \
+  int owl_amd_##N##_ndarray_maxpool_cuboid_backward(OWL_AMD_POOL3D_BWD_ARGS(T));         \
+  int owl_amd_##N##_ndarray_avgpool_cuboid_backward(OWL_AMD_POOL3D_BWD_ARGS(T));
+
+OWL_AMD_DECLARE_DROPINS(float32, float)
+OWL_AMD_DECLARE_DROPINS(float64, double)
+
+/* Complex dtypes (owl_ndarray_conv_stub.c:91-122) are not provided on the GPU:
+ * owl_amd_complex_supported() returns 0 and the OCaml shim keeps Owl's CPU
+ * stubs for complex32 / complex64. */
+int owl_amd_complex_supported(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OWL_AMD_H */

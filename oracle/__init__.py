@@ -1,0 +1,195 @@
+"""Python access to the CPU oracle (test infrastructure only).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this package; the product path never does.  The library
+``oracle/liboracle.so`` is built from ``oracle/owl_oracle.c`` by
+``oracle/Makefile`` (``__graft_entry__.build()`` runs it).
+"""
+import ctypes
+import glob
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class ConvGeom(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "B", "H", "W", "D", "Ci", "kh", "kw", "kd", "Co", "Ho", "Wo", "Do",
+        "sh", "sw", "sd", "rh", "rw", "rd", "ph", "pw", "pd")]
+
+
+class PoolGeom(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "B", "H", "W", "D", "C", "kh", "kw", "kd", "Ho", "Wo", "Do",
+        "sh", "sw", "sd", "ph", "pw", "pd")]
+
+
+SGEMM_T = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                           ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_float,
+                           ctypes.c_void_p, ctypes.c_int)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        for n in ("oracle_conv_fwd_f64", "oracle_conv_bwd_input_f64", "oracle_conv_bwd_kernel_f64"):
+            getattr(L, n).argtypes = [ctypes.POINTER(ConvGeom), vp, vp, vp]
+            getattr(L, n).restype = None
+        for n in ("oracle_conv_fwd_owl_f32", "oracle_conv_bwd_input_owl_f32",
+                  "oracle_conv_bwd_kernel_owl_f32"):
+            getattr(L, n).argtypes = [ctypes.POINTER(ConvGeom), vp, vp, vp, vp]
+            getattr(L, n).restype = ctypes.c_int
+        for s in ("f32", "f64"):
+            getattr(L, "oracle_pool_fwd_" + s).argtypes = [ctypes.POINTER(PoolGeom), ctypes.c_int, vp, vp]
+            getattr(L, "oracle_pool_argmax_" + s).argtypes = [ctypes.POINTER(PoolGeom), vp, vp, vp]
+            getattr(L, "oracle_pool_bwd_" + s).argtypes = [ctypes.POINTER(PoolGeom), ctypes.c_int, vp, vp, vp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------- geometry
+def conv_geom(x_shape, k_shape, out_sp, stride, rate, pads):
+    """x_shape [B,H,W,(D,)Ci], k_shape [kh,kw,(kd,)Ci,Co], out_sp (Ho,Wo[,Do])."""
+    if len(x_shape) == 4:
+        B, H, W, Ci = x_shape
+        D = 1
+        kh, kw, _, Co = k_shape
+        kd = 1
+        Ho, Wo = out_sp
+        Do = 1
+        stride, rate, pads = tuple(stride) + (1,), tuple(rate) + (1,), tuple(pads) + (0,)
+    else:
+        B, H, W, D, Ci = x_shape
+        kh, kw, kd, _, Co = k_shape
+        Ho, Wo, Do = out_sp
+    return ConvGeom(B, H, W, D, Ci, kh, kw, kd, Co, Ho, Wo, Do, *stride, *rate, *pads)
+
+
+def pool_geom(x_shape, win, out_sp, stride, pads):
+    if len(x_shape) == 4:
+        B, H, W, C = x_shape
+        D = 1
+        win, out_sp = tuple(win) + (1,), tuple(out_sp) + (1,)
+        stride, pads = tuple(stride) + (1,), tuple(pads) + (0,)
+    else:
+        B, H, W, D, C = x_shape
+    return PoolGeom(B, H, W, D, C, *win, *out_sp, *stride, *pads)
+
+
+# ---------------------------------------------------------------- conv
+def conv_fwd(g, x, w):
+    x = np.ascontiguousarray(x, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    y = np.zeros(g.B * g.Ho * g.Wo * g.Do * g.Co, np.float64)
+    lib().oracle_conv_fwd_f64(ctypes.byref(g), _p(x), _p(w), _p(y))
+    return y
+
+
+def conv_bwd_input(g, w, dy):
+    w = np.ascontiguousarray(w, np.float64)
+    dy = np.ascontiguousarray(dy, np.float64)
+    dx = np.zeros(g.B * g.H * g.W * g.D * g.Ci, np.float64)
+    lib().oracle_conv_bwd_input_f64(ctypes.byref(g), _p(w), _p(dy), _p(dx))
+    return dx
+
+
+def conv_bwd_kernel(g, x, dy):
+    x = np.ascontiguousarray(x, np.float64)
+    dy = np.ascontiguousarray(dy, np.float64)
+    dw = np.zeros(g.kh * g.kw * g.kd * g.Ci * g.Co, np.float64)
+    lib().oracle_conv_bwd_kernel_f64(ctypes.byref(g), _p(x), _p(dy), _p(dw))
+    return dw
+
+
+# ---------------------------------------------------------------- Owl's CPU algorithm
+_SGEMM = None
+
+
+def openblas_sgemm():
+    """cblas_sgemm of the OpenBLAS that ships with scipy / numpy in this image."""
+    global _SGEMM
+    if _SGEMM is None:
+        cands = []
+        for pat in ("scipy.libs/libscipy_openblas-*.so", "numpy.libs/libscipy_openblas64_*.so"):
+            for d in __import__("site").getsitepackages() + ["/usr/local/lib/python3.10/dist-packages"]:
+                cands += glob.glob(os.path.join(d, pat))
+        for c in cands:
+            L = ctypes.CDLL(c)
+            for sym in ("scipy_cblas_sgemm", "cblas_sgemm", "scipy_cblas_sgemm64_"):
+                f = getattr(L, sym, None)
+                if f is not None and "64_" not in sym:
+                    _SGEMM = (L, ctypes.cast(f, ctypes.c_void_p), c)
+                    break
+            if _SGEMM:
+                break
+        if _SGEMM is None:
+            raise RuntimeError("no LP64 OpenBLAS cblas_sgemm found")
+    return _SGEMM
+
+
+def conv_fwd_owl_f32(g, x, w):
+    x = np.ascontiguousarray(x, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    y = np.zeros(g.B * g.Ho * g.Wo * g.Do * g.Co, np.float32)
+    rc = lib().oracle_conv_fwd_owl_f32(ctypes.byref(g), _p(x), _p(w), _p(y), openblas_sgemm()[1])
+    assert rc == 0
+    return y
+
+
+def conv_bwd_input_owl_f32(g, w, dy):
+    w = np.ascontiguousarray(w, np.float32)
+    dy = np.ascontiguousarray(dy, np.float32)
+    dx = np.zeros(g.B * g.H * g.W * g.D * g.Ci, np.float32)
+    rc = lib().oracle_conv_bwd_input_owl_f32(ctypes.byref(g), _p(w), _p(dy), _p(dx), openblas_sgemm()[1])
+    assert rc == 0
+    return dx
+
+
+def conv_bwd_kernel_owl_f32(g, x, dy):
+    x = np.ascontiguousarray(x, np.float32)
+    dy = np.ascontiguousarray(dy, np.float32)
+    dw = np.zeros(g.kh * g.kw * g.kd * g.Ci * g.Co, np.float32)
+    rc = lib().oracle_conv_bwd_kernel_owl_f32(ctypes.byref(g), _p(x), _p(dy), _p(dw), openblas_sgemm()[1])
+    assert rc == 0
+    return dw
+
+
+# ---------------------------------------------------------------- pooling
+def _suf(dt):
+    return "f32" if np.dtype(dt) == np.float32 else "f64"
+
+
+def pool_fwd(g, x, is_max):
+    y = np.zeros(g.B * g.Ho * g.Wo * g.Do * g.C, x.dtype)
+    x = np.ascontiguousarray(x)
+    getattr(lib(), "oracle_pool_fwd_" + _suf(x.dtype))(ctypes.byref(g), int(is_max), _p(x), _p(y))
+    return y
+
+
+def pool_argmax(g, x):
+    x = np.ascontiguousarray(x)
+    y = np.zeros(g.B * g.Ho * g.Wo * g.Do * g.C, x.dtype)
+    a = np.zeros(g.B * g.Ho * g.Wo * g.Do * g.C, np.int64)
+    getattr(lib(), "oracle_pool_argmax_" + _suf(x.dtype))(ctypes.byref(g), _p(x), _p(y), _p(a))
+    return y, a
+
+
+def pool_bwd(g, x, dy, is_max):
+    x = np.ascontiguousarray(x)
+    dy = np.ascontiguousarray(dy, x.dtype)
+    dx = np.zeros(g.B * g.H * g.W * g.D * g.C, x.dtype)
+    getattr(lib(), "oracle_pool_bwd_" + _suf(x.dtype))(ctypes.byref(g), int(is_max), _p(x), _p(dy), _p(dx))
+    return dx

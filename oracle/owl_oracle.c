@@ -1,0 +1,365 @@
+/*
+ * owl_oracle.c -- CPU restatement of Owl's convolution / pooling hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.  The
+ * product path (owl_amd/) never links, loads or calls it.
+ *
+ * Conventions (Owl's, owl_ndarray_conv_impl.h:246-253 and :1211-1220):
+ *   input  x  [B][H][W][D][Ci]      (Owl names H "cols", W "rows", D "dpts")
+ *   kernel w  [kh][kw][kd][Ci][Co]  (HWIO / DHWIO, row-major)
+ *   output y  [B][Ho][Wo][Do][Co]
+ * A 2-D problem is the 3-D one with D = Do = kd = 1, which is exactly how
+ * the 2-D and 3-D templates index memory (the depth term vanishes).
+ * Strides s*, dilation rates r* (Owl "*_in_stride") and leading pads p* are
+ * per axis; the padding is computed by the caller from Owl's formula
+ * (owl_ndarray_conv_impl.h:288-294, 502-505, 3141-3150):
+ *     p = max(0, (s*(o-1) + k_eff - n) / 2)  with C truncating division,
+ *     k_eff = k + (k-1)*(r-1).
+ *
+ * Parity pins (tests/): Owl's golden vectors (test/unit_*conv*.ml,
+ * test/unit_pool*.ml), brute-force adjoint identities, 1x1 conv == GEMM,
+ * FP64 central finite differences.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+  int64_t B, H, W, D, Ci;
+  int64_t kh, kw, kd, Co;
+  int64_t Ho, Wo, Do;
+  int64_t sh, sw, sd;
+  int64_t rh, rw, rd;
+  int64_t ph, pw, pd;
+} oracle_conv_geom;
+
+typedef struct {
+  int64_t B, H, W, D, C;
+  int64_t kh, kw, kd;
+  int64_t Ho, Wo, Do;
+  int64_t sh, sw, sd;
+  int64_t ph, pw, pd;
+} oracle_pool_geom;
+
+#define IN_IDX(g, b, h, w, d, c) \
+  ((((((b) * (g)->H + (h)) * (g)->W + (w)) * (g)->D + (d)) * (g)->Ci) + (c))
+#define OUT_IDX(g, b, h, w, d, o) \
+  ((((((b) * (g)->Ho + (h)) * (g)->Wo + (w)) * (g)->Do + (d)) * (g)->Co) + (o))
+#define K_IDX(g, a, c, e, q, o) \
+  ((((((a) * (g)->kw + (c)) * (g)->kd + (e)) * (g)->Ci + (q)) * (g)->Co) + (o))
+
+/* ------------------------------------------------------------------------
+ * FP64 direct loops (the oracle of record).
+ * Forward: owl_ndarray_conv_impl.h:2440-2535 (spatial_naive),
+ * :2735-2850 (cuboid_naive), dilation as in :3097-3190.  Sums are carried in
+ * long double so the oracle sits well below FP64 rounding of the result.
+ * ---------------------------------------------------------------------- */
+void oracle_conv_fwd_f64(const oracle_conv_geom *g, const double *x,
+                         const double *w, double *y) {
+  for (int64_t b = 0; b < g->B; ++b)
+    for (int64_t i = 0; i < g->Ho; ++i)
+      for (int64_t j = 0; j < g->Wo; ++j)
+        for (int64_t l = 0; l < g->Do; ++l)
+          for (int64_t o = 0; o < g->Co; ++o) {
+            long double s = 0.0L;
+            for (int64_t a = 0; a < g->kh; ++a) {
+              int64_t h = i * g->sh - g->ph + a * g->rh;
+              if (h < 0 || h >= g->H) continue;
+              for (int64_t c = 0; c < g->kw; ++c) {
+                int64_t ww = j * g->sw - g->pw + c * g->rw;
+                if (ww < 0 || ww >= g->W) continue;
+                for (int64_t e = 0; e < g->kd; ++e) {
+                  int64_t d = l * g->sd - g->pd + e * g->rd;
+                  if (d < 0 || d >= g->D) continue;
+                  const double *xp = x + IN_IDX(g, b, h, ww, d, 0);
+                  const double *wp = w + K_IDX(g, a, c, e, 0, o);
+                  for (int64_t q = 0; q < g->Ci; ++q)
+                    s += (long double)xp[q] * (long double)wp[q * g->Co];
+                }
+              }
+            }
+            y[OUT_IDX(g, b, i, j, l, o)] = (double)s;
+          }
+}
+
+/* Backward input: dx[b,h,w,d,q] = sum over (i,j,l,a,c,e,o) with
+ * h = i*sh - ph + a*rh (etc.) of dy[b,i,j,l,o] * w[a,c,e,q,o]
+ * (owl_ndarray_conv_impl.h:2642-2725, :2977-3082).  Restated as a gather
+ * over output positions so every dx element is one long-double sum. */
+void oracle_conv_bwd_input_f64(const oracle_conv_geom *g, const double *w,
+                               const double *dy, double *dx) {
+  for (int64_t b = 0; b < g->B; ++b)
+    for (int64_t h = 0; h < g->H; ++h)
+      for (int64_t ww = 0; ww < g->W; ++ww)
+        for (int64_t d = 0; d < g->D; ++d)
+          for (int64_t q = 0; q < g->Ci; ++q) {
+            long double s = 0.0L;
+            for (int64_t a = 0; a < g->kh; ++a) {
+              int64_t th = h + g->ph - a * g->rh;
+              if (th < 0 || th % g->sh) continue;
+              int64_t i = th / g->sh;
+              if (i >= g->Ho) continue;
+              for (int64_t c = 0; c < g->kw; ++c) {
+                int64_t tw = ww + g->pw - c * g->rw;
+                if (tw < 0 || tw % g->sw) continue;
+                int64_t j = tw / g->sw;
+                if (j >= g->Wo) continue;
+                for (int64_t e = 0; e < g->kd; ++e) {
+                  int64_t td = d + g->pd - e * g->rd;
+                  if (td < 0 || td % g->sd) continue;
+                  int64_t l = td / g->sd;
+                  if (l >= g->Do) continue;
+                  const double *gp = dy + OUT_IDX(g, b, i, j, l, 0);
+                  const double *wp = w + K_IDX(g, a, c, e, q, 0);
+                  for (int64_t o = 0; o < g->Co; ++o)
+                    s += (long double)gp[o] * (long double)wp[o];
+                }
+              }
+            }
+            dx[IN_IDX(g, b, h, ww, d, q)] = (double)s;
+          }
+}
+
+/* Backward kernel: dw[a,c,e,q,o] = sum_{b,i,j,l} x[b,h(i,a),w(j,c),d(l,e),q]
+ * * dy[b,i,j,l,o]  (owl_ndarray_conv_impl.h:2546-2632, :2860-2967). */
+void oracle_conv_bwd_kernel_f64(const oracle_conv_geom *g, const double *x,
+                                const double *dy, double *dw) {
+  int64_t nk = g->kh * g->kw * g->kd * g->Ci * g->Co;
+  long double *acc = (long double *)calloc((size_t)nk, sizeof(long double));
+  for (int64_t b = 0; b < g->B; ++b)
+    for (int64_t i = 0; i < g->Ho; ++i)
+      for (int64_t j = 0; j < g->Wo; ++j)
+        for (int64_t l = 0; l < g->Do; ++l) {
+          const double *gp = dy + OUT_IDX(g, b, i, j, l, 0);
+          for (int64_t a = 0; a < g->kh; ++a) {
+            int64_t h = i * g->sh - g->ph + a * g->rh;
+            if (h < 0 || h >= g->H) continue;
+            for (int64_t c = 0; c < g->kw; ++c) {
+              int64_t ww = j * g->sw - g->pw + c * g->rw;
+              if (ww < 0 || ww >= g->W) continue;
+              for (int64_t e = 0; e < g->kd; ++e) {
+                int64_t d = l * g->sd - g->pd + e * g->rd;
+                if (d < 0 || d >= g->D) continue;
+                const double *xp = x + IN_IDX(g, b, h, ww, d, 0);
+                long double *ap = acc + K_IDX(g, a, c, e, 0, 0);
+                for (int64_t q = 0; q < g->Ci; ++q) {
+                  long double xv = xp[q];
+                  for (int64_t o = 0; o < g->Co; ++o)
+                    ap[q * g->Co + o] += xv * (long double)gp[o];
+                }
+              }
+            }
+          }
+        }
+  for (int64_t t = 0; t < nk; ++t) dw[t] = (double)acc[t];
+  free(acc);
+}
+
+/* ------------------------------------------------------------------------
+ * FP32 restatement of Owl's CPU algorithm (im2col + one SGEMM), used for
+ * the CPU baseline timing and the FP32 comparison.  Packing loops follow
+ * owl_ndarray_conv_impl.h:302-333 (forward), :515-546 (backward input:
+ * GEMM then serial col2im +=), :728-766 (backward kernel: GEMM into
+ * [Co][K] then transpose into HWIO).  The SGEMM is the host's OpenBLAS,
+ * passed in as a function pointer with the CBLAS signature.  Owl's blocked
+ * (GEBP) branch is the same product split into cache blocks; its
+ * idx_kn_base/idx_mn_base offset defect (owl_ndarray_conv_impl.h:589-590,
+ * :854-855) is not reproduced.
+ * ---------------------------------------------------------------------- */
+typedef void (*cblas_sgemm_fn)(int order, int ta, int tb, int m, int n, int k,
+                               float alpha, const float *a, int lda,
+                               const float *b, int ldb, float beta, float *c,
+                               int ldc);
+enum { CblasRowMajor_ = 101, CblasNoTrans_ = 111, CblasTrans_ = 112 };
+
+static void im2col_f32(const oracle_conv_geom *g, const float *x, float *col) {
+  const int64_t K = g->kh * g->kw * g->kd * g->Ci;
+  const int64_t M = g->B * g->Ho * g->Wo * g->Do;
+  for (int64_t m = 0; m < M; ++m) {
+    int64_t l = m % g->Do, t = m / g->Do;
+    int64_t j = t % g->Wo;
+    t /= g->Wo;
+    int64_t i = t % g->Ho, b = t / g->Ho;
+    float *cp = col + m * K;
+    int64_t cnt = 0;
+    for (int64_t a = 0; a < g->kh; ++a) {
+      int64_t h = i * g->sh - g->ph + a * g->rh;
+      for (int64_t c = 0; c < g->kw; ++c) {
+        int64_t ww = j * g->sw - g->pw + c * g->rw;
+        for (int64_t e = 0; e < g->kd; ++e) {
+          int64_t d = l * g->sd - g->pd + e * g->rd;
+          int inb = h >= 0 && h < g->H && ww >= 0 && ww < g->W && d >= 0 && d < g->D;
+          if (inb) {
+            const float *xp = x + IN_IDX(g, b, h, ww, d, 0);
+            for (int64_t q = 0; q < g->Ci; ++q) cp[cnt + q] = xp[q];
+          } else {
+            for (int64_t q = 0; q < g->Ci; ++q) cp[cnt + q] = 0.0f;
+          }
+          cnt += g->Ci;
+        }
+      }
+    }
+  }
+}
+
+int oracle_conv_fwd_owl_f32(const oracle_conv_geom *g, const float *x,
+                            const float *w, float *y, cblas_sgemm_fn gemm) {
+  const int64_t K = g->kh * g->kw * g->kd * g->Ci;
+  const int64_t M = g->B * g->Ho * g->Wo * g->Do;
+  float *col = (float *)malloc((size_t)(M * K) * sizeof(float));
+  if (!col) return -1;
+  im2col_f32(g, x, col);
+  gemm(CblasRowMajor_, CblasNoTrans_, CblasNoTrans_, (int)M, (int)g->Co, (int)K,
+       1.0f, col, (int)K, w, (int)g->Co, 0.0f, y, (int)g->Co);
+  free(col);
+  return 0;
+}
+
+int oracle_conv_bwd_input_owl_f32(const oracle_conv_geom *g, const float *w,
+                                  const float *dy, float *dx, cblas_sgemm_fn gemm) {
+  const int64_t K = g->kh * g->kw * g->kd * g->Ci;
+  const int64_t M = g->B * g->Ho * g->Wo * g->Do;
+  float *col = (float *)malloc((size_t)(M * K) * sizeof(float));
+  if (!col) return -1;
+  gemm(CblasRowMajor_, CblasNoTrans_, CblasTrans_, (int)M, (int)K, (int)g->Co,
+       1.0f, dy, (int)g->Co, w, (int)g->Co, 0.0f, col, (int)K);
+  memset(dx, 0, (size_t)(g->B * g->H * g->W * g->D * g->Ci) * sizeof(float));
+  for (int64_t m = 0; m < M; ++m) {
+    int64_t l = m % g->Do, t = m / g->Do;
+    int64_t j = t % g->Wo;
+    t /= g->Wo;
+    int64_t i = t % g->Ho, b = t / g->Ho;
+    const float *cp = col + m * K;
+    int64_t cnt = 0;
+    for (int64_t a = 0; a < g->kh; ++a) {
+      int64_t h = i * g->sh - g->ph + a * g->rh;
+      for (int64_t c = 0; c < g->kw; ++c) {
+        int64_t ww = j * g->sw - g->pw + c * g->rw;
+        for (int64_t e = 0; e < g->kd; ++e) {
+          int64_t d = l * g->sd - g->pd + e * g->rd;
+          if (h >= 0 && h < g->H && ww >= 0 && ww < g->W && d >= 0 && d < g->D) {
+            float *xp = dx + IN_IDX(g, b, h, ww, d, 0);
+            for (int64_t q = 0; q < g->Ci; ++q) xp[q] += cp[cnt + q];
+          }
+          cnt += g->Ci;
+        }
+      }
+    }
+  }
+  free(col);
+  return 0;
+}
+
+int oracle_conv_bwd_kernel_owl_f32(const oracle_conv_geom *g, const float *x,
+                                   const float *dy, float *dw, cblas_sgemm_fn gemm) {
+  const int64_t K = g->kh * g->kw * g->kd * g->Ci;
+  const int64_t M = g->B * g->Ho * g->Wo * g->Do;
+  float *col = (float *)malloc((size_t)(M * K) * sizeof(float));
+  float *k2 = (float *)malloc((size_t)(K * g->Co) * sizeof(float));
+  if (!col || !k2) { free(col); free(k2); return -1; }
+  im2col_f32(g, x, col);
+  gemm(CblasRowMajor_, CblasTrans_, CblasNoTrans_, (int)g->Co, (int)K, (int)M,
+       1.0f, dy, (int)g->Co, col, (int)K, 0.0f, k2, (int)K);
+  int64_t cnt = 0;
+  for (int64_t kk = 0; kk < K; ++kk)
+    for (int64_t o = 0; o < g->Co; ++o) dw[cnt++] = k2[o * K + kk];
+  free(col);
+  free(k2);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * Pooling, bit-exact restatement of owl_ndarray_pool_impl.h with the macro
+ * semantics of owl_ndarray_pool_stub.c:16-51:
+ *   max: acc = -INF; if (acc < t) acc = t          (first strict max wins,
+ *        NaN never wins)
+ *   avg: acc = 0; acc += t; result acc / count_in_bounds
+ * Window scan order: H ("cols") outer, W ("rows"), D ("dpts") inner, as in
+ * :66-77 and :285-300.  A 2-D call is the 3-D one with D = kd = 1.
+ * ---------------------------------------------------------------------- */
+#define PIN(g, b, h, w, d, c) \
+  ((((((b) * (g)->H + (h)) * (g)->W + (w)) * (g)->D + (d)) * (g)->C) + (c))
+#define POUT(g, b, h, w, d, c) \
+  ((((((b) * (g)->Ho + (h)) * (g)->Wo + (w)) * (g)->Do + (d)) * (g)->C) + (c))
+
+#define DEFINE_POOL(SUF, T)                                                      \
+  void oracle_pool_fwd_##SUF(const oracle_pool_geom *g, int is_max,             \
+                             const T *x, T *y) {                                 \
+    for (int64_t b = 0; b < g->B; ++b)                                           \
+      for (int64_t i = 0; i < g->Ho; ++i)                                        \
+        for (int64_t j = 0; j < g->Wo; ++j)                                      \
+          for (int64_t l = 0; l < g->Do; ++l)                                    \
+            for (int64_t ch = 0; ch < g->C; ++ch) {                              \
+              T acc = is_max ? (T)(-INFINITY) : (T)0;                            \
+              int cnt = 0;                                                       \
+              for (int64_t h = i * g->sh - g->ph; h < i * g->sh - g->ph + g->kh; ++h) \
+                for (int64_t w = j * g->sw - g->pw; w < j * g->sw - g->pw + g->kw; ++w) \
+                  for (int64_t d = l * g->sd - g->pd; d < l * g->sd - g->pd + g->kd; ++d) \
+                    if (h >= 0 && h < g->H && w >= 0 && w < g->W && d >= 0 && d < g->D) { \
+                      T t = x[PIN(g, b, h, w, d, ch)];                           \
+                      if (is_max) { if (acc < t) acc = t; }                      \
+                      else acc += t;                                             \
+                      cnt++;                                                     \
+                    }                                                            \
+              y[POUT(g, b, i, j, l, ch)] = is_max ? acc : acc / (T)cnt;          \
+            }                                                                    \
+  }                                                                              \
+  /* owl_ndarray_pool_impl.h:459-544: value + flat int64 argmax, -1 if none */   \
+  void oracle_pool_argmax_##SUF(const oracle_pool_geom *g, const T *x, T *y,    \
+                                int64_t *arg) {                                  \
+    for (int64_t b = 0; b < g->B; ++b)                                           \
+      for (int64_t i = 0; i < g->Ho; ++i)                                        \
+        for (int64_t j = 0; j < g->Wo; ++j)                                      \
+          for (int64_t l = 0; l < g->Do; ++l)                                    \
+            for (int64_t ch = 0; ch < g->C; ++ch) {                              \
+              T acc = (T)(-INFINITY);                                            \
+              int64_t mi = -1;                                                   \
+              for (int64_t h = i * g->sh - g->ph; h < i * g->sh - g->ph + g->kh; ++h) \
+                for (int64_t w = j * g->sw - g->pw; w < j * g->sw - g->pw + g->kw; ++w) \
+                  for (int64_t d = l * g->sd - g->pd; d < l * g->sd - g->pd + g->kd; ++d) \
+                    if (h >= 0 && h < g->H && w >= 0 && w < g->W && d >= 0 && d < g->D) { \
+                      int64_t id = PIN(g, b, h, w, d, ch);                       \
+                      T t = x[id];                                               \
+                      if (acc < t) { acc = t; mi = id; }                         \
+                    }                                                            \
+              y[POUT(g, b, i, j, l, ch)] = acc;                                  \
+              arg[POUT(g, b, i, j, l, ch)] = mi;                                 \
+            }                                                                    \
+  }                                                                              \
+  /* owl_ndarray_pool_impl.h:102-202 and :324-447: dx zeroed, then for each   \
+   * window in (b,i,j,l,ch) order: max adds dy to the recomputed first strict  \
+   * max (flat index 0 if none), avg adds dy/count to every in-bounds cell. */  \
+  void oracle_pool_bwd_##SUF(const oracle_pool_geom *g, int is_max,             \
+                             const T *x, const T *dy, T *dx) {                   \
+    memset(dx, 0, (size_t)(g->B * g->H * g->W * g->D * g->C) * sizeof(T));       \
+    for (int64_t b = 0; b < g->B; ++b)                                           \
+      for (int64_t i = 0; i < g->Ho; ++i)                                        \
+        for (int64_t j = 0; j < g->Wo; ++j)                                      \
+          for (int64_t l = 0; l < g->Do; ++l)                                    \
+            for (int64_t ch = 0; ch < g->C; ++ch) {                              \
+              T m = dy[POUT(g, b, i, j, l, ch)];                                 \
+              T acc = (T)(-INFINITY);                                            \
+              int64_t mi = 0;                                                    \
+              int cnt = 0;                                                       \
+              for (int64_t h = i * g->sh - g->ph; h < i * g->sh - g->ph + g->kh; ++h) \
+                for (int64_t w = j * g->sw - g->pw; w < j * g->sw - g->pw + g->kw; ++w) \
+                  for (int64_t d = l * g->sd - g->pd; d < l * g->sd - g->pd + g->kd; ++d) \
+                    if (h >= 0 && h < g->H && w >= 0 && w < g->W && d >= 0 && d < g->D) { \
+                      int64_t id = PIN(g, b, h, w, d, ch);                       \
+                      if (is_max) { T t = x[id]; if (acc < t) { acc = t; mi = id; } } \
+                      cnt++;                                                     \
+                    }                                                            \
+              if (is_max) { dx[mi] += m; continue; }                             \
+              T v = m / (T)cnt;                                                  \
+              for (int64_t h = i * g->sh - g->ph; h < i * g->sh - g->ph + g->kh; ++h) \
+                for (int64_t w = j * g->sw - g->pw; w < j * g->sw - g->pw + g->kw; ++w) \
+                  for (int64_t d = l * g->sd - g->pd; d < l * g->sd - g->pd + g->kd; ++d) \
+                    if (h >= 0 && h < g->H && w >= 0 && w < g->W && d >= 0 && d < g->D) \
+                      dx[PIN(g, b, h, w, d, ch)] += v;                           \
+            }                                                                    \
+  }
+
+DEFINE_POOL(f32, float)
+DEFINE_POOL(f64, double)

@@ -1,0 +1,810 @@
+// C ABI of libowl_amd: argument checking, GEMM-view set-up, dtype dispatch,
+// split-K planning, and the Owl drop-in entry points that own device buffers.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "conv_kernels.h"
+#include "owl_amd_common.h"
+#include "pool_kernels.h"
+
+using namespace owl_amd;
+
+// ------------------------------------------------------------------ errors
+static thread_local char g_err[512] = "";
+
+extern "C" void owl_amd_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char *owl_amd_last_error(void) { return g_err; }
+
+#define ARG_FAIL(...)                \
+  do {                               \
+    owl_amd_set_error(__VA_ARGS__);  \
+    return OWL_AMD_ERR_ARG;          \
+  } while (0)
+
+// ------------------------------------------------------------------ per-device state
+namespace {
+
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct DeviceState {
+  bool ready = false;
+  hipStream_t stream = nullptr;  // default priority, used by the drop-ins
+  int *flag = nullptr;           // pooling orphan flag (one int)
+  DevBuf ws;                     // split-K workspace
+  DevBuf slot[4];                // drop-in staging buffers
+};
+
+std::mutex g_mu;
+DeviceState g_dev[64];
+
+int current_device(int *dev) {
+  hipError_t e = hipGetDevice(dev);
+  if (e != hipSuccess) {
+    owl_amd_set_error("hipGetDevice: %s", hipGetErrorString(e));
+    return OWL_AMD_ERR_HIP;
+  }
+  if (*dev < 0 || *dev >= 64) ARG_FAIL("device index %d out of range", *dev);
+  return OWL_AMD_OK;
+}
+
+int ensure_state(DeviceState **out) {
+  int dev;
+  int rc = current_device(&dev);
+  if (rc) return rc;
+  DeviceState &s = g_dev[dev];
+  if (!s.ready) {
+    OWL_CHECK_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    OWL_CHECK_HIP(hipMalloc(&s.flag, 256));
+    OWL_CHECK_HIP(hipMemset(s.flag, 0, 256));
+    OWL_CHECK_HIP(hipDeviceSynchronize());
+    s.ready = true;
+  }
+  *out = &s;
+  return OWL_AMD_OK;
+}
+
+// Grow-only device buffer.  Growing waits for the stream that used it.
+int ensure_buf(DevBuf &b, size_t bytes, hipStream_t st) {
+  if (b.bytes >= bytes) return OWL_AMD_OK;
+  if (b.ptr) {
+    OWL_CHECK_HIP(hipStreamSynchronize(st));
+    OWL_CHECK_HIP(hipFree(b.ptr));
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes < 256 ? 256 : bytes;
+  hipError_t e = hipMalloc(&b.ptr, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    owl_amd_set_error("hipMalloc(%zu): %s", want, hipGetErrorString(e));
+    return OWL_AMD_ERR_NOMEM;
+  }
+  b.bytes = want;
+  return OWL_AMD_OK;
+}
+
+size_t dtype_size(int t) {
+  switch (t) {
+    case OWL_AMD_F32: return 4;
+    case OWL_AMD_F64: return 8;
+    case OWL_AMD_BF16:
+    case OWL_AMD_F16: return 2;
+    case OWL_AMD_FP8E4M3: return 1;
+  }
+  return 0;
+}
+
+bool fits_int(int64_t v) { return v >= 0 && v <= 0x7fffffffLL; }
+
+int check_conv(const owl_amd_conv_desc *d) {
+  if (!d) ARG_FAIL("null descriptor");
+  const int64_t v[] = {d->B, d->H, d->W, d->D, d->Ci, d->kh, d->kw, d->kd, d->Co,
+                       d->Ho, d->Wo, d->Do, d->ph, d->pw, d->pd};
+  for (int64_t x : v)
+    if (!fits_int(x)) ARG_FAIL("conv size out of range (negative or > INT32_MAX)");
+  if (d->sh < 1 || d->sw < 1 || d->sd < 1) ARG_FAIL("conv stride must be >= 1");
+  if (d->rh < 1 || d->rw < 1 || d->rd < 1) ARG_FAIL("conv dilation must be >= 1");
+  if (d->kh < 1 || d->kw < 1 || d->kd < 1) ARG_FAIL("conv kernel extent must be >= 1");
+  if (!fits_int(d->B * d->H * d->W * d->D) || !fits_int(d->B * d->Ho * d->Wo * d->Do) ||
+      !fits_int(d->kh * d->kw * d->kd * d->Ci * d->Co))
+    ARG_FAIL("conv tensor too large for 32-bit pixel indexing");
+  return OWL_AMD_OK;
+}
+
+ConvParams make_params(const owl_amd_conv_desc *d) {
+  ConvParams p;
+  memset(&p, 0, sizeof(p));
+  p.B = (int)d->B; p.H = (int)d->H; p.W = (int)d->W; p.D = (int)d->D; p.Ci = (int)d->Ci;
+  p.kh = (int)d->kh; p.kw = (int)d->kw; p.kd = (int)d->kd; p.Co = (int)d->Co;
+  p.Ho = (int)d->Ho; p.Wo = (int)d->Wo; p.Do = (int)d->Do;
+  p.sh = (int)d->sh; p.sw = (int)d->sw; p.sd = (int)d->sd;
+  p.rh = (int)d->rh; p.rw = (int)d->rw; p.rd = (int)d->rd;
+  p.ph = (int)d->ph; p.pw = (int)d->pw; p.pd = (int)d->pd;
+  p.split = 1;
+  return p;
+}
+
+bool aligned16(const void *a, const void *b, const void *c) {
+  return (((uintptr_t)a | (uintptr_t)b | (uintptr_t)c) & 15) == 0;
+}
+
+template <int PASS>
+int dispatch_igemm(const ConvParams &p, int in_t, int out_t, bool aligned, hipStream_t st) {
+  if (in_t == OWL_AMD_F32 && out_t == OWL_AMD_F32) run_igemm<PASS, float, float>(p, aligned, st);
+  else if (in_t == OWL_AMD_F64 && out_t == OWL_AMD_F64) run_igemm<PASS, double, double>(p, aligned, st);
+  else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) run_igemm<PASS, bf16_t, bf16_t>(p, aligned, st);
+  else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_F32) run_igemm<PASS, bf16_t, float>(p, aligned, st);
+  else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F16) run_igemm<PASS, f16_t, f16_t>(p, aligned, st);
+  else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F32) run_igemm<PASS, f16_t, float>(p, aligned, st);
+  else {
+    owl_amd_set_error("unsupported dtype pair in=%d out=%d", in_t, out_t);
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    owl_amd_set_error("kernel launch: %s", hipGetErrorString(e));
+    return OWL_AMD_ERR_HIP;
+  }
+  return OWL_AMD_OK;
+}
+
+int vec_of(int t) { return 16 / (int)dtype_size(t); }
+
+int tile_k(int t) {
+  switch (t) {
+    case OWL_AMD_F32: return tile_bk<float>();
+    case OWL_AMD_F64: return tile_bk<double>();
+    case OWL_AMD_BF16: return tile_bk<bf16_t>();
+    default: return tile_bk<f16_t>();
+  }
+}
+int tile_m(int t) {
+  switch (t) {
+    case OWL_AMD_F32: return tile_bm<float>();
+    case OWL_AMD_F64: return tile_bm<double>();
+    case OWL_AMD_BF16: return tile_bm<bf16_t>();
+    default: return tile_bm<f16_t>();
+  }
+}
+
+// Split-K plan for the backward-kernel GEMM (M = T*Ci, N = Co, K = pixels):
+// enough workgroups to fill 256 CUs several times over, each split a
+// multiple of the K-step.
+void plan_splitk(const ConvParams &p, int in_t, int *split, int64_t *kps) {
+  const int bm = tile_m(in_t), bk = tile_k(in_t);
+  int64_t tiles = ((p.M + bm - 1) / bm) * (int64_t)((p.N + bm - 1) / bm);
+  int64_t want = (1024 + tiles - 1) / tiles;
+  int64_t max_split = (p.K + 4 * bk - 1) / (4 * bk);  // at least 4 K-steps per split
+  if (want > max_split) want = max_split;
+  if (want < 1) want = 1;
+  int64_t k = (p.K + want - 1) / want;
+  k = (k + bk - 1) / bk * bk;
+  *kps = k;
+  *split = (int)((p.K + k - 1) / k);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ init
+extern "C" int owl_amd_init(int device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  OWL_CHECK_HIP(hipSetDevice(device));
+  DeviceState *s;
+  return ensure_state(&s);
+}
+
+extern "C" int owl_amd_complex_supported(void) { return 0; }
+
+extern "C" int owl_amd_cache_stats(int64_t *n, int64_t *bytes) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int64_t cnt = 0, tot = 0;
+  for (auto &s : g_dev) {
+    if (!s.ready) continue;
+    for (auto &b : s.slot)
+      if (b.ptr) { cnt++; tot += (int64_t)b.bytes; }
+    if (s.ws.ptr) { cnt++; tot += (int64_t)s.ws.bytes; }
+  }
+  if (n) *n = cnt;
+  if (bytes) *bytes = tot;
+  return OWL_AMD_OK;
+}
+
+extern "C" void owl_amd_cache_release(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto &s : g_dev) {
+    if (!s.ready) continue;
+    (void)hipStreamSynchronize(s.stream);
+    (void)hipDeviceSynchronize();
+    for (auto &b : s.slot) {
+      if (b.ptr) (void)hipFree(b.ptr);
+      b = DevBuf();
+    }
+    if (s.ws.ptr) (void)hipFree(s.ws.ptr);
+    s.ws = DevBuf();
+  }
+}
+
+// ------------------------------------------------------------------ device API: conv
+extern "C" int owl_amd_conv_fwd(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x,
+                                const void *w, void *y, void *stream) {
+  int rc = check_conv(d);
+  if (rc) return rc;
+  ConvParams p = make_params(d);
+  p.M = d->B * d->Ho * d->Wo * d->Do;
+  p.N = p.Co;
+  p.K = d->kh * d->kw * d->kd * d->Ci;
+  if (p.M == 0 || p.N == 0) return OWL_AMD_OK;
+  if (!x || !w || !y) ARG_FAIL("null pointer");
+  p.a = x; p.b = w; p.c = y; p.ldc = p.Co;
+  const int v = vec_of(in_t);
+  bool aligned = dtype_size(in_t) && p.Ci % v == 0 && p.Co % v == 0 && aligned16(x, w, nullptr);
+  return dispatch_igemm<0>(p, in_t, out_t, aligned, (hipStream_t)stream);
+}
+
+extern "C" int owl_amd_conv_bwd_input(const owl_amd_conv_desc *d, int in_t, int out_t,
+                                      const void *w, const void *dy, void *dx, void *stream) {
+  int rc = check_conv(d);
+  if (rc) return rc;
+  ConvParams p = make_params(d);
+  p.M = d->B * d->H * d->W * d->D;
+  p.N = p.Ci;
+  p.K = d->kh * d->kw * d->kd * d->Co;
+  if (p.M == 0 || p.N == 0) return OWL_AMD_OK;
+  if (!w || !dy || !dx) ARG_FAIL("null pointer");
+  p.a = dy; p.b = w; p.c = dx; p.ldc = p.Ci;
+  const int v = vec_of(in_t);
+  bool aligned = dtype_size(in_t) && p.Co % v == 0 && aligned16(dy, w, nullptr);
+  return dispatch_igemm<1>(p, in_t, out_t, aligned, (hipStream_t)stream);
+}
+
+static void bwd_kernel_params(const owl_amd_conv_desc *d, ConvParams &p) {
+  p = make_params(d);
+  p.M = d->kh * d->kw * d->kd * d->Ci;
+  p.N = p.Co;
+  p.K = d->B * d->Ho * d->Wo * d->Do;
+}
+
+extern "C" size_t owl_amd_conv_bwd_kernel_workspace(const owl_amd_conv_desc *d, int in_t) {
+  if (check_conv(d)) return 0;
+  ConvParams p;
+  bwd_kernel_params(d, p);
+  if (p.M == 0 || p.N == 0 || p.K == 0) return 0;
+  int split;
+  int64_t kps;
+  plan_splitk(p, in_t, &split, &kps);
+  if (split <= 1) return 0;
+  size_t acc = in_t == OWL_AMD_F64 ? 8 : 4;
+  return (size_t)split * (size_t)p.M * (size_t)p.N * acc;
+}
+
+extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int out_t,
+                                       const void *x, const void *dy, void *dw, void *workspace,
+                                       size_t ws_bytes, void *stream) {
+  int rc = check_conv(d);
+  if (rc) return rc;
+  ConvParams p;
+  bwd_kernel_params(d, p);
+  hipStream_t st = (hipStream_t)stream;
+  if (p.M == 0 || p.N == 0) return OWL_AMD_OK;
+  if (!x || !dy || !dw) ARG_FAIL("null pointer");
+  if (p.K == 0) {
+    OWL_CHECK_HIP(hipMemsetAsync(dw, 0, (size_t)p.M * p.N * dtype_size(out_t), st));
+    return OWL_AMD_OK;
+  }
+  plan_splitk(p, in_t, &p.split, &p.k_per_split);
+  p.a = x; p.b = dy; p.ldc = p.Co;
+  const int v = vec_of(in_t);
+  bool aligned = dtype_size(in_t) && p.Ci % v == 0 && p.Co % v == 0 && aligned16(x, dy, nullptr);
+  if (p.split <= 1) {
+    p.c = dw;
+    return dispatch_igemm<2>(p, in_t, out_t, aligned, st);
+  }
+  size_t need = owl_amd_conv_bwd_kernel_workspace(d, in_t);
+  void *ws = workspace;
+  if (!ws) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState *s;
+    rc = ensure_state(&s);
+    if (rc) return rc;
+    rc = ensure_buf(s->ws, need, st);
+    if (rc) return rc;
+    ws = s->ws.ptr;
+  } else if (ws_bytes < need) {
+    ARG_FAIL("workspace too small: %zu < %zu bytes", ws_bytes, need);
+  }
+  p.c = ws;
+  rc = dispatch_igemm<2>(p, in_t, out_t, aligned, st);
+  if (rc) return rc;
+  const int64_t n = p.M * p.N;
+  if (in_t == OWL_AMD_F64) run_splitk_reduce<double, double>((const double *)ws, (double *)dw, n, p.split, st);
+  else if (out_t == OWL_AMD_F32) run_splitk_reduce<float, float>((const float *)ws, (float *)dw, n, p.split, st);
+  else if (out_t == OWL_AMD_BF16) run_splitk_reduce<float, bf16_t>((const float *)ws, (bf16_t *)dw, n, p.split, st);
+  else run_splitk_reduce<float, f16_t>((const float *)ws, (f16_t *)dw, n, p.split, st);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+// ------------------------------------------------------------------ device API: pool
+static int check_pool(const owl_amd_pool_desc *d, int t) {
+  if (!d) ARG_FAIL("null descriptor");
+  const int64_t v[] = {d->B, d->H, d->W, d->D, d->C, d->Ho, d->Wo, d->Do, d->ph, d->pw, d->pd};
+  for (int64_t x : v)
+    if (!fits_int(x)) ARG_FAIL("pool size out of range");
+  if (d->kh < 1 || d->kw < 1 || d->kd < 1) ARG_FAIL("pool window must be >= 1");
+  if (d->sh < 1 || d->sw < 1 || d->sd < 1) ARG_FAIL("pool stride must be >= 1");
+  if (!fits_int(d->B * d->H * d->W * d->D * d->C) || !fits_int(d->B * d->Ho * d->Wo * d->Do * d->C))
+    ARG_FAIL("pool tensor too large for 32-bit indexing");
+  if (t != OWL_AMD_F32 && t != OWL_AMD_F64) {
+    owl_amd_set_error("pooling supports F32 and F64");
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  return OWL_AMD_OK;
+}
+
+static PoolParams make_pool(const owl_amd_pool_desc *d) {
+  PoolParams p;
+  p.B = (int)d->B; p.H = (int)d->H; p.W = (int)d->W; p.D = (int)d->D; p.C = (int)d->C;
+  p.kh = (int)d->kh; p.kw = (int)d->kw; p.kd = (int)d->kd;
+  p.Ho = (int)d->Ho; p.Wo = (int)d->Wo; p.Do = (int)d->Do;
+  p.sh = (int)d->sh; p.sw = (int)d->sw; p.sd = (int)d->sd;
+  p.ph = (int)d->ph; p.pw = (int)d->pw; p.pd = (int)d->pd;
+  return p;
+}
+
+extern "C" int owl_amd_pool_fwd(const owl_amd_pool_desc *d, int t, int is_max, const void *x,
+                                void *y, void *stream) {
+  int rc = check_pool(d, t);
+  if (rc) return rc;
+  PoolParams p = make_pool(d);
+  if ((int64_t)p.B * p.Ho * p.Wo * p.Do * p.C == 0) return OWL_AMD_OK;
+  if (!x || !y) ARG_FAIL("null pointer");
+  if (t == OWL_AMD_F32) run_pool_fwd<float>(p, is_max ? 1 : 0, (const float *)x, (float *)y, nullptr, (hipStream_t)stream);
+  else run_pool_fwd<double>(p, is_max ? 1 : 0, (const double *)x, (double *)y, nullptr, (hipStream_t)stream);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+extern "C" int owl_amd_pool_argmax(const owl_amd_pool_desc *d, int t, const void *x, void *y,
+                                   int64_t *argmax, void *stream) {
+  int rc = check_pool(d, t);
+  if (rc) return rc;
+  PoolParams p = make_pool(d);
+  if ((int64_t)p.B * p.Ho * p.Wo * p.Do * p.C == 0) return OWL_AMD_OK;
+  if (!x || !y || !argmax) ARG_FAIL("null pointer");
+  if (t == OWL_AMD_F32) run_pool_fwd<float>(p, 2, (const float *)x, (float *)y, argmax, (hipStream_t)stream);
+  else run_pool_fwd<double>(p, 2, (const double *)x, (double *)y, argmax, (hipStream_t)stream);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+// True if some window along one axis has no in-bounds cell.
+static bool axis_has_empty(int64_t n, int64_t k, int64_t s, int64_t p, int64_t o) {
+  if (o <= 0) return false;
+  return (k - p <= 0) || ((o - 1) * s - p >= n);
+}
+
+extern "C" int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max, const void *x,
+                                const void *dy, void *dx, void *stream) {
+  int rc = check_pool(d, t);
+  if (rc) return rc;
+  PoolParams p = make_pool(d);
+  hipStream_t st = (hipStream_t)stream;
+  if ((int64_t)p.B * p.H * p.W * p.D * p.C == 0) return OWL_AMD_OK;
+  if (!x || !dy || !dx) ARG_FAIL("null pointer");
+  int *flag = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState *s;
+    rc = ensure_state(&s);
+    if (rc) return rc;
+    flag = s->flag;
+  }
+  bool empty = axis_has_empty(d->H, d->kh, d->sh, d->ph, d->Ho) ||
+               axis_has_empty(d->W, d->kw, d->sw, d->pw, d->Wo) ||
+               axis_has_empty(d->D, d->kd, d->sd, d->pd, d->Do);
+  if (is_max && empty && (int64_t)p.Ho * p.Wo * p.Do * p.B * p.C > 0) run_set_flag(flag, st);
+  if (t == OWL_AMD_F32) run_pool_bwd<float>(p, is_max != 0, (const float *)x, (const float *)dy, (float *)dx, flag, st);
+  else run_pool_bwd<double>(p, is_max != 0, (const double *)x, (const double *)dy, (double *)dx, flag, st);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+// ------------------------------------------------------------------ Owl drop-ins
+namespace {
+
+// Leading pad exactly as Owl's C derives it:  p = (s*(o-1) + k_eff - n) / 2
+// (C truncation), clamped at 0  (owl_ndarray_conv_impl.h:288-294, 502-505).
+int64_t owl_pad(int64_t s, int64_t o, int64_t k_eff, int64_t n) {
+  int64_t t = s * (o - 1) + k_eff - n;
+  int64_t q = t / 2;  // C++ truncates toward zero like C
+  return q < 0 ? 0 : q;
+}
+
+struct HostArg {
+  const void *src;  // host data to upload (nullptr: output only)
+  size_t bytes;
+};
+
+// Runs one drop-in call: uploads inputs into the device's staging slots,
+// enqueues `launch`, downloads `out_slot` into `out_host` and waits.
+template <typename F>
+int run_dropin(std::initializer_list<HostArg> args, int out_slot, void *out_host, size_t out_bytes,
+               F launch, int extra_out_slot = -1, void *extra_host = nullptr, size_t extra_bytes = 0) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  DeviceState *s;
+  int rc = ensure_state(&s);
+  if (rc) return rc;
+  void *dev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int i = 0;
+  for (const HostArg &a : args) {
+    rc = ensure_buf(s->slot[i], a.bytes, s->stream);
+    if (rc) return rc;
+    dev[i] = s->slot[i].ptr;
+    if (a.src && a.bytes)
+      OWL_CHECK_HIP(hipMemcpyAsync(dev[i], a.src, a.bytes, hipMemcpyHostToDevice, s->stream));
+    ++i;
+  }
+  rc = launch(dev, s->stream);
+  if (rc) {
+    (void)hipStreamSynchronize(s->stream);
+    return rc;
+  }
+  if (out_bytes)
+    OWL_CHECK_HIP(hipMemcpyAsync(out_host, dev[out_slot], out_bytes, hipMemcpyDeviceToHost, s->stream));
+  if (extra_out_slot >= 0 && extra_bytes)
+    OWL_CHECK_HIP(hipMemcpyAsync(extra_host, dev[extra_out_slot], extra_bytes, hipMemcpyDeviceToHost, s->stream));
+  OWL_CHECK_HIP(hipStreamSynchronize(s->stream));
+  return OWL_AMD_OK;
+}
+
+template <typename T> constexpr int dt();
+template <> constexpr int dt<float>() { return OWL_AMD_F32; }
+template <> constexpr int dt<double>() { return OWL_AMD_F64; }
+
+enum ConvPass { FWD = 0, BWD_IN = 1, BWD_KER = 2 };
+
+// Common body of every conv drop-in.  `padding` < 0 means "backward: infer
+// the pad from the shapes"; 0 = SAME, 1 = VALID (forward).
+template <typename T>
+int conv_dropin(ConvPass pass, T *input, T *kernel, T *output, int64_t B, int64_t H, int64_t W,
+                int64_t D, int64_t Ci, int64_t kh, int64_t kw, int64_t kd, int64_t Ho, int64_t Wo,
+                int64_t Do, int64_t Co, int64_t sh, int64_t sw, int64_t sd, int64_t rh, int64_t rw,
+                int64_t rd, int64_t padding) {
+  owl_amd_conv_desc d;
+  d.B = B; d.H = H; d.W = W; d.D = D; d.Ci = Ci;
+  d.kh = kh; d.kw = kw; d.kd = kd; d.Co = Co;
+  d.Ho = Ho; d.Wo = Wo; d.Do = Do;
+  d.sh = sh; d.sw = sw; d.sd = sd;
+  d.rh = rh; d.rw = rw; d.rd = rd;
+  if (padding == 1) {
+    d.ph = d.pw = d.pd = 0;
+  } else {
+    d.ph = owl_pad(sh, Ho, kh + (kh - 1) * (rh - 1), H);
+    d.pw = owl_pad(sw, Wo, kw + (kw - 1) * (rw - 1), W);
+    d.pd = owl_pad(sd, Do, kd + (kd - 1) * (rd - 1), D);
+  }
+  int rc = check_conv(&d);
+  if (rc) return rc;
+  if (!input || !kernel || !output) ARG_FAIL("null pointer");
+  const size_t nx = (size_t)(B * H * W * D * Ci) * sizeof(T);
+  const size_t nw = (size_t)(kh * kw * kd * Ci * Co) * sizeof(T);
+  const size_t ny = (size_t)(B * Ho * Wo * Do * Co) * sizeof(T);
+  const int t = dt<T>();
+  if (pass == FWD) {
+    // Owl memsets the output first (owl_ndarray_conv_impl.h:285): an empty
+    // GEMM (K == 0 cannot occur, M == 0 writes nothing) leaves zeros.
+    return run_dropin({{input, nx}, {kernel, nw}, {nullptr, ny}}, 2, output, ny,
+                      [&](void **dv, hipStream_t st) {
+                        return owl_amd_conv_fwd(&d, t, t, dv[0], dv[1], dv[2], st);
+                      });
+  } else if (pass == BWD_IN) {
+    return run_dropin({{kernel, nw}, {output, ny}, {nullptr, nx}}, 2, input, nx,
+                      [&](void **dv, hipStream_t st) {
+                        return owl_amd_conv_bwd_input(&d, t, t, dv[0], dv[1], dv[2], st);
+                      });
+  } else {
+    return run_dropin({{input, nx}, {output, ny}, {nullptr, nw}}, 2, kernel, nw,
+                      [&](void **dv, hipStream_t st) {
+                        return owl_amd_conv_bwd_kernel(&d, t, t, dv[0], dv[1], dv[2], nullptr, 0, st);
+                      });
+  }
+}
+
+template <typename T>
+int pool_dropin_fwd(bool is_max, T *input, T *output, int64_t B, int64_t H, int64_t W, int64_t D,
+                    int64_t C, int64_t kh, int64_t kw, int64_t kd, int64_t Ho, int64_t Wo,
+                    int64_t Do, int64_t sh, int64_t sw, int64_t sd, int64_t padding) {
+  owl_amd_pool_desc d;
+  d.B = B; d.H = H; d.W = W; d.D = D; d.C = C;
+  d.kh = kh; d.kw = kw; d.kd = kd;
+  d.Ho = Ho; d.Wo = Wo; d.Do = Do;
+  d.sh = sh; d.sw = sw; d.sd = sd;
+  if (padding == 1) {
+    d.ph = d.pw = d.pd = 0;
+  } else {  // owl_ndarray_pool_impl.h:42-48, :263-271
+    d.ph = owl_pad(sh, Ho, kh, H);
+    d.pw = owl_pad(sw, Wo, kw, W);
+    d.pd = owl_pad(sd, Do, kd, D);
+  }
+  int rc = check_pool(&d, dt<T>());
+  if (rc) return rc;
+  if (!input || !output) ARG_FAIL("null pointer");
+  const size_t nx = (size_t)(B * H * W * D * C) * sizeof(T);
+  const size_t ny = (size_t)(B * Ho * Wo * Do * C) * sizeof(T);
+  return run_dropin({{input, nx}, {nullptr, ny}}, 1, output, ny, [&](void **dv, hipStream_t st) {
+    return owl_amd_pool_fwd(&d, dt<T>(), is_max, dv[0], dv[1], st);
+  });
+}
+
+template <typename T>
+int pool_dropin_bwd(bool is_max, T *input, T *output_back, T *input_back, int64_t B, int64_t H,
+                    int64_t W, int64_t D, int64_t C, int64_t kh, int64_t kw, int64_t kd,
+                    int64_t Ho, int64_t Wo, int64_t Do, int64_t sh, int64_t sw, int64_t sd,
+                    int64_t ph, int64_t pw, int64_t pd) {
+  owl_amd_pool_desc d;
+  d.B = B; d.H = H; d.W = W; d.D = D; d.C = C;
+  d.kh = kh; d.kw = kw; d.kd = kd;
+  d.Ho = Ho; d.Wo = Wo; d.Do = Do;
+  d.sh = sh; d.sw = sw; d.sd = sd;
+  d.ph = ph < 0 ? 0 : ph;  // owl_ndarray_pool_impl.h:135-136
+  d.pw = pw < 0 ? 0 : pw;
+  d.pd = pd < 0 ? 0 : pd;
+  int rc = check_pool(&d, dt<T>());
+  if (rc) return rc;
+  if (!input || !output_back || !input_back) ARG_FAIL("null pointer");
+  const size_t nx = (size_t)(B * H * W * D * C) * sizeof(T);
+  const size_t ny = (size_t)(B * Ho * Wo * Do * C) * sizeof(T);
+  // Owl's average backward is called with the output buffer as a dummy input
+  // (owl_ndarray_pool.ml:85-119); it is never read for the average.
+  return run_dropin({{is_max ? (const void *)input : nullptr, is_max ? nx : 16}, {output_back, ny}, {nullptr, nx}}, 2,
+                    input_back, nx, [&](void **dv, hipStream_t st) {
+                      return owl_amd_pool_bwd(&d, dt<T>(), is_max, dv[0], dv[1], dv[2], st);
+                    });
+}
+
+template <typename T>
+int pool_dropin_arg(T *input, T *output, int64_t *argmax, int64_t B, int64_t H, int64_t W,
+                    int64_t C, int64_t kh, int64_t kw, int64_t Ho, int64_t Wo, int64_t sh,
+                    int64_t sw, int64_t ph, int64_t pw) {
+  owl_amd_pool_desc d;
+  d.B = B; d.H = H; d.W = W; d.D = 1; d.C = C;
+  d.kh = kh; d.kw = kw; d.kd = 1;
+  d.Ho = Ho; d.Wo = Wo; d.Do = 1;
+  d.sh = sh; d.sw = sw; d.sd = 1;
+  d.ph = ph < 0 ? 0 : ph;  // owl_ndarray_pool_impl.h:491-492
+  d.pw = pw < 0 ? 0 : pw;
+  d.pd = 0;
+  int rc = check_pool(&d, dt<T>());
+  if (rc) return rc;
+  if (!input || !output || !argmax) ARG_FAIL("null pointer");
+  const size_t nx = (size_t)(B * H * W * C) * sizeof(T);
+  const size_t ny = (size_t)(B * Ho * Wo * C) * sizeof(T);
+  const size_t na = (size_t)(B * Ho * Wo * C) * sizeof(int64_t);
+  return run_dropin({{input, nx}, {nullptr, ny}, {nullptr, na}}, 1, output, ny,
+                    [&](void **dv, hipStream_t st) {
+                      return owl_amd_pool_argmax(&d, dt<T>(), dv[0], dv[1], (int64_t *)dv[2], st);
+                    },
+                    2, argmax, na);
+}
+
+}  // namespace
+
+// Owl passes (row = W, col = H) and strides as (row_stride, col_stride); the
+// 2-D drop-ins map them onto the H / W axes of the descriptor.
+#define DEFINE_DROPINS(N, T)                                                                      \
+  extern "C" int owl_amd_##N##_ndarray_conv_spatial(OWL_AMD_CONV2D_FWD_ARGS(T)) {                \
+    return conv_dropin<T>(FWD, input, kernel, output, batches, input_cols, input_rows, 1,        \
+                          in_channel, kernel_cols, kernel_rows, 1, output_cols, output_rows, 1,  \
+                          out_channel, col_stride, row_stride, 1, 1, 1, 1, padding);             \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_spatial_backward_input(OWL_AMD_CONV2D_BWD_ARGS(T)) { \
+    return conv_dropin<T>(BWD_IN, input, kernel, output, batches, input_cols, input_rows, 1,     \
+                          in_channel, kernel_cols, kernel_rows, 1, output_cols, output_rows, 1,  \
+                          out_channel, col_stride, row_stride, 1, 1, 1, 1, -1);                  \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_spatial_backward_kernel(OWL_AMD_CONV2D_BWD_ARGS(T)) { \
+    return conv_dropin<T>(BWD_KER, input, kernel, output, batches, input_cols, input_rows, 1,    \
+                          in_channel, kernel_cols, kernel_rows, 1, output_cols, output_rows, 1,  \
+                          out_channel, col_stride, row_stride, 1, 1, 1, 1, -1);                  \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_dilated_spatial_im2col(OWL_AMD_CONV2D_FWD_ARGS(T)) { \
+    return conv_dropin<T>(FWD, input, kernel, output, batches, input_cols, input_rows, 1,        \
+                          in_channel, kernel_cols, kernel_rows, 1, output_cols, output_rows, 1,  \
+                          out_channel, col_stride, row_stride, 1, col_in_stride, row_in_stride,  \
+                          1, padding);                                                           \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_dilated_spatial_backward_input_im2col(              \
+      OWL_AMD_CONV2D_BWD_ARGS(T)) {                                                              \
+    return conv_dropin<T>(BWD_IN, input, kernel, output, batches, input_cols, input_rows, 1,     \
+                          in_channel, kernel_cols, kernel_rows, 1, output_cols, output_rows, 1,  \
+                          out_channel, col_stride, row_stride, 1, col_in_stride, row_in_stride,  \
+                          1, -1);                                                                \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_dilated_spatial_backward_kernel_im2col(             \
+      OWL_AMD_CONV2D_BWD_ARGS(T)) {                                                              \
+    return conv_dropin<T>(BWD_KER, input, kernel, output, batches, input_cols, input_rows, 1,    \
+                          in_channel, kernel_cols, kernel_rows, 1, output_cols, output_rows, 1,  \
+                          out_channel, col_stride, row_stride, 1, col_in_stride, row_in_stride,  \
+                          1, -1);                                                                \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_cuboid_im2col(OWL_AMD_CONV3D_FWD_ARGS(T)) {          \
+    return conv_dropin<T>(FWD, input, kernel, output, batches, input_cols, input_rows,           \
+                          input_dpts, in_channel, kernel_cols, kernel_rows, kernel_dpts,         \
+                          output_cols, output_rows, output_dpts, out_channel, col_stride,        \
+                          row_stride, dpt_stride, 1, 1, 1, padding);                             \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_cuboid_backward_input_im2col(                       \
+      OWL_AMD_CONV3D_BWD_ARGS(T)) {                                                              \
+    return conv_dropin<T>(BWD_IN, input, kernel, output, batches, input_cols, input_rows,        \
+                          input_dpts, in_channel, kernel_cols, kernel_rows, kernel_dpts,         \
+                          output_cols, output_rows, output_dpts, out_channel, col_stride,        \
+                          row_stride, dpt_stride, 1, 1, 1, -1);                                  \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_cuboid_backward_kernel_im2col(                      \
+      OWL_AMD_CONV3D_BWD_ARGS(T)) {                                                              \
+    return conv_dropin<T>(BWD_KER, input, kernel, output, batches, input_cols, input_rows,       \
+                          input_dpts, in_channel, kernel_cols, kernel_rows, kernel_dpts,         \
+                          output_cols, output_rows, output_dpts, out_channel, col_stride,        \
+                          row_stride, dpt_stride, 1, 1, 1, -1);                                  \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_dilated_cuboid_im2col(OWL_AMD_DCONV3D_FWD_ARGS(T)) { \
+    return conv_dropin<T>(FWD, input, kernel, output, batches, input_cols, input_rows,           \
+                          input_dpts, in_channel, kernel_cols, kernel_rows, kernel_dpts,         \
+                          output_cols, output_rows, output_dpts, out_channel, col_stride,        \
+                          row_stride, dpt_stride, col_in_stride, row_in_stride, dpt_in_stride,   \
+                          padding);                                                              \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_dilated_cuboid_backward_input_im2col(               \
+      OWL_AMD_DCONV3D_BWD_ARGS(T)) {                                                             \
+    return conv_dropin<T>(BWD_IN, input, kernel, output, batches, input_cols, input_rows,        \
+                          input_dpts, in_channel, kernel_cols, kernel_rows, kernel_dpts,         \
+                          output_cols, output_rows, output_dpts, out_channel, col_stride,        \
+                          row_stride, dpt_stride, col_in_stride, row_in_stride, dpt_in_stride,   \
+                          -1);                                                                   \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_dilated_cuboid_backward_kernel_im2col(              \
+      OWL_AMD_DCONV3D_BWD_ARGS(T)) {                                                             \
+    return conv_dropin<T>(BWD_KER, input, kernel, output, batches, input_cols, input_rows,       \
+                          input_dpts, in_channel, kernel_cols, kernel_rows, kernel_dpts,         \
+                          output_cols, output_rows, output_dpts, out_channel, col_stride,        \
+                          row_stride, dpt_stride, col_in_stride, row_in_stride, dpt_in_stride,   \
+                          -1);                                                                   \
+  }                                                                                               \
+  /* The im2col / mec / naive spellings compute the same function as the            */         \
+  /* GEBP entry points (test/unit_conv_mec_naive.ml reuses one set of vectors).        */         \
+  extern "C" int owl_amd_##N##_ndarray_conv_spatial_im2col(OWL_AMD_CONV2D_FWD_ARGS(T)) {         \
+    return owl_amd_##N##_ndarray_conv_spatial(input, kernel, output, batches, input_cols,        \
+        input_rows, in_channel, kernel_cols, kernel_rows, output_cols, output_rows, out_channel, \
+        row_stride, col_stride, padding, row_in_stride, col_in_stride);                          \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_spatial_mec(OWL_AMD_CONV2D_FWD_ARGS(T)) {            \
+    return owl_amd_##N##_ndarray_conv_spatial(input, kernel, output, batches, input_cols,        \
+        input_rows, in_channel, kernel_cols, kernel_rows, output_cols, output_rows, out_channel, \
+        row_stride, col_stride, padding, row_in_stride, col_in_stride);                          \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_conv_spatial_naive(OWL_AMD_CONV2D_FWD_ARGS(T)) {          \
+    return owl_amd_##N##_ndarray_conv_spatial(input, kernel, output, batches, input_cols,        \
+        input_rows, in_channel, kernel_cols, kernel_rows, output_cols, output_rows, out_channel, \
+        row_stride, col_stride, padding, row_in_stride, col_in_stride);                          \
+  }                                                                                               \
+  DEFINE_BWD_ALIAS(N, T, spatial_backward_input, im2col)                                          \
+  DEFINE_BWD_ALIAS(N, T, spatial_backward_input, mec)                                             \
+  DEFINE_BWD_ALIAS(N, T, spatial_backward_input, naive)                                           \
+  DEFINE_BWD_ALIAS(N, T, spatial_backward_kernel, im2col)                                         \
+  DEFINE_BWD_ALIAS(N, T, spatial_backward_kernel, mec)                                            \
+  DEFINE_BWD_ALIAS(N, T, spatial_backward_kernel, naive)                                          \
+  DEFINE_C3_ALIAS(N, T, cuboid, im2col, mec, FWD)                                                 \
+  DEFINE_C3_ALIAS(N, T, cuboid, im2col, naive, FWD)                                               \
+  DEFINE_C3_ALIAS(N, T, cuboid_backward_input, im2col, mec, BWD)                                  \
+  DEFINE_C3_ALIAS(N, T, cuboid_backward_input, im2col, naive, BWD)                                \
+  DEFINE_C3_ALIAS(N, T, cuboid_backward_kernel, im2col, mec, BWD)                                 \
+  DEFINE_C3_ALIAS(N, T, cuboid_backward_kernel, im2col, naive, BWD)                               \
+  /* pooling */                                                                                   \
+  extern "C" int owl_amd_##N##_ndarray_maxpool_spatial(OWL_AMD_POOL2D_FWD_ARGS(T)) {             \
+    return pool_dropin_fwd<T>(true, input, output, batches, input_cols, input_rows, 1,           \
+                              in_channel, kernel_cols, kernel_rows, 1, output_cols, output_rows, \
+                              1, col_stride, row_stride, 1, padding);                            \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_avgpool_spatial(OWL_AMD_POOL2D_FWD_ARGS(T)) {             \
+    return pool_dropin_fwd<T>(false, input, output, batches, input_cols, input_rows, 1,          \
+                              in_channel, kernel_cols, kernel_rows, 1, output_cols, output_rows, \
+                              1, col_stride, row_stride, 1, padding);                            \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_maxpool_spatial_backward(OWL_AMD_POOL2D_BWD_ARGS(T)) {    \
+    return pool_dropin_bwd<T>(true, input, output_back, input_back, batches, input_cols,         \
+                              input_rows, 1, in_channel, kernel_cols, kernel_rows, 1,            \
+                              output_cols, output_rows, 1, col_stride, row_stride, 1, pad_cols,  \
+                              pad_rows, 0);                                                      \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_avgpool_spatial_backward(OWL_AMD_POOL2D_BWD_ARGS(T)) {    \
+    return pool_dropin_bwd<T>(false, input, output_back, input_back, batches, input_cols,        \
+                              input_rows, 1, in_channel, kernel_cols, kernel_rows, 1,            \
+                              output_cols, output_rows, 1, col_stride, row_stride, 1, pad_cols,  \
+                              pad_rows, 0);                                                      \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_maxpool_spatial_arg(OWL_AMD_POOL2D_ARG_ARGS(T)) {         \
+    return pool_dropin_arg<T>(input, output, argmax, batches, input_cols, input_rows,            \
+                              in_channel, kernel_cols, kernel_rows, output_cols, output_rows,    \
+                              col_stride, row_stride, pad_cols, pad_rows);                       \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_maxpool_cuboid(OWL_AMD_POOL3D_FWD_ARGS(T)) {              \
+    return pool_dropin_fwd<T>(true, input, output, batches, input_cols, input_rows, input_dpts,  \
+                              in_channel, kernel_cols, kernel_rows, kernel_dpts, output_cols,    \
+                              output_rows, output_dpts, col_stride, row_stride, dpt_stride,      \
+                              padding);                                                          \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_avgpool_cuboid(OWL_AMD_POOL3D_FWD_ARGS(T)) {              \
+    return pool_dropin_fwd<T>(false, input, output, batches, input_cols, input_rows, input_dpts, \
+                              in_channel, kernel_cols, kernel_rows, kernel_dpts, output_cols,    \
+                              output_rows, output_dpts, col_stride, row_stride, dpt_stride,      \
+                              padding);                                                          \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_maxpool_cuboid_backward(OWL_AMD_POOL3D_BWD_ARGS(T)) {     \
+    return pool3d_bwd_dropin<T>(true, input, output_back, input_back, batches, input_cols,       \
+                                input_rows, input_dpts, in_channel, kernel_cols, kernel_rows,    \
+                                kernel_dpts, output_cols, output_rows, output_dpts, col_stride,  \
+                                row_stride, dpt_stride, padding);                                \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_avgpool_cuboid_backward(OWL_AMD_POOL3D_BWD_ARGS(T)) {     \
+    return pool3d_bwd_dropin<T>(false, input, output_back, input_back, batches, input_cols,      \
+                                input_rows, input_dpts, in_channel, kernel_cols, kernel_rows,    \
+                                kernel_dpts, output_cols, output_rows, output_dpts, col_stride,  \
+                                row_stride, dpt_stride, padding);                                \
+  }
+
+#define DEFINE_BWD_ALIAS(N, T, OP, SUF)                                                           \
+  extern "C" int owl_amd_##N##_ndarray_conv_##OP##_##SUF(OWL_AMD_CONV2D_BWD_ARGS(T)) {           \
+    return owl_amd_##N##_ndarray_conv_##OP(input, kernel, output, batches, input_cols,           \
+        input_rows, in_channel, kernel_cols, kernel_rows, output_cols, output_rows, out_channel, \
+        row_stride, col_stride, row_in_stride, col_in_stride);                                   \
+  }
+
+#define C3_CALL_FWD                                                                               \
+  (input, kernel, output, batches, input_cols, input_rows, input_dpts, in_channel, kernel_cols,  \
+   kernel_rows, kernel_dpts, output_cols, output_rows, output_dpts, out_channel, dpt_stride,     \
+   row_stride, col_stride, padding)
+#define C3_CALL_BWD                                                                               \
+  (input, kernel, output, batches, input_cols, input_rows, input_dpts, in_channel, kernel_cols,  \
+   kernel_rows, kernel_dpts, output_cols, output_rows, output_dpts, out_channel, dpt_stride,     \
+   row_stride, col_stride)
+#define C3_ARGS_FWD(T) OWL_AMD_CONV3D_FWD_ARGS(T)
+#define C3_ARGS_BWD(T) OWL_AMD_CONV3D_BWD_ARGS(T)
+#define DEFINE_C3_ALIAS(N, T, OP, BASE, SUF, KIND)                                                \
+  extern "C" int owl_amd_##N##_ndarray_conv_##OP##_##SUF(C3_ARGS_##KIND(T)) {                    \
+    return owl_amd_##N##_ndarray_conv_##OP##_##BASE C3_CALL_##KIND;                              \
+  }
+
+namespace {
+// 3-D pooling backward derives its pad from `padding` like the forward
+// (owl_ndarray_pool_impl.h:363-373).
+template <typename T>
+int pool3d_bwd_dropin(bool is_max, T *input, T *output_back, T *input_back, int64_t B, int64_t H,
+                      int64_t W, int64_t D, int64_t C, int64_t kh, int64_t kw, int64_t kd,
+                      int64_t Ho, int64_t Wo, int64_t Do, int64_t sh, int64_t sw, int64_t sd,
+                      int64_t padding) {
+  int64_t ph = 0, pw = 0, pd = 0;
+  if (padding != 1) {
+    ph = owl_pad(sh, Ho, kh, H);
+    pw = owl_pad(sw, Wo, kw, W);
+    pd = owl_pad(sd, Do, kd, D);
+  }
+  return pool_dropin_bwd<T>(is_max, input, output_back, input_back, B, H, W, D, C, kh, kw, kd, Ho,
+                            Wo, Do, sh, sw, sd, ph, pw, pd);
+}
+}  // namespace
+
+DEFINE_DROPINS(float32, float)
+DEFINE_DROPINS(float64, double)

@@ -1,0 +1,33 @@
+// Kernel-side parameter block and launcher declarations for conv.hip.
+#pragma once
+#include "owl_amd_common.h"
+
+namespace owl_amd {
+
+// One implicit GEMM (see conv.hip for the M / N / K mapping of each pass).
+struct ConvParams {
+  int B, H, W, D, Ci;
+  int kh, kw, kd, Co;
+  int Ho, Wo, Do;
+  int sh, sw, sd;
+  int rh, rw, rd;
+  int ph, pw, pd;
+  int64_t M;
+  int N;
+  int64_t K;
+  const void *a;  // forward: x, backward input: dy, backward kernel: x
+  const void *b;  // forward: w, backward input: w,  backward kernel: dy
+  void *c;        // y, dx, dw (or split-K workspace)
+  int64_t ldc;
+  int split;            // backward kernel: number of K splits
+  int64_t k_per_split;  // multiple of the K-step
+};
+
+template <int PASS, typename TI, typename TO>
+void run_igemm(const ConvParams &p, bool aligned, hipStream_t st);
+template <typename TA, typename TO>
+void run_splitk_reduce(const TA *ws, TO *out, int64_t n, int splits, hipStream_t st);
+template <typename TI> int tile_bm();
+template <typename TI> int tile_bk();
+
+}  // namespace owl_amd

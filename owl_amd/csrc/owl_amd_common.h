@@ -1,0 +1,33 @@
+// Shared definitions for the owl_amd HIP library (gfx950 / CDNA4 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/owl_amd.h"
+
+typedef __bf16 bf16_t;
+typedef _Float16 f16_t;
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+#define OWL_CHECK_HIP(expr)                                   \
+  do {                                                        \
+    hipError_t _e = (expr);                                   \
+    if (_e != hipSuccess) {                                   \
+      owl_amd_set_error("%s:%d %s: %s", __FILE__, __LINE__,   \
+                        #expr, hipGetErrorString(_e));        \
+      return OWL_AMD_ERR_HIP;                                 \
+    }                                                         \
+  } while (0)
+
+extern "C" void owl_amd_set_error(const char *fmt, ...);
+
+// Conversions between storage types and the FP32 accumulator.
+template <typename T> __device__ __forceinline__ float to_f32(T v) { return (float)v; }
+template <typename T> __device__ __forceinline__ T from_f32(float v) { return (T)v; }
+template <> __device__ __forceinline__ double from_f32<double>(float v) { return (double)v; }

@@ -1,0 +1,295 @@
+// Max / average pooling for gfx950, bit-identical to Owl's C
+// (owl_ndarray_pool_impl.h with the macros of owl_ndarray_pool_stub.c:16-51).
+//
+// These are bandwidth kernels: one thread per output cell (forward) or per
+// input cell (backward), each covering VEC consecutive channels so a wave
+// reads and writes contiguous 16-byte chunks of the NHWC / NDHWC arrays.
+// Within a cell the arithmetic is Owl's, in Owl's order:
+//   * the window is scanned H ("cols") outer, W ("rows"), D ("dpts") inner;
+//   * max: acc = -inf; if (acc < t) acc = t   -> first strict max, NaN never wins;
+//   * avg: acc = 0; acc += t; acc / count_in_bounds  (FP32 stays FP32);
+//   * backward is a gather: an input cell adds the contributions of the
+//     windows that cover it in ascending (i, j, l) order, the order in which
+//     Owl's scatter loop (:139-198, :377-440) adds them, starting from 0.
+// Owl's max backward sends the gradient of a window with no candidate (all
+// NaN / -inf) to flat index 0 of the whole input (max_idx starts at 0,
+// :167); such windows raise a flag and a one-workgroup fix-up kernel rebuilds
+// element 0 in Owl's order.
+#include "owl_amd_common.h"
+#include "pool_kernels.h"
+
+namespace owl_amd {
+
+template <typename T> struct PoolAcc { typedef float type; };
+template <> struct PoolAcc<double> { typedef double type; };
+
+template <typename T> __device__ __forceinline__ T neg_inf();
+template <> __device__ __forceinline__ float neg_inf<float>() { return -__builtin_inff(); }
+template <> __device__ __forceinline__ double neg_inf<double>() { return -__builtin_inf(); }
+
+struct Win {
+  int lo, hi;  // in-bounds index range [lo, hi) of the window along one axis
+};
+
+__device__ __forceinline__ Win window(int o, int s, int p, int k, int n) {
+  int st = o * s - p;
+  int lo = st < 0 ? 0 : st;
+  int hi = st + k > n ? n : st + k;
+  return {lo, hi};
+}
+
+// ---------------------------------------------------------------- forward
+template <typename T, int VEC, int MODE>  // MODE 0 avg, 1 max, 2 max + argmax
+__global__ __launch_bounds__(256) void pool_fwd_kernel(PoolParams p, const T *__restrict__ x,
+                                                       T *__restrict__ y,
+                                                       int64_t *__restrict__ arg) {
+  typedef typename PoolAcc<T>::type A;
+  const int CV = p.C / VEC;
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)p.B * p.Ho * p.Wo * p.Do * CV;
+  if (gid >= total) return;
+  int cv = (int)(gid % CV);
+  int64_t t = gid / CV;
+  int l = (int)(t % p.Do); t /= p.Do;
+  int j = (int)(t % p.Wo); t /= p.Wo;
+  int i = (int)(t % p.Ho);
+  int64_t b = t / p.Ho;
+  Win wh = window(i, p.sh, p.ph, p.kh, p.H);
+  Win ww = window(j, p.sw, p.pw, p.kw, p.W);
+  Win wd = window(l, p.sd, p.pd, p.kd, p.D);
+  const int c0 = cv * VEC;
+  A acc[VEC];
+  int64_t mi[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) {
+    acc[v] = MODE == 0 ? (A)0 : (A)neg_inf<A>();
+    mi[v] = -1;
+  }
+  for (int h = wh.lo; h < wh.hi; ++h)
+    for (int w = ww.lo; w < ww.hi; ++w)
+      for (int d = wd.lo; d < wd.hi; ++d) {
+        int64_t id = (((b * p.H + h) * (int64_t)p.W + w) * p.D + d) * (int64_t)p.C + c0;
+        T tv[VEC];
+        __builtin_memcpy(tv, x + id, sizeof(tv));
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          A tval = (A)tv[v];
+          if (MODE == 0) {
+            acc[v] += tval;
+          } else if (acc[v] < tval) {
+            acc[v] = tval;
+            mi[v] = id + v;
+          }
+        }
+      }
+  int cnt = (wh.hi - wh.lo) * (ww.hi - ww.lo) * (wd.hi - wd.lo);
+  if (wh.hi < wh.lo || ww.hi < ww.lo || wd.hi < wd.lo) cnt = 0;
+  int64_t od = gid * VEC;
+  T out[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) out[v] = (T)(MODE == 0 ? acc[v] / (A)cnt : acc[v]);
+  __builtin_memcpy(y + od, out, sizeof(out));
+  if (MODE == 2) {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) arg[od + v] = mi[v];
+  }
+}
+
+// ---------------------------------------------------------------- backward
+// Range of output indices o whose window covers input index h along one axis:
+// o*s - p <= h < o*s - p + k  ->  ceil((h + p - k + 1) / s) <= o <= floor((h + p) / s).
+__device__ __forceinline__ void covering(int h, int s, int p, int k, int O, int &lo, int &hi) {
+  int num = h + p - k + 1;
+  lo = num <= 0 ? 0 : (num + s - 1) / s;
+  int top = h + p;
+  hi = top < 0 ? -1 : top / s;
+  if (hi > O - 1) hi = O - 1;
+}
+
+template <typename T, int VEC, bool IS_MAX>
+__global__ __launch_bounds__(256) void pool_bwd_kernel(PoolParams p, const T *__restrict__ x,
+                                                       const T *__restrict__ dy,
+                                                       T *__restrict__ dx, int *orphan_flag) {
+  typedef typename PoolAcc<T>::type A;
+  const int CV = p.C / VEC;
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)p.B * p.H * p.W * p.D * CV;
+  if (gid >= total) return;
+  int cv = (int)(gid % CV);
+  int64_t t = gid / CV;
+  int d = (int)(t % p.D); t /= p.D;
+  int w = (int)(t % p.W); t /= p.W;
+  int h = (int)(t % p.H);
+  int64_t b = t / p.H;
+  const int c0 = cv * VEC;
+  const int64_t my_id = (((b * p.H + h) * (int64_t)p.W + w) * p.D + d) * (int64_t)p.C + c0;
+
+  int ilo, ihi, jlo, jhi, llo, lhi;
+  covering(h, p.sh, p.ph, p.kh, p.Ho, ilo, ihi);
+  covering(w, p.sw, p.pw, p.kw, p.Wo, jlo, jhi);
+  covering(d, p.sd, p.pd, p.kd, p.Do, llo, lhi);
+
+  T acc[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) acc[v] = (T)0;
+
+  for (int i = ilo; i <= ihi; ++i) {
+    Win wh = window(i, p.sh, p.ph, p.kh, p.H);
+    for (int j = jlo; j <= jhi; ++j) {
+      Win ww = window(j, p.sw, p.pw, p.kw, p.W);
+      for (int l = llo; l <= lhi; ++l) {
+        Win wd = window(l, p.sd, p.pd, p.kd, p.D);
+        int64_t od = (((b * p.Ho + i) * (int64_t)p.Wo + j) * p.Do + l) * (int64_t)p.C + c0;
+        T g[VEC];
+        __builtin_memcpy(g, dy + od, sizeof(g));
+        if (IS_MAX) {
+          A best[VEC];
+          int64_t bi[VEC];
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            best[v] = neg_inf<A>();
+            bi[v] = -1;
+          }
+          for (int hh = wh.lo; hh < wh.hi; ++hh)
+            for (int w2 = ww.lo; w2 < ww.hi; ++w2)
+              for (int dd = wd.lo; dd < wd.hi; ++dd) {
+                int64_t id = (((b * p.H + hh) * (int64_t)p.W + w2) * p.D + dd) * (int64_t)p.C + c0;
+                T tv[VEC];
+                __builtin_memcpy(tv, x + id, sizeof(tv));
+#pragma unroll
+                for (int v = 0; v < VEC; ++v)
+                  if (best[v] < (A)tv[v]) {
+                    best[v] = (A)tv[v];
+                    bi[v] = id + v;
+                  }
+              }
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            if (bi[v] == my_id + v) acc[v] += g[v];
+            if (bi[v] < 0) *orphan_flag = 1;  // Owl sends this gradient to index 0
+          }
+        } else {
+          int cnt = (wh.hi - wh.lo) * (ww.hi - ww.lo) * (wd.hi - wd.lo);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) acc[v] += g[v] / (T)cnt;
+        }
+      }
+    }
+  }
+  __builtin_memcpy(dx + my_id, acc, sizeof(acc));
+}
+
+// Rebuild dx[0] in Owl's order when some window had no max candidate.  One
+// workgroup walks the windows in Owl's loop order (b, i, j, l, c) 256 at a
+// time; thread 0 adds the flagged gradients in that order.
+template <typename T>
+__global__ __launch_bounds__(256) void pool_bwd_fixup_kernel(PoolParams p, const T *__restrict__ x,
+                                                             const T *__restrict__ dy,
+                                                             T *__restrict__ dx, int *orphan_flag) {
+  typedef typename PoolAcc<T>::type A;
+  if (*orphan_flag == 0) return;
+  __shared__ int hit[256];
+  __shared__ T val[256];
+  int64_t total = (int64_t)p.B * p.Ho * p.Wo * p.Do * p.C;
+  T acc = (T)0;
+  for (int64_t base = 0; base < total; base += 256) {
+    int64_t gid = base + threadIdx.x;
+    int flag = 0;
+    T g = (T)0;
+    if (gid < total) {
+      int c = (int)(gid % p.C);
+      int64_t t = gid / p.C;
+      int l = (int)(t % p.Do); t /= p.Do;
+      int j = (int)(t % p.Wo); t /= p.Wo;
+      int i = (int)(t % p.Ho);
+      int64_t b = t / p.Ho;
+      Win wh = window(i, p.sh, p.ph, p.kh, p.H);
+      Win ww = window(j, p.sw, p.pw, p.kw, p.W);
+      Win wd = window(l, p.sd, p.pd, p.kd, p.D);
+      A best = neg_inf<A>();
+      int64_t bi = 0;
+      for (int hh = wh.lo; hh < wh.hi; ++hh)
+        for (int w2 = ww.lo; w2 < ww.hi; ++w2)
+          for (int dd = wd.lo; dd < wd.hi; ++dd) {
+            int64_t id = (((b * p.H + hh) * (int64_t)p.W + w2) * p.D + dd) * (int64_t)p.C + c;
+            A tv = (A)x[id];
+            if (best < tv) {
+              best = tv;
+              bi = id;
+            }
+          }
+      flag = bi == 0;
+      g = dy[gid];
+    }
+    hit[threadIdx.x] = flag;
+    val[threadIdx.x] = g;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 256; ++k)
+        if (hit[k]) acc += val[k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    dx[0] = acc;
+    *orphan_flag = 0;
+  }
+}
+
+__global__ void set_flag_kernel(int *p) { *p = 1; }
+
+// ---------------------------------------------------------------- launchers
+template <typename T>
+static int pick_vec(const PoolParams &p, const void *a, const void *b, const void *c) {
+  const int v = 16 / (int)sizeof(T);
+  uintptr_t al = (uintptr_t)a | (uintptr_t)b | (uintptr_t)c;
+  if (p.C % v == 0 && (al % 16) == 0) return v;
+  return 1;
+}
+
+template <typename T>
+void run_pool_fwd(const PoolParams &p, int mode, const T *x, T *y, int64_t *arg, hipStream_t st) {
+  int vec = pick_vec<T>(p, x, y, nullptr);
+  if (mode == 2) vec = 1;  // int64 argmax writes: keep one channel per thread
+  int64_t total = (int64_t)p.B * p.Ho * p.Wo * p.Do * (p.C / vec);
+  dim3 grid((unsigned)((total + 255) / 256)), block(256);
+  constexpr int V = 16 / (int)sizeof(T);
+  if (mode == 0) {
+    if (vec == V) hipLaunchKernelGGL((pool_fwd_kernel<T, V, 0>), grid, block, 0, st, p, x, y, arg);
+    else hipLaunchKernelGGL((pool_fwd_kernel<T, 1, 0>), grid, block, 0, st, p, x, y, arg);
+  } else if (mode == 1) {
+    if (vec == V) hipLaunchKernelGGL((pool_fwd_kernel<T, V, 1>), grid, block, 0, st, p, x, y, arg);
+    else hipLaunchKernelGGL((pool_fwd_kernel<T, 1, 1>), grid, block, 0, st, p, x, y, arg);
+  } else {
+    hipLaunchKernelGGL((pool_fwd_kernel<T, 1, 2>), grid, block, 0, st, p, x, y, arg);
+  }
+}
+
+template <typename T>
+void run_pool_bwd(const PoolParams &p, bool is_max, const T *x, const T *dy, T *dx, int *flag,
+                  hipStream_t st) {
+  int vec = pick_vec<T>(p, x, dy, dx);
+  int64_t total = (int64_t)p.B * p.H * p.W * p.D * (p.C / vec);
+  dim3 grid((unsigned)((total + 255) / 256)), block(256);
+  constexpr int V = 16 / (int)sizeof(T);
+  if (is_max) {
+    if (vec == V) hipLaunchKernelGGL((pool_bwd_kernel<T, V, true>), grid, block, 0, st, p, x, dy, dx, flag);
+    else hipLaunchKernelGGL((pool_bwd_kernel<T, 1, true>), grid, block, 0, st, p, x, dy, dx, flag);
+    hipLaunchKernelGGL((pool_bwd_fixup_kernel<T>), dim3(1), dim3(256), 0, st, p, x, dy, dx, flag);
+  } else {
+    if (vec == V) hipLaunchKernelGGL((pool_bwd_kernel<T, V, false>), grid, block, 0, st, p, x, dy, dx, flag);
+    else hipLaunchKernelGGL((pool_bwd_kernel<T, 1, false>), grid, block, 0, st, p, x, dy, dx, flag);
+  }
+}
+
+// A window with no in-bounds cell (only possible with caller-chosen pads)
+// is an orphan too; the host detects it from the sizes and raises the flag.
+void run_set_flag(int *flag, hipStream_t st) {
+  hipLaunchKernelGGL(set_flag_kernel, dim3(1), dim3(1), 0, st, flag);
+}
+
+template void run_pool_fwd<float>(const PoolParams &, int, const float *, float *, int64_t *, hipStream_t);
+template void run_pool_fwd<double>(const PoolParams &, int, const double *, double *, int64_t *, hipStream_t);
+template void run_pool_bwd<float>(const PoolParams &, bool, const float *, const float *, float *, int *, hipStream_t);
+template void run_pool_bwd<double>(const PoolParams &, bool, const double *, const double *, double *, int *, hipStream_t);
+
+}  // namespace owl_amd

@@ -1,0 +1,514 @@
+"""Owl's ``Dense.Ndarray`` convolution / pooling API on the MI355X path.
+
+Each function mirrors the OCaml wrapper of the same name in
+``src/owl/dense/owl_dense_ndarray_generic.ml`` (argument order, shape checks,
+output shape, SAME/VALID handling, stride/rate unpacking) and then calls the
+C ABI of ``libowl_amd.so``:
+
+* numpy arrays (float32 = Owl's S, float64 = Owl's D) go through the Owl
+  drop-in entry points (``owl_amd_<dtype>_ndarray_*``) with Owl's stub
+  arguments, exactly as the OCaml externals would call them;
+* torch CUDA tensors (float32, float64, bfloat16, float16) stay in HBM and go
+  through the device entry points on torch's current stream.
+
+Shapes use Owl's order: input ``[b; h; w; c]`` (3-D ``[b; h; w; d; c]``),
+kernel ``[kh; kw; ci; co]``, ``stride = [| h_stride; w_stride |]``.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .shapes import (SAME, VALID, InvalidArgument, c_pad, calc_conv2d_padding, conv_out_dim,
+                     k_eff, transpose_out_dim)
+
+try:  # torch is plumbing for device memory and streams only
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+__all__ = [
+    "conv2d", "conv2d_backward_input", "conv2d_backward_kernel",
+    "conv3d", "conv3d_backward_input", "conv3d_backward_kernel",
+    "dilated_conv2d", "dilated_conv2d_backward_input", "dilated_conv2d_backward_kernel",
+    "dilated_conv3d", "dilated_conv3d_backward_input", "dilated_conv3d_backward_kernel",
+    "transpose_conv2d", "transpose_conv2d_backward_input", "transpose_conv2d_backward_kernel",
+    "transpose_conv3d", "transpose_conv3d_backward_input", "transpose_conv3d_backward_kernel",
+    "max_pool2d", "avg_pool2d", "max_pool3d", "avg_pool3d", "max_pool2d_argmax",
+    "max_pool2d_backward", "avg_pool2d_backward", "max_pool3d_backward", "avg_pool3d_backward",
+    "SAME", "VALID", "InvalidArgument",
+]
+
+
+# ----------------------------------------------------------------------------- helpers
+def _is_torch(a):
+    return torch is not None and isinstance(a, torch.Tensor)
+
+
+def _np_prefix(a):
+    if a.dtype == np.float32:
+        return "float32"
+    if a.dtype == np.float64:
+        return "float64"
+    raise InvalidArgument(f"unsupported dtype {a.dtype} (Owl S = float32, D = float64)")
+
+
+_TORCH_DT = {}
+if torch is not None:
+    _TORCH_DT = {torch.float32: _lib.F32, torch.float64: _lib.F64,
+                 torch.bfloat16: _lib.BF16, torch.float16: _lib.F16}
+
+
+def _tdt(t):
+    if t.dtype not in _TORCH_DT:
+        raise InvalidArgument(f"unsupported tensor dtype {t.dtype}")
+    return _TORCH_DT[t.dtype]
+
+
+def _ptr(a):
+    if _is_torch(a):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _contig(a):
+    if _is_torch(a):
+        if not a.is_cuda:
+            raise InvalidArgument("torch tensors must be on a CUDA (HIP) device")
+        return a.contiguous()
+    return np.ascontiguousarray(a)
+
+
+def _empty_like_kind(ref, shape, dtype=None):
+    if _is_torch(ref):
+        return torch.empty(shape, dtype=dtype or ref.dtype, device=ref.device)
+    return np.empty(shape, dtype=dtype or ref.dtype)
+
+
+def _verify(cond, msg):
+    if not cond:
+        raise InvalidArgument(msg)
+
+
+def _same_kind(*arrs):
+    t = [_is_torch(a) for a in arrs]
+    _verify(all(t) or not any(t), "mix of numpy arrays and device tensors")
+    if not t[0]:
+        _verify(len({a.dtype for a in arrs}) == 1, "arguments must share one dtype")
+
+
+def _desc(B, H, W, D, Ci, kh, kw, kd, Co, Ho, Wo, Do, s, r, pads):
+    return _lib.ConvDesc(B, H, W, D, Ci, kh, kw, kd, Co, Ho, Wo, Do, *s, *r, *pads)
+
+
+# ----------------------------------------------------------------------------- conv core
+# A conv problem in the library's 3-D form; 2-D problems have D = kd = Do = 1.
+class _Conv:
+    def __init__(self, B, sp_in, Ci, ksp, Co, sp_out, stride, rate, pads):
+        self.B, self.sp_in, self.Ci, self.ksp, self.Co = B, list(sp_in), Ci, list(ksp), Co
+        self.sp_out, self.stride, self.rate, self.pads = list(sp_out), list(stride), list(rate), list(pads)
+
+    def desc(self):
+        f = lambda v, d: list(v) + [d] * (3 - len(v))
+        return _desc(self.B, *f(self.sp_in, 1), self.Ci, *f(self.ksp, 1), self.Co,
+                     *f(self.sp_out, 1), f(self.stride, 1), f(self.rate, 1), f(self.pads, 0))
+
+
+def _pads_from_shapes(sp_in, ksp, sp_out, stride, rate):
+    return [c_pad(s, o, k_eff(k, r), n) for n, k, o, s, r in zip(sp_in, ksp, sp_out, stride, rate)]
+
+
+def _dev_conv(pass_, g, a, b, out, out_t=None):
+    """Run one pass on device tensors.  pass_: fwd(a=x, b=w), bi(a=w, b=dy), bk(a=x, b=dy)."""
+    st = _stream()
+    d = g.desc()
+    it = _tdt(a)
+    ot = _TORCH_DT[out.dtype] if out_t is None else out_t
+    L = _lib.lib()
+    if pass_ == "fwd":
+        _lib.check(L.owl_amd_conv_fwd(ctypes.byref(d), it, ot, _ptr(a), _ptr(b), _ptr(out), st))
+    elif pass_ == "bi":
+        _lib.check(L.owl_amd_conv_bwd_input(ctypes.byref(d), it, ot, _ptr(a), _ptr(b), _ptr(out), st))
+    else:
+        _lib.check(L.owl_amd_conv_bwd_kernel(ctypes.byref(d), it, ot, _ptr(a), _ptr(b), _ptr(out),
+                                            None, 0, st))
+    return out
+
+
+def _host_conv2d(variant, x, w, y, g, padding_code=None, rate=(1, 1)):
+    """Call an Owl 2-D conv drop-in with Owl's stub argument order
+    (owl_ndarray_conv_impl.h:246-253 / :465-471)."""
+    H, W = g.sp_in
+    kh, kw = g.ksp
+    Ho, Wo = g.sp_out
+    sh, sw = g.stride
+    args = [_ptr(x), _ptr(w), _ptr(y), g.B, H, W, g.Ci, kh, kw, Ho, Wo, g.Co, sw, sh]
+    if padding_code is not None:
+        args.append(padding_code)
+    args += [rate[1], rate[0]]
+    _lib.call(f"owl_amd_{_np_prefix(x)}_ndarray_conv_{variant}", *args)
+
+
+def _host_conv3d(variant, x, w, y, g, padding_code=None, rate=None):
+    """Owl 3-D stub order (owl_ndarray_conv_impl.h:1211-1220, :3417-3427)."""
+    H, W, D = g.sp_in
+    kh, kw, kd = g.ksp
+    Ho, Wo, Do = g.sp_out
+    sh, sw, sd = g.stride
+    args = [_ptr(x), _ptr(w), _ptr(y), g.B, H, W, D, g.Ci, kh, kw, kd, Ho, Wo, Do, g.Co, sd, sw, sh]
+    if rate is not None:
+        args += [rate[2], rate[1], rate[0]]
+    if padding_code is not None:
+        args.append(padding_code)
+    _lib.call(f"owl_amd_{_np_prefix(x)}_ndarray_conv_{variant}", *args)
+
+
+def _check_conv_args(name, nd, x, k, stride, rate=None, dy=None):
+    _verify(len(x.shape) == nd + 2, f"{name}: input dimension = {len(x.shape)} (should be {nd + 2})")
+    _verify(len(k.shape) == nd + 2, f"{name}: kernel dimension = {len(k.shape)} (should be {nd + 2})")
+    _verify(len(stride) == nd, f"{name}: stride dimension = {len(stride)} (should be {nd})")
+    if rate is not None:
+        _verify(len(rate) == nd, f"{name}: rate dimension = {len(rate)} (should be {nd})")
+    _verify(x.shape[-1] == k.shape[-2],
+            f"{name}: the {nd + 2}th dimension of input shape should be equal to the "
+            f"{nd + 1}rd dimension of kernel shape")
+    if dy is not None:
+        _verify(len(dy.shape) == nd + 2, f"{name}: output' dimension = {len(dy.shape)} (should be {nd + 2})")
+        _verify(dy.shape[0] == x.shape[0] and dy.shape[-1] == k.shape[-1],
+                f"{name}: output' shape does not match input / kernel")
+
+
+def _conv_fwd(name, nd, x, k, stride, rate, padding, variant, out_dtype=None):
+    _same_kind(x, k)
+    _check_conv_args(name, nd, x, k, stride, rate)
+    x, k = _contig(x), _contig(k)
+    B, sp_in, Ci = x.shape[0], list(x.shape[1:-1]), x.shape[-1]
+    ksp, Co = list(k.shape[:-2]), k.shape[-1]
+    rate = list(rate) if rate is not None else [1] * nd
+    sp_out = [conv_out_dim(padding, n, k_eff(kk, r), s) for n, kk, s, r in zip(sp_in, ksp, stride, rate)]
+    _verify(all(o > 0 for o in sp_out), f"{name}: empty output shape {sp_out}")
+    pads = _pads_from_shapes(sp_in, ksp, sp_out, stride, rate) if padding == SAME else [0] * nd
+    g = _Conv(B, sp_in, Ci, ksp, Co, sp_out, stride, rate, pads)
+    y = _empty_like_kind(x, [B, *sp_out, Co], dtype=out_dtype)
+    if _is_torch(x):
+        return _dev_conv("fwd", g, x, k, y)
+    pc = 0 if padding == SAME else 1
+    if nd == 2:
+        _host_conv2d(variant, x, k, y, g, pc, rate)
+    else:
+        _host_conv3d(variant, x, k, y, g, pc, rate if variant.startswith("dilated") else None)
+    return y
+
+
+def _conv_bwd(name, nd, which, x, k, stride, rate, dy, variant, out_dtype=None):
+    _same_kind(x, k, dy)
+    _check_conv_args(name, nd, x, k, stride, rate, dy)
+    x, k, dy = _contig(x), _contig(k), _contig(dy)
+    B, sp_in, Ci = x.shape[0], list(x.shape[1:-1]), x.shape[-1]
+    ksp, Co = list(k.shape[:-2]), k.shape[-1]
+    sp_out = list(dy.shape[1:-1])
+    rate = list(rate) if rate is not None else [1] * nd
+    pads = _pads_from_shapes(sp_in, ksp, sp_out, stride, rate)
+    g = _Conv(B, sp_in, Ci, ksp, Co, sp_out, stride, rate, pads)
+    if which == "input":
+        out = _empty_like_kind(x, list(x.shape), dtype=out_dtype)
+        if _is_torch(x):
+            return _dev_conv("bi", g, k, dy, out)
+        args = (out, k, dy)
+    else:
+        out = _empty_like_kind(k, list(k.shape), dtype=out_dtype)
+        if _is_torch(x):
+            return _dev_conv("bk", g, x, dy, out)
+        args = (x, out, dy)
+    if nd == 2:
+        _host_conv2d(variant, *args, g, None, rate)
+    else:
+        _host_conv3d(variant, *args, g, None, rate if variant.startswith("dilated") else None)
+    return out
+
+
+# ----------------------------------------------------------------------------- conv2d / 3d
+def conv2d(input, kernel, stride, padding=SAME, out_dtype=None):
+    """owl_dense_ndarray_generic.ml:2324-2400."""
+    return _conv_fwd("conv2d", 2, input, kernel, stride, None, padding, "spatial", out_dtype)
+
+
+def conv2d_backward_input(input, kernel, stride, output_grad, out_dtype=None):
+    """owl_dense_ndarray_generic.ml:2479-2560."""
+    return _conv_bwd("conv2d_backward_input", 2, "input", input, kernel, stride, None, output_grad,
+                     "spatial_backward_input", out_dtype)
+
+
+def conv2d_backward_kernel(input, kernel, stride, output_grad, out_dtype=None):
+    """owl_dense_ndarray_generic.ml:2660-2740."""
+    return _conv_bwd("conv2d_backward_kernel", 2, "kernel", input, kernel, stride, None, output_grad,
+                     "spatial_backward_kernel", out_dtype)
+
+
+def conv3d(input, kernel, stride, padding=SAME, out_dtype=None):
+    """owl_dense_ndarray_generic.ml:2848 (dispatches to cuboid_im2col, owl_ndarray_conv.ml:826)."""
+    return _conv_fwd("conv3d", 3, input, kernel, stride, None, padding, "cuboid_im2col", out_dtype)
+
+
+def conv3d_backward_input(input, kernel, stride, output_grad, out_dtype=None):
+    """owl_dense_ndarray_generic.ml:3017."""
+    return _conv_bwd("conv3d_backward_input", 3, "input", input, kernel, stride, None, output_grad,
+                     "cuboid_backward_input_im2col", out_dtype)
+
+
+def conv3d_backward_kernel(input, kernel, stride, output_grad, out_dtype=None):
+    return _conv_bwd("conv3d_backward_kernel", 3, "kernel", input, kernel, stride, None, output_grad,
+                     "cuboid_backward_kernel_im2col", out_dtype)
+
+
+def dilated_conv2d(input, kernel, stride, rate, padding=SAME, out_dtype=None):
+    """owl_dense_ndarray_generic.ml:3797-3880 (output shape uses the dilated extent)."""
+    return _conv_fwd("dilated_conv2d", 2, input, kernel, stride, rate, padding, "dilated_spatial_im2col",
+                     out_dtype)
+
+
+def dilated_conv2d_backward_input(input, kernel, stride, rate, output_grad, out_dtype=None):
+    return _conv_bwd("dilated_conv2d_backward_input", 2, "input", input, kernel, stride, rate, output_grad,
+                     "dilated_spatial_backward_input_im2col", out_dtype)
+
+
+def dilated_conv2d_backward_kernel(input, kernel, stride, rate, output_grad, out_dtype=None):
+    return _conv_bwd("dilated_conv2d_backward_kernel", 2, "kernel", input, kernel, stride, rate,
+                     output_grad, "dilated_spatial_backward_kernel_im2col", out_dtype)
+
+
+def dilated_conv3d(input, kernel, stride, rate, padding=SAME, out_dtype=None):
+    return _conv_fwd("dilated_conv3d", 3, input, kernel, stride, rate, padding, "dilated_cuboid_im2col",
+                     out_dtype)
+
+
+def dilated_conv3d_backward_input(input, kernel, stride, rate, output_grad, out_dtype=None):
+    return _conv_bwd("dilated_conv3d_backward_input", 3, "input", input, kernel, stride, rate, output_grad,
+                     "dilated_cuboid_backward_input_im2col", out_dtype)
+
+
+def dilated_conv3d_backward_kernel(input, kernel, stride, rate, output_grad, out_dtype=None):
+    return _conv_bwd("dilated_conv3d_backward_kernel", 3, "kernel", input, kernel, stride, rate,
+                     output_grad, "dilated_cuboid_backward_kernel_im2col", out_dtype)
+
+
+# ----------------------------------------------------------------------------- transpose conv
+# Owl runs transpose convolutions on the ordinary kernels with the roles of
+# input and output swapped and the kernel buffer [kh; kw; ci; co] read as
+# [kh; kw; co; ci] -- a reinterpretation, not a transpose
+# (owl_dense_ndarray_generic.ml:5371-5440, 5514-5611, 5713-5806).
+def _tconv_geom(x, k, sp_out):
+    B, sp_in, Ci = x.shape[0], list(x.shape[1:-1]), x.shape[-1]
+    ksp, Co = list(k.shape[:-2]), k.shape[-1]
+    return B, sp_in, Ci, ksp, Co
+
+
+def transpose_conv2d(input, kernel, stride, padding=SAME):
+    return _transpose_fwd("transpose_conv2d", 2, input, kernel, stride, padding)
+
+
+def transpose_conv3d(input, kernel, stride, padding=SAME):
+    return _transpose_fwd("transpose_conv3d", 3, input, kernel, stride, padding)
+
+
+def _transpose_fwd(name, nd, x, k, stride, padding):
+    _same_kind(x, k)
+    _check_conv_args(name, nd, x, k, stride)
+    x, k = _contig(x), _contig(k)
+    B, sp_in, Ci, ksp, Co = _tconv_geom(x, k, None)
+    sp_out = [transpose_out_dim(padding, n, kk, s) for n, kk, s in zip(sp_in, ksp, stride)]
+    y = _empty_like_kind(x, [B, *sp_out, Co])
+    # backward_input of a conv whose input is y ([.., Co]) and output is x ([.., Ci])
+    pads = _pads_from_shapes(sp_out, ksp, sp_in, stride, [1] * nd)
+    g = _Conv(B, sp_out, Co, ksp, Ci, sp_in, stride, [1] * nd, pads)
+    if _is_torch(x):
+        return _dev_conv("bi", g, k, x, y)
+    if nd == 2:
+        _host_conv2d("spatial_backward_input", y, k, x, g, None)
+    else:
+        _host_conv3d("cuboid_backward_input_im2col", y, k, x, g, None)
+    return y
+
+
+def transpose_conv2d_backward_input(input, kernel, stride, output_grad):
+    return _transpose_bwd("transpose_conv2d_backward_input", 2, "input", input, kernel, stride, output_grad)
+
+
+def transpose_conv2d_backward_kernel(input, kernel, stride, output_grad):
+    return _transpose_bwd("transpose_conv2d_backward_kernel", 2, "kernel", input, kernel, stride, output_grad)
+
+
+def transpose_conv3d_backward_input(input, kernel, stride, output_grad):
+    return _transpose_bwd("transpose_conv3d_backward_input", 3, "input", input, kernel, stride, output_grad)
+
+
+def transpose_conv3d_backward_kernel(input, kernel, stride, output_grad):
+    return _transpose_bwd("transpose_conv3d_backward_kernel", 3, "kernel", input, kernel, stride, output_grad)
+
+
+def _transpose_bwd(name, nd, which, x, k, stride, dy):
+    _same_kind(x, k, dy)
+    _check_conv_args(name, nd, x, k, stride, None, dy)
+    x, k, dy = _contig(x), _contig(k), _contig(dy)
+    B, sp_in, Ci, ksp, Co = _tconv_geom(x, k, None)
+    sp_out = list(dy.shape[1:-1])
+    # conv with input = dy ([.., Co]) and output = x ([.., Ci])
+    pads = _pads_from_shapes(sp_out, ksp, sp_in, stride, [1] * nd)
+    g = _Conv(B, sp_out, Co, ksp, Ci, sp_in, stride, [1] * nd, pads)
+    if which == "input":
+        out = _empty_like_kind(x, list(x.shape))
+        if _is_torch(x):
+            return _dev_conv("fwd", g, dy, k, out)
+        if nd == 2:
+            # Owl passes a dummy SAME code here (owl_dense_ndarray_generic.ml:5783)
+            _host_conv2d("spatial", dy, k, out, g, 0)
+        else:
+            _host_conv3d("cuboid_im2col", dy, k, out, g, 0)
+    else:
+        out = _empty_like_kind(k, list(k.shape))
+        if _is_torch(x):
+            return _dev_conv("bk", g, dy, x, out)
+        if nd == 2:
+            _host_conv2d("spatial_backward_kernel", dy, out, x, g, None)
+        else:
+            _host_conv3d("cuboid_backward_kernel_im2col", dy, out, x, g, None)
+    return out
+
+
+# ----------------------------------------------------------------------------- pooling
+def _pool_check(name, nd, x, kernel, stride):
+    _verify(len(x.shape) == nd + 2, f"{name}: input dimension = {len(x.shape)} (should be {nd + 2})")
+    _verify(len(kernel) == nd, f"{name}: kernel dimension = {len(kernel)} (should be {nd})")
+    _verify(len(stride) == nd, f"{name}: stride dimension = {len(stride)} (should be {nd})")
+
+
+def _pool_geom(x, kernel, stride, padding):
+    B, sp_in, C = x.shape[0], list(x.shape[1:-1]), x.shape[-1]
+    sp_out = [conv_out_dim(padding, n, k, s) for n, k, s in zip(sp_in, kernel, stride)]
+    if padding == SAME:
+        pads = [c_pad(s, o, k, n) for n, k, o, s in zip(sp_in, kernel, sp_out, stride)]
+    else:
+        pads = [0] * len(sp_in)
+    return B, sp_in, C, sp_out, pads
+
+
+def _pool_desc(B, sp_in, C, kernel, sp_out, stride, pads):
+    f = lambda v, d: list(v) + [d] * (3 - len(v))
+    return _lib.PoolDesc(B, *f(sp_in, 1), C, *f(kernel, 1), *f(sp_out, 1), *f(stride, 1), *f(pads, 0))
+
+
+def _pool_fwd(name, nd, is_max, x, kernel, stride, padding):
+    _pool_check(name, nd, x, kernel, stride)
+    x = _contig(x)
+    B, sp_in, C, sp_out, pads = _pool_geom(x, kernel, stride, padding)
+    y = _empty_like_kind(x, [B, *sp_out, C])
+    if _is_torch(x):
+        d = _pool_desc(B, sp_in, C, kernel, sp_out, stride, pads)
+        _lib.check(_lib.lib().owl_amd_pool_fwd(ctypes.byref(d), _tdt(x), int(is_max), _ptr(x), _ptr(y),
+                                               _stream()))
+        return y
+    pc = 0 if padding == SAME else 1
+    op = ("maxpool" if is_max else "avgpool") + ("_spatial" if nd == 2 else "_cuboid")
+    if nd == 2:
+        # owl_ndarray_pool_impl.h:9-15
+        _lib.call(f"owl_amd_{_np_prefix(x)}_ndarray_{op}", _ptr(x), _ptr(y), B, sp_in[0], sp_in[1], C,
+                  kernel[0], kernel[1], sp_out[0], sp_out[1], stride[1], stride[0], pc, 1, 1)
+    else:
+        # owl_ndarray_pool_impl.h:212-219 (strides dpt, row, col)
+        _lib.call(f"owl_amd_{_np_prefix(x)}_ndarray_{op}", _ptr(x), _ptr(y), B, *sp_in, C, *kernel,
+                  *sp_out, stride[2], stride[1], stride[0], pc)
+    return y
+
+
+def max_pool2d(input, kernel, stride, padding=SAME):
+    """owl_dense_ndarray_generic.ml:6905-6961."""
+    return _pool_fwd("max_pool2d", 2, True, input, kernel, stride, padding)
+
+
+def avg_pool2d(input, kernel, stride, padding=SAME):
+    return _pool_fwd("avg_pool2d", 2, False, input, kernel, stride, padding)
+
+
+def max_pool3d(input, kernel, stride, padding=SAME):
+    return _pool_fwd("max_pool3d", 3, True, input, kernel, stride, padding)
+
+
+def avg_pool3d(input, kernel, stride, padding=SAME):
+    return _pool_fwd("avg_pool3d", 3, False, input, kernel, stride, padding)
+
+
+def max_pool2d_argmax(input, kernel, stride, padding=SAME):
+    """owl_dense_ndarray_generic.ml:7517-7581: (output, int64 flat argmax)."""
+    _pool_check("max_pool2d_argmax", 2, input, kernel, stride)
+    x = _contig(input)
+    B, (H, W), C = x.shape[0], x.shape[1:3], x.shape[3]
+    Ho, Wo = (conv_out_dim(padding, n, k, s) for n, k, s in zip((H, W), kernel, stride))
+    top, left, _, _ = calc_conv2d_padding(H, W, kernel[0], kernel[1], Ho, Wo, stride[1], stride[0])
+    y = _empty_like_kind(x, [B, Ho, Wo, C])
+    if _is_torch(x):
+        arg = torch.empty([B, Ho, Wo, C], dtype=torch.int64, device=x.device)
+        d = _pool_desc(B, [H, W], C, kernel, [Ho, Wo], stride, [left, top])
+        _lib.check(_lib.lib().owl_amd_pool_argmax(ctypes.byref(d), _tdt(x), _ptr(x), _ptr(y), _ptr(arg),
+                                                  _stream()))
+        return y, arg
+    arg = np.empty([B, Ho, Wo, C], dtype=np.int64)
+    _lib.call(f"owl_amd_{_np_prefix(x)}_ndarray_maxpool_spatial_arg", _ptr(x), _ptr(y), _ptr(arg), B, H, W,
+              C, kernel[0], kernel[1], Ho, Wo, stride[1], stride[0], top, left)
+    return y, arg
+
+
+def _pool_bwd(name, nd, is_max, padding, x, kernel, stride, dy):
+    _pool_check(name, nd, x, kernel, stride)
+    _same_kind(x, dy)
+    x, dy = _contig(x), _contig(dy)
+    B, sp_in, C, sp_out, _ = _pool_geom(x, kernel, stride, padding)
+    _verify(list(dy.shape) == [B, *sp_out, C], f"{name}: output' shape {list(dy.shape)} does not match")
+    dx = _empty_like_kind(x, list(x.shape))
+    if nd == 2:
+        top, left, _, _ = calc_conv2d_padding(sp_in[0], sp_in[1], kernel[0], kernel[1], sp_out[0],
+                                              sp_out[1], stride[1], stride[0])
+        pads = [left, top]
+    else:
+        pads = [c_pad(s, o, k, n) for n, k, o, s in zip(sp_in, kernel, sp_out, stride)] \
+            if padding == SAME else [0, 0, 0]
+    if _is_torch(x):
+        d = _pool_desc(B, sp_in, C, kernel, sp_out, stride, pads)
+        _lib.check(_lib.lib().owl_amd_pool_bwd(ctypes.byref(d), _tdt(x), int(is_max), _ptr(x), _ptr(dy),
+                                               _ptr(dx), _stream()))
+        return dx
+    op = ("maxpool" if is_max else "avgpool") + ("_spatial" if nd == 2 else "_cuboid") + "_backward"
+    # Owl's average backward passes the output buffer as the dummy input
+    # (owl_ndarray_pool.ml:85-119)
+    src = x if is_max else dx
+    if nd == 2:
+        # owl_ndarray_pool_impl.h:102-108 (pad_rows = top, pad_cols = left)
+        _lib.call(f"owl_amd_{_np_prefix(x)}_ndarray_{op}", _ptr(src), _ptr(dy), _ptr(dx), B, sp_in[0],
+                  sp_in[1], C, kernel[0], kernel[1], sp_out[0], sp_out[1], stride[1], stride[0],
+                  pads[1], pads[0])
+    else:
+        # owl_ndarray_pool_impl.h:324-331 (strides col, row, dpt; padding code)
+        pc = 0 if padding == SAME else 1
+        _lib.call(f"owl_amd_{_np_prefix(x)}_ndarray_{op}", _ptr(src), _ptr(dy), _ptr(dx), B, *sp_in, C,
+                  *kernel, *sp_out, stride[0], stride[1], stride[2], pc)
+    return dx
+
+
+def max_pool2d_backward(padding, input, kernel, stride, output_grad):
+    """owl_dense_ndarray_generic.ml:7715-7775."""
+    return _pool_bwd("max_pool2d_backward", 2, True, padding, input, kernel, stride, output_grad)
+
+
+def avg_pool2d_backward(padding, input, kernel, stride, output_grad):
+    return _pool_bwd("avg_pool2d_backward", 2, False, padding, input, kernel, stride, output_grad)
+
+
+def max_pool3d_backward(padding, input, kernel, stride, output_grad):
+    return _pool_bwd("max_pool3d_backward", 3, True, padding, input, kernel, stride, output_grad)
+
+
+def avg_pool3d_backward(padding, input, kernel, stride, output_grad):
+    return _pool_bwd("avg_pool3d_backward", 3, False, padding, input, kernel, stride, output_grad)
