@@ -80,6 +80,20 @@ void owl_amd_cache_release(void);
 int owl_amd_conv_fwd(const owl_amd_conv_desc *d, int in_t, int out_t,
                      const void *x, const void *w, void *y, void *stream);
 
+/* FP8 e4m3 forward with per-tensor scales: x = x_q * x_scale, w = w_q * w_scale
+ * (the scales are what an amax pass produced; owl_amd_fp8_quantize below).
+ * out_t: F32 or BF16. */
+int owl_amd_conv_fwd_fp8(const owl_amd_conv_desc *d, int out_t, const void *xq,
+                         const void *wq, const float *x_scale, const float *w_scale,
+                         void *y, void *stream);
+
+/* Per-tensor amax -> scale = amax / 448 and e4m3 quantisation (saturating,
+ * round to nearest even).  src_t: F32 or BF16.  scale is a device float. */
+int owl_amd_fp8_quantize(int src_t, const void *src, int64_t n, void *dst_q,
+                         float *scale, void *stream);
+/* Plain float -> e4m3 encoding (no scaling), for conversion checks. */
+int owl_amd_fp8_encode(const float *src, int64_t n, void *dst, void *stream);
+
 /* dx = backward_input(w, dy): owl_ndarray_conv_impl.h:465-664, :1456-1554,
  * :3317-3406, :3678-3784.  Gather formulation, no atomics. */
 int owl_amd_conv_bwd_input(const owl_amd_conv_desc *d, int in_t, int out_t,

@@ -96,6 +96,9 @@ def lib():
             "owl_amd_pool_fwd": ([pd, i32, i32, vp, vp, vp], i32),
             "owl_amd_pool_argmax": ([pd, i32, vp, vp, vp, vp], i32),
             "owl_amd_pool_bwd": ([pd, i32, i32, vp, vp, vp, vp], i32),
+            "owl_amd_conv_fwd_fp8": ([cd, i32, vp, vp, vp, vp, vp, vp], i32),
+            "owl_amd_fp8_quantize": ([i32, vp, i64, vp, vp, vp], i32),
+            "owl_amd_fp8_encode": ([vp, i64, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name, None)

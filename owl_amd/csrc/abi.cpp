@@ -47,7 +47,9 @@ struct DeviceState {
   DevBuf slot[4];                // drop-in staging buffers
 };
 
-std::mutex g_mu;
+// Recursive: the drop-ins hold it while calling the device API, which takes it
+// again for the split-K workspace and the pooling flag.
+std::recursive_mutex g_mu;
 DeviceState g_dev[64];
 
 int current_device(int *dev) {
@@ -126,7 +128,7 @@ int check_conv(const owl_amd_conv_desc *d) {
 
 ConvParams make_params(const owl_amd_conv_desc *d) {
   ConvParams p;
-  memset(&p, 0, sizeof(p));
+  memset(&p, 0, sizeof(p));  // scale_a / scale_b null: not FP8
   p.B = (int)d->B; p.H = (int)d->H; p.W = (int)d->W; p.D = (int)d->D; p.Ci = (int)d->Ci;
   p.kh = (int)d->kh; p.kw = (int)d->kw; p.kd = (int)d->kd; p.Co = (int)d->Co;
   p.Ho = (int)d->Ho; p.Wo = (int)d->Wo; p.Do = (int)d->Do;
@@ -200,7 +202,7 @@ void plan_splitk(const ConvParams &p, int in_t, int *split, int64_t *kps) {
 
 // ------------------------------------------------------------------ init
 extern "C" int owl_amd_init(int device) {
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
   OWL_CHECK_HIP(hipSetDevice(device));
   DeviceState *s;
   return ensure_state(&s);
@@ -209,7 +211,7 @@ extern "C" int owl_amd_init(int device) {
 extern "C" int owl_amd_complex_supported(void) { return 0; }
 
 extern "C" int owl_amd_cache_stats(int64_t *n, int64_t *bytes) {
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
   int64_t cnt = 0, tot = 0;
   for (auto &s : g_dev) {
     if (!s.ready) continue;
@@ -223,7 +225,7 @@ extern "C" int owl_amd_cache_stats(int64_t *n, int64_t *bytes) {
 }
 
 extern "C" void owl_amd_cache_release(void) {
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
   for (auto &s : g_dev) {
     if (!s.ready) continue;
     (void)hipStreamSynchronize(s.stream);
@@ -315,7 +317,7 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   size_t need = owl_amd_conv_bwd_kernel_workspace(d, in_t);
   void *ws = workspace;
   if (!ws) {
-    std::lock_guard<std::mutex> lk(g_mu);
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
     DeviceState *s;
     rc = ensure_state(&s);
     if (rc) return rc;
@@ -333,6 +335,55 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   else if (out_t == OWL_AMD_F32) run_splitk_reduce<float, float>((const float *)ws, (float *)dw, n, p.split, st);
   else if (out_t == OWL_AMD_BF16) run_splitk_reduce<float, bf16_t>((const float *)ws, (bf16_t *)dw, n, p.split, st);
   else run_splitk_reduce<float, f16_t>((const float *)ws, (f16_t *)dw, n, p.split, st);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+// ------------------------------------------------------------------ device API: FP8
+extern "C" int owl_amd_fp8_quantize(int src_t, const void *src, int64_t n, void *dst_q, float *scale,
+                                    void *stream) {
+  if (n < 0) ARG_FAIL("negative element count");
+  if (n > 0 && (!src || !dst_q)) ARG_FAIL("null pointer");
+  if (!scale) ARG_FAIL("null scale pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (src_t == OWL_AMD_F32) run_fp8_quantize<float>((const float *)src, n, (unsigned char *)dst_q, scale, st);
+  else if (src_t == OWL_AMD_BF16) run_fp8_quantize<bf16_t>((const bf16_t *)src, n, (unsigned char *)dst_q, scale, st);
+  else {
+    owl_amd_set_error("fp8 quantize: source must be F32 or BF16");
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+extern "C" int owl_amd_fp8_encode(const float *src, int64_t n, void *dst, void *stream) {
+  if (n < 0 || (n > 0 && (!src || !dst))) ARG_FAIL("bad arguments");
+  if (n) run_e4m3_encode(src, n, (unsigned char *)dst, (hipStream_t)stream);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+extern "C" int owl_amd_conv_fwd_fp8(const owl_amd_conv_desc *d, int out_t, const void *xq, const void *wq,
+                                    const float *x_scale, const float *w_scale, void *y, void *stream) {
+  int rc = check_conv(d);
+  if (rc) return rc;
+  ConvParams p = make_params(d);
+  p.M = d->B * d->Ho * d->Wo * d->Do;
+  p.N = p.Co;
+  p.K = d->kh * d->kw * d->kd * d->Ci;
+  if (p.M == 0 || p.N == 0) return OWL_AMD_OK;
+  if (!xq || !wq || !y || !x_scale || !w_scale) ARG_FAIL("null pointer");
+  p.a = xq; p.b = wq; p.c = y; p.ldc = p.Co;
+  p.scale_a = x_scale;
+  p.scale_b = w_scale;
+  bool aligned = p.Ci % 16 == 0 && p.Co % 16 == 0 && aligned16(xq, wq, nullptr);
+  hipStream_t st = (hipStream_t)stream;
+  if (out_t == OWL_AMD_F32) run_igemm<0, fp8e4m3_t, float>(p, aligned, st);
+  else if (out_t == OWL_AMD_BF16) run_igemm<0, fp8e4m3_t, bf16_t>(p, aligned, st);
+  else {
+    owl_amd_set_error("fp8 forward: output must be F32 or BF16");
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
   OWL_CHECK_HIP(hipGetLastError());
   return OWL_AMD_OK;
 }
@@ -406,7 +457,7 @@ extern "C" int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max, c
   if (!x || !dy || !dx) ARG_FAIL("null pointer");
   int *flag = nullptr;
   {
-    std::lock_guard<std::mutex> lk(g_mu);
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
     DeviceState *s;
     rc = ensure_state(&s);
     if (rc) return rc;
@@ -443,7 +494,7 @@ struct HostArg {
 template <typename F>
 int run_dropin(std::initializer_list<HostArg> args, int out_slot, void *out_host, size_t out_bytes,
                F launch, int extra_out_slot = -1, void *extra_host = nullptr, size_t extra_bytes = 0) {
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
   DeviceState *s;
   int rc = ensure_state(&s);
   if (rc) return rc;

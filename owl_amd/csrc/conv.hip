@@ -32,6 +32,27 @@ namespace owl_amd {
 
 template <typename TI> struct Mfma;
 
+// E8M0 exponent that the scaled FP8 MFMA reads as 2^0 (the per-tensor scales
+// are applied in the epilogue instead).
+#ifndef OWL_FP8_MFMA_SCALE_ONE
+#define OWL_FP8_MFMA_SCALE_ONE 127
+#endif
+
+template <> struct Mfma<fp8e4m3_t> {
+  static constexpr int TM = 32, KPL = 32, BK = 64, BM = 128, BN = 128;
+  typedef float acc_elem;
+  typedef f32x16 acc_t;
+  static constexpr int NACC = 16;
+  __device__ static __forceinline__ acc_t mma(const fp8e4m3_t *a, const fp8e4m3_t *b, acc_t c) {
+    i32x8 av, bv;
+    __builtin_memcpy(&av, a, 32);
+    __builtin_memcpy(&bv, b, 32);
+    // cbsz = blgp = 0: both operands OCP FP8 e4m3
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 0, 0, 0, OWL_FP8_MFMA_SCALE_ONE, 0,
+                                                           OWL_FP8_MFMA_SCALE_ONE);
+  }
+};
+
 template <> struct Mfma<bf16_t> {
   static constexpr int TM = 32, KPL = 8, BK = 32, BM = 128, BN = 128;
   typedef float acc_elem;
@@ -83,8 +104,10 @@ template <int TM> __device__ __forceinline__ int acc_row(int lane, int r);
 template <> __device__ __forceinline__ int acc_row<32>(int lane, int r) {
   return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 }
+// v_mfma_f64_16x16x4f64 (measured on gfx950): lane l, register r holds
+// D[l / 16 + 4 r][l % 16].
 template <> __device__ __forceinline__ int acc_row<16>(int lane, int r) {
-  return 4 * (lane >> 4) + r;
+  return (lane >> 4) + 4 * r;
 }
 
 constexpr int NT = 256;  // threads per workgroup (4 waves, 2 x 2)
@@ -115,6 +138,7 @@ template <typename TI> struct Tile {
 };
 
 template <typename T> __device__ __forceinline__ T zero_of() { return (T)0.0f; }
+template <> __device__ __forceinline__ fp8e4m3_t zero_of<fp8e4m3_t>() { return (fp8e4m3_t)0; }
 
 // Decode a tap index t into (a, c, e) for a [kh][kw][kd] kernel.
 __device__ __forceinline__ void tap_decode(int t, int kw, int kd, int &a, int &c, int &e) {
@@ -489,6 +513,8 @@ __global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p) {
 
   // ---------------- epilogue ----------------
   const int col = lane % MF::TM;
+  // FP8: undo the per-tensor quantisation scales of both operands
+  const float oscale = p.scale_a ? (*p.scale_a) * (*p.scale_b) : 1.0f;
 #pragma unroll
   for (int i = 0; i < TL::FM; ++i)
 #pragma unroll
@@ -504,7 +530,10 @@ __global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p) {
           TA *ws = (TA *)p.c + (int64_t)blockIdx.y * M * N;
           ws[m * N + n] = (TA)acc[i][j][r];
         } else {
-          ((TO *)p.c)[m * p.ldc + n] = (TO)acc[i][j][r];
+          if (p.scale_a)
+            ((TO *)p.c)[m * p.ldc + n] = (TO)((float)acc[i][j][r] * oscale);
+          else
+            ((TO *)p.c)[m * p.ldc + n] = (TO)acc[i][j][r];
         }
       }
     }
@@ -580,7 +609,11 @@ OWL_INST(2, f16_t, f16_t)
 OWL_INST(0, f16_t, float)
 OWL_INST(1, f16_t, float)
 OWL_INST(2, f16_t, float)
+OWL_INST(0, fp8e4m3_t, float)
+OWL_INST(0, fp8e4m3_t, bf16_t)
 #undef OWL_INST
+template int tile_bm<fp8e4m3_t>();
+template int tile_bk<fp8e4m3_t>();
 template void run_splitk_reduce<float, float>(const float *, float *, int64_t, int, hipStream_t);
 template void run_splitk_reduce<float, bf16_t>(const float *, bf16_t *, int64_t, int, hipStream_t);
 template void run_splitk_reduce<float, f16_t>(const float *, f16_t *, int64_t, int, hipStream_t);

@@ -19,6 +19,8 @@ struct ConvParams {
   const void *b;  // forward: w, backward input: w,  backward kernel: dy
   void *c;        // y, dx, dw (or split-K workspace)
   int64_t ldc;
+  const float *scale_a;  // FP8 forward: per-tensor scales of x and w (else null)
+  const float *scale_b;
   int split;            // backward kernel: number of K splits
   int64_t k_per_split;  // multiple of the K-step
 };
@@ -27,6 +29,9 @@ template <int PASS, typename TI, typename TO>
 void run_igemm(const ConvParams &p, bool aligned, hipStream_t st);
 template <typename TA, typename TO>
 void run_splitk_reduce(const TA *ws, TO *out, int64_t n, int splits, hipStream_t st);
+template <typename T>
+void run_fp8_quantize(const T *src, int64_t n, unsigned char *dst, float *scale, hipStream_t st);
+void run_e4m3_encode(const float *src, int64_t n, unsigned char *dst, hipStream_t st);
 template <typename TI> int tile_bm();
 template <typename TI> int tile_bk();
 

@@ -7,6 +7,8 @@
 
 typedef __bf16 bf16_t;
 typedef _Float16 f16_t;
+// OCP FP8 e4m3 (e4m3fn: bias 7, max 448, 0x7f = NaN), stored as a byte.
+enum class fp8e4m3_t : unsigned char {};
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
