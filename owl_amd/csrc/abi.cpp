@@ -2,6 +2,7 @@
 // split-K planning, and the Owl drop-in entry points that own device buffers.
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <mutex>
@@ -165,6 +166,38 @@ int dispatch_igemm(const ConvParams &p, int in_t, int out_t, bool aligned, hipSt
 
 int vec_of(int t) { return 16 / (int)dtype_size(t); }
 
+// Kernel family selection: OWL_AMD_CONV_PATH=generic forces the LDS-staged
+// kernel of conv.hip everywhere (used to cross-check the two families).
+bool fast_path_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("OWL_AMD_CONV_PATH");
+    v = (e && strcmp(e, "generic") == 0) ? 0 : 1;
+  }
+  return v == 1;
+}
+
+int fast_shape_for(int N) { return N <= 64 ? 0 : (N <= 128 ? 1 : 2); }
+
+// Forward / backward-input on the conv_fast.hip kernels when the problem
+// fits them; returns false (nothing launched) otherwise.
+template <int PASS>
+bool try_fast(const ConvParams &p, int in_t, int out_t, bool aligned, hipStream_t st) {
+  if (!fast_path_enabled() || !aligned) return false;
+  if (in_t != OWL_AMD_F32 && in_t != OWL_AMD_BF16 && in_t != OWL_AMD_F16) return false;
+  const int C = PASS == 0 ? p.Ci : p.Co;
+  if (C % fast_bk((int)dtype_size(in_t)) != 0) return false;
+  if (PASS == 1 && (p.sh != 1 || p.sw != 1 || p.sd != 1)) return false;
+  const int shape = fast_shape_for(p.N);
+  if (in_t == OWL_AMD_F32 && out_t == OWL_AMD_F32) run_fast<PASS, float, float>(p, shape, st);
+  else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) run_fast<PASS, bf16_t, bf16_t>(p, shape, st);
+  else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_F32) run_fast<PASS, bf16_t, float>(p, shape, st);
+  else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F16) run_fast<PASS, f16_t, f16_t>(p, shape, st);
+  else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F32) run_fast<PASS, f16_t, float>(p, shape, st);
+  else return false;
+  return true;
+}
+
 int tile_k(int t) {
   switch (t) {
     case OWL_AMD_F32: return tile_bk<float>();
@@ -183,13 +216,13 @@ int tile_m(int t) {
 }
 
 // Split-K plan for the backward-kernel GEMM (M = T*Ci, N = Co, K = pixels):
-// enough workgroups to fill 256 CUs several times over, each split a
-// multiple of the K-step.
+// about two workgroups per CU (256 CUs), each split at least 16 K-steps long,
+// so the FP32 partials stay a small fraction of the inputs' bytes.
 void plan_splitk(const ConvParams &p, int in_t, int *split, int64_t *kps) {
   const int bm = tile_m(in_t), bk = tile_k(in_t);
   int64_t tiles = ((p.M + bm - 1) / bm) * (int64_t)((p.N + bm - 1) / bm);
-  int64_t want = (1024 + tiles - 1) / tiles;
-  int64_t max_split = (p.K + 4 * bk - 1) / (4 * bk);  // at least 4 K-steps per split
+  int64_t want = (512 + tiles - 1) / tiles;
+  int64_t max_split = (p.K + 16 * bk - 1) / (16 * bk);
   if (want > max_split) want = max_split;
   if (want < 1) want = 1;
   int64_t k = (p.K + want - 1) / want;
@@ -252,7 +285,11 @@ extern "C" int owl_amd_conv_fwd(const owl_amd_conv_desc *d, int in_t, int out_t,
   if (!x || !w || !y) ARG_FAIL("null pointer");
   p.a = x; p.b = w; p.c = y; p.ldc = p.Co;
   const int v = vec_of(in_t);
-  bool aligned = dtype_size(in_t) && p.Ci % v == 0 && p.Co % v == 0 && aligned16(x, w, nullptr);
+  bool aligned = dtype_size(in_t) && p.Ci % v == 0 && p.Co % v == 0 && aligned16(x, w, y);
+  if (try_fast<0>(p, in_t, out_t, aligned, (hipStream_t)stream)) {
+    OWL_CHECK_HIP(hipGetLastError());
+    return OWL_AMD_OK;
+  }
   return dispatch_igemm<0>(p, in_t, out_t, aligned, (hipStream_t)stream);
 }
 
@@ -268,7 +305,12 @@ extern "C" int owl_amd_conv_bwd_input(const owl_amd_conv_desc *d, int in_t, int 
   if (!w || !dy || !dx) ARG_FAIL("null pointer");
   p.a = dy; p.b = w; p.c = dx; p.ldc = p.Ci;
   const int v = vec_of(in_t);
-  bool aligned = dtype_size(in_t) && p.Co % v == 0 && aligned16(dy, w, nullptr);
+  bool aligned = dtype_size(in_t) && p.Co % v == 0 && p.Ci % 4 == 0 && aligned16(dy, w, dx);
+  if (try_fast<1>(p, in_t, out_t, aligned, (hipStream_t)stream)) {
+    OWL_CHECK_HIP(hipGetLastError());
+    return OWL_AMD_OK;
+  }
+  aligned = dtype_size(in_t) && p.Co % v == 0 && aligned16(dy, w, nullptr);
   return dispatch_igemm<1>(p, in_t, out_t, aligned, (hipStream_t)stream);
 }
 

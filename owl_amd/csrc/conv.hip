@@ -539,25 +539,50 @@ __global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p) {
     }
 }
 
-// Deterministic split-K reduction: dw[e] = sum over splits in order, carried
-// with Neumaier compensation in the accumulator type (FP32, or FP64 for the
-// FP64 path).
+// Deterministic split-K reduction: dw[e] = sum over splits in split order,
+// carried with Neumaier compensation in the accumulator type (FP32, or FP64
+// for the FP64 path).  Each thread owns 4 consecutive elements and keeps
+// several splits' loads in flight.
+template <typename TA>
+__device__ __forceinline__ void neumaier(TA &s, TA &comp, TA v) {
+  TA t = s + v;
+  if (fabs(s) >= fabs(v))
+    comp += (s - t) + v;
+  else
+    comp += (v - t) + s;
+  s = t;
+}
+
 template <typename TA, typename TO>
-__global__ void splitk_reduce_kernel(const TA *__restrict__ ws, TO *__restrict__ out,
-                                     int64_t n, int splits) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  TA s = 0, comp = 0;
-  for (int k = 0; k < splits; ++k) {
-    TA v = ws[(int64_t)k * n + e];
-    TA t = s + v;
-    if (fabs(s) >= fabs(v))
-      comp += (s - t) + v;
-    else
-      comp += (v - t) + s;
-    s = t;
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const TA *__restrict__ ws, TO *__restrict__ out,
+                                                            int64_t n, int splits) {
+  int64_t e0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (e0 >= n) return;
+  TA s[4] = {0, 0, 0, 0}, comp[4] = {0, 0, 0, 0};
+  if (e0 + 4 <= n && (n % 4) == 0) {
+    int k = 0;
+    for (; k + 4 <= splits; k += 4) {
+      TA v[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[u][q] = ws[(int64_t)(k + u) * n + e0 + q];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) neumaier(s[q], comp[q], v[u][q]);
+    }
+    for (; k < splits; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) neumaier(s[q], comp[q], ws[(int64_t)k * n + e0 + q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[e0 + q] = (TO)(s[q] + comp[q]);
+  } else {
+    for (int q = 0; q < 4 && e0 + q < n; ++q) {
+      for (int k = 0; k < splits; ++k) neumaier(s[q], comp[q], ws[(int64_t)k * n + e0 + q]);
+      out[e0 + q] = (TO)(s[q] + comp[q]);
+    }
   }
-  out[e] = (TO)(s + comp);
 }
 
 // ---------------------------------------------------------------------------
@@ -584,7 +609,7 @@ void run_igemm(const ConvParams &p, bool aligned, hipStream_t st) {
 
 template <typename TA, typename TO>
 void run_splitk_reduce(const TA *ws, TO *out, int64_t n, int splits, hipStream_t st) {
-  int64_t blocks = (n + 255) / 256;
+  int64_t blocks = (n + 1023) / 1024;
   hipLaunchKernelGGL((splitk_reduce_kernel<TA, TO>), dim3((unsigned)blocks), dim3(256), 0, st,
                      ws, out, n, splits);
 }

@@ -32,6 +32,12 @@ void run_splitk_reduce(const TA *ws, TO *out, int64_t n, int splits, hipStream_t
 template <typename T>
 void run_fp8_quantize(const T *src, int64_t n, unsigned char *dst, float *scale, hipStream_t st);
 void run_e4m3_encode(const float *src, int64_t n, unsigned char *dst, hipStream_t st);
+// conv_fast.hip: forward / unit-stride backward input with A fragments read
+// straight from global memory.  shape: 0 -> 256x64, 1 -> 256x128,
+// 2 -> 128x256, 3 -> 128x128 (BM x BN).
+template <int PASS, typename TI, typename TO>
+void run_fast(const ConvParams &p, int shape, hipStream_t st);
+int fast_bk(int elem_bytes);
 template <typename TI> int tile_bm();
 template <typename TI> int tile_bk();
 

@@ -179,6 +179,65 @@ __global__ __launch_bounds__(256) void pool_bwd_kernel(PoolParams p, const T *__
   __builtin_memcpy(dx + my_id, acc, sizeof(acc));
 }
 
+// Max backward when the windows tile the input exactly (window == stride,
+// no padding, every cell in exactly one window): one thread per window and
+// VEC channels reads the window once and writes all of its cells (dy at the
+// first strict max, 0 elsewhere).  Each cell then receives Owl's single
+// addition 0 + dy, so the result is the same bits as the gather kernel.
+template <typename T, int VEC>
+__global__ __launch_bounds__(256) void maxpool_bwd_tiled_kernel(PoolParams p, const T *__restrict__ x,
+                                                                const T *__restrict__ dy,
+                                                                T *__restrict__ dx, int *orphan_flag) {
+  typedef typename PoolAcc<T>::type A;
+  const int CV = p.C / VEC;
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)p.B * p.Ho * p.Wo * p.Do * CV;
+  if (gid >= total) return;
+  int cv = (int)(gid % CV);
+  int64_t t = gid / CV;
+  int l = (int)(t % p.Do); t /= p.Do;
+  int j = (int)(t % p.Wo); t /= p.Wo;
+  int i = (int)(t % p.Ho);
+  int64_t b = t / p.Ho;
+  const int c0 = cv * VEC;
+  A best[VEC];
+  int widx[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) {
+    best[v] = neg_inf<A>();
+    widx[v] = -1;
+  }
+  int n = 0;
+  for (int h = i * p.sh; h < i * p.sh + p.kh; ++h)
+    for (int w = j * p.sw; w < j * p.sw + p.kw; ++w)
+      for (int d = l * p.sd; d < l * p.sd + p.kd; ++d, ++n) {
+        int64_t id = (((b * p.H + h) * (int64_t)p.W + w) * p.D + d) * (int64_t)p.C + c0;
+        T tv[VEC];
+        __builtin_memcpy(tv, x + id, sizeof(tv));
+#pragma unroll
+        for (int v = 0; v < VEC; ++v)
+          if (best[v] < (A)tv[v]) {
+            best[v] = (A)tv[v];
+            widx[v] = n;
+          }
+      }
+  T g[VEC];
+  __builtin_memcpy(g, dy + gid * VEC, sizeof(g));
+#pragma unroll
+  for (int v = 0; v < VEC; ++v)
+    if (widx[v] < 0) *orphan_flag = 1;  // Owl sends this gradient to index 0
+  n = 0;
+  for (int h = i * p.sh; h < i * p.sh + p.kh; ++h)
+    for (int w = j * p.sw; w < j * p.sw + p.kw; ++w)
+      for (int d = l * p.sd; d < l * p.sd + p.kd; ++d, ++n) {
+        int64_t id = (((b * p.H + h) * (int64_t)p.W + w) * p.D + d) * (int64_t)p.C + c0;
+        T o[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) o[v] = widx[v] == n ? (T)((T)0 + g[v]) : (T)0;
+        __builtin_memcpy(dx + id, o, sizeof(o));
+      }
+}
+
 // Rebuild dx[0] in Owl's order when some window had no max candidate.  One
 // workgroup walks the windows in Owl's loop order (b, i, j, l, c) 256 at a
 // time; thread 0 adds the flagged gradients in that order.
@@ -271,7 +330,15 @@ void run_pool_bwd(const PoolParams &p, bool is_max, const T *x, const T *dy, T *
   int64_t total = (int64_t)p.B * p.H * p.W * p.D * (p.C / vec);
   dim3 grid((unsigned)((total + 255) / 256)), block(256);
   constexpr int V = 16 / (int)sizeof(T);
-  if (is_max) {
+  const bool tiled = p.kh == p.sh && p.kw == p.sw && p.kd == p.sd && p.ph == 0 && p.pw == 0 &&
+                     p.pd == 0 && p.H == p.Ho * p.sh && p.W == p.Wo * p.sw && p.D == p.Do * p.sd;
+  if (is_max && tiled) {
+    int64_t wt = (int64_t)p.B * p.Ho * p.Wo * p.Do * (p.C / vec);
+    dim3 g2((unsigned)((wt + 255) / 256));
+    if (vec == V) hipLaunchKernelGGL((maxpool_bwd_tiled_kernel<T, V>), g2, block, 0, st, p, x, dy, dx, flag);
+    else hipLaunchKernelGGL((maxpool_bwd_tiled_kernel<T, 1>), g2, block, 0, st, p, x, dy, dx, flag);
+    hipLaunchKernelGGL((pool_bwd_fixup_kernel<T>), dim3(1), dim3(256), 0, st, p, x, dy, dx, flag);
+  } else if (is_max) {
     if (vec == V) hipLaunchKernelGGL((pool_bwd_kernel<T, V, true>), grid, block, 0, st, p, x, dy, dx, flag);
     else hipLaunchKernelGGL((pool_bwd_kernel<T, 1, true>), grid, block, 0, st, p, x, dy, dx, flag);
     hipLaunchKernelGGL((pool_bwd_fixup_kernel<T>), dim3(1), dim3(256), 0, st, p, x, dy, dx, flag);

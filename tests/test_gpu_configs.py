@@ -1,0 +1,124 @@
+"""The benchmark configurations at their full sizes.
+
+The FP64 oracle is too slow for whole layers, so:
+  * forward and backward-input are checked on a 2-image slice (each output
+    image depends on its own input image only);
+  * backward-kernel is checked on the same slice by running it on the slice;
+  * the full-size tensors are checked through the adjoint identity
+    <conv(x,w), g> = <x, dgrad(w,g)> = <w, wgrad(x,g)>, which holds for any
+    size, evaluated in FP64 on the host from the GPU's outputs.
+Tolerances: FP32 rel-L2 1e-5 against FP64; BF16 1e-2."""
+import numpy as np
+import pytest
+
+import oracle_ndarray as on
+from owl_amd import ndarray as nd
+from owl_amd.shapes import SAME, VALID
+
+torch = pytest.importorskip("torch")
+DEV = "cuda:0"
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def layer(xs, ks, stride, dtype, seed, uniform=False):
+    g = torch.Generator(device=DEV)
+    g.manual_seed(seed)
+    if uniform:
+        x = torch.rand(xs, device=DEV, generator=g) * 2 - 1
+        w = torch.rand(ks, device=DEV, generator=g) * 2 - 1
+    else:
+        x = torch.randn(xs, device=DEV, generator=g)
+        fan = int(np.prod(ks[:-1]))
+        w = torch.randn(ks, device=DEV, generator=g) * (2.0 / fan) ** 0.5
+    return x.to(dtype).contiguous(), w.to(dtype).contiguous(), g
+
+
+def h64(t):
+    return t.double().cpu().numpy()
+
+
+@pytest.mark.gpu
+def test_config1_conv2d_forward_fp32():
+    """[8,32,32,3] (*) [3,3,3,16] SAME, uniform[-1,1]: whole output vs FP64."""
+    x, w, _ = layer([8, 32, 32, 3], [3, 3, 3, 16], [1, 1], torch.float32, 1, uniform=True)
+    y = nd.conv2d(x, w, [1, 1], SAME)
+    r = on.conv2d(h64(x), h64(w), [1, 1], SAME)
+    assert tuple(y.shape) == (8, 32, 32, 16)
+    assert rel(h64(y), r) < 1e-5
+    assert np.abs(h64(y) - r).max() < 1e-5 * np.abs(r).max() * 10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-5), (torch.bfloat16, 1e-2)], ids=["fp32", "bf16"])
+def test_config2_vgg_layer(dtype, tol):
+    x, w, g = layer([64, 56, 56, 64], [3, 3, 64, 128], [1, 1], dtype, 2)
+    dy = torch.randn([64, 56, 56, 128], device=DEV, generator=g).to(dtype)
+    y = nd.conv2d(x, w, [1, 1], SAME)
+    dx = nd.conv2d_backward_input(x, w, [1, 1], dy)
+    dw = nd.conv2d_backward_kernel(x, w, [1, 1], dy, out_dtype=torch.float32)
+    torch.cuda.synchronize()
+    n = 2
+    xs, ws, gs = h64(x[:n]), h64(w), h64(dy[:n])
+    assert rel(h64(y[:n]), on.conv2d(xs, ws, [1, 1], SAME)) < tol
+    assert rel(h64(dx[:n]), on.conv2d_backward_input(xs, ws, [1, 1], gs)) < tol
+    dws = nd.conv2d_backward_kernel(x[:n].contiguous(), w, [1, 1], dy[:n].contiguous(), out_dtype=torch.float32)
+    assert rel(h64(dws), on.conv2d_backward_kernel(xs, ws, [1, 1], gs)) < tol
+    # full size: adjoint identity in FP64 from the GPU outputs
+    yd, dyd = h64(y), h64(dy)
+    lhs = float((yd * dyd).sum())
+    assert abs(lhs - float((h64(x) * h64(dx)).sum())) < 10 * tol * np.abs(yd * dyd).sum() ** 1
+    assert abs(lhs - float((h64(w) * h64(dw)).sum())) < 10 * tol * np.abs(yd * dyd).sum()
+
+
+@pytest.mark.gpu
+def test_config2_maxpool_bit_exact():
+    x, w, g = layer([64, 56, 56, 64], [3, 3, 64, 128], [1, 1], torch.float32, 3)
+    y = nd.conv2d(x, w, [1, 1], SAME)
+    p, a = nd.max_pool2d_argmax(y, [2, 2], [2, 2], VALID)
+    gp = torch.randn(tuple(p.shape), device=DEV, generator=g)
+    dx = nd.max_pool2d_backward(VALID, y, [2, 2], [2, 2], gp)
+    yh = y.cpu().numpy()
+    rp, ra = on.max_pool2d_argmax(yh, [2, 2], [2, 2], VALID)
+    assert np.array_equal(p.cpu().numpy(), rp)
+    assert np.array_equal(a.cpu().numpy(), ra)
+    assert np.array_equal(dx.cpu().numpy(), on.max_pool2d_backward(VALID, yh, [2, 2], [2, 2], gp.cpu().numpy()))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-5), (torch.bfloat16, 1e-2), (torch.float16, 1e-2)],
+                         ids=["fp32", "bf16", "fp16"])
+def test_config3_resnet_stack(dtype, tol):
+    """[128,28,28,256]: 3x3 256->256 stride 1 SAME, then 1x1 256->512 stride 2."""
+    x, w, g = layer([128, 28, 28, 256], [3, 3, 256, 256], [1, 1], dtype, 4)
+    w2 = (torch.randn([1, 1, 256, 512], device=DEV, generator=g) * (2.0 / 256) ** 0.5).to(dtype)
+    n = 2
+    for ww, stride in ((w, [1, 1]), (w2, [2, 2])):
+        y = nd.conv2d(x, ww, stride, SAME)
+        dy = torch.randn(tuple(y.shape), device=DEV, generator=g).to(dtype)
+        dx = nd.conv2d_backward_input(x, ww, stride, dy)
+        xs, ws, gs = h64(x[:n]), h64(ww), h64(dy[:n])
+        assert rel(h64(y[:n]), on.conv2d(xs, ws, stride, SAME)) < tol
+        assert rel(h64(dx[:n]), on.conv2d_backward_input(xs, ws, stride, gs)) < tol
+        dws = nd.conv2d_backward_kernel(x[:n].contiguous(), ww, stride, dy[:n].contiguous(),
+                                        out_dtype=torch.float32)
+        assert rel(h64(dws), on.conv2d_backward_kernel(xs, ws, stride, gs)) < tol
+
+
+@pytest.mark.gpu
+def test_config5_conv3d_stack_fp32():
+    """conv3d [8,32,32,32,16] (*) [3,3,3,16,32] SAME then (*) [3,3,3,32,64], avg_pool3d 2x2x2."""
+    x, w1, g = layer([8, 32, 32, 32, 16], [3, 3, 3, 16, 32], [1, 1, 1], torch.float32, 5)
+    w2 = (torch.randn([3, 3, 3, 32, 64], device=DEV, generator=g) * (2.0 / 864) ** 0.5)
+    y1 = nd.conv3d(x, w1, [1, 1, 1], SAME)
+    y2 = nd.conv3d(y1, w2, [1, 1, 1], SAME)
+    p = nd.avg_pool3d(y2, [2, 2, 2], [2, 2, 2], VALID)
+    n = 1
+    r1 = on.conv3d(h64(x[:n]), h64(w1), [1, 1, 1], SAME)
+    assert rel(h64(y1[:n]), r1) < 1e-5
+    r2 = on.conv3d(h64(y1[:n]), h64(w2), [1, 1, 1], SAME)
+    assert rel(h64(y2[:n]), r2) < 1e-5
+    y2h = y2[:n].cpu().numpy()
+    assert np.array_equal(p[:n].cpu().numpy(), on.avg_pool3d(y2h, [2, 2, 2], [2, 2, 2], VALID))
