@@ -179,6 +179,18 @@ bool fast_path_enabled() {
 
 int fast_shape_for(int N) { return N <= 64 ? 0 : (N <= 128 ? 1 : 2); }
 
+int env_int(const char *name, int dflt) {
+  const char *e = getenv(name);
+  return e ? atoi(e) : dflt;
+}
+
+// A-prefetch depth of the fast kernels (OWL_AMD_FAST_PD overrides).
+int fast_pd() {
+  static int v = -1;
+  if (v < 0) v = env_int("OWL_AMD_FAST_PD", 2);
+  return v;
+}
+
 // Forward / backward-input on the conv_fast.hip kernels when the problem
 // fits them; returns false (nothing launched) otherwise.
 template <int PASS>
@@ -188,12 +200,20 @@ bool try_fast(const ConvParams &p, int in_t, int out_t, bool aligned, hipStream_
   const int C = PASS == 0 ? p.Ci : p.Co;
   if (C % fast_bk((int)dtype_size(in_t)) != 0) return false;
   if (PASS == 1 && (p.sh != 1 || p.sw != 1 || p.sd != 1)) return false;
+  // 32-bit byte offsets: operands and the GEMM rows must stay below 2 GiB
+  const int64_t es = (int64_t)dtype_size(in_t);
+  const int64_t a_elems = PASS == 0 ? (int64_t)p.B * p.H * p.W * p.D * p.Ci
+                                    : (int64_t)p.B * p.Ho * p.Wo * p.Do * p.Co;
+  const int64_t c_elems = p.M * (int64_t)p.ldc;
+  if (a_elems * es >= (1LL << 31) || (int64_t)p.kh * p.kw * p.kd * p.Ci * p.Co * es >= (1LL << 31) ||
+      c_elems * (int64_t)dtype_size(out_t) >= (1LL << 31))
+    return false;
   const int shape = fast_shape_for(p.N);
-  if (in_t == OWL_AMD_F32 && out_t == OWL_AMD_F32) run_fast<PASS, float, float>(p, shape, st);
-  else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) run_fast<PASS, bf16_t, bf16_t>(p, shape, st);
-  else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_F32) run_fast<PASS, bf16_t, float>(p, shape, st);
-  else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F16) run_fast<PASS, f16_t, f16_t>(p, shape, st);
-  else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F32) run_fast<PASS, f16_t, float>(p, shape, st);
+  if (in_t == OWL_AMD_F32 && out_t == OWL_AMD_F32) run_fast<PASS, float, float>(p, shape, fast_pd(), st);
+  else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) run_fast<PASS, bf16_t, bf16_t>(p, shape, fast_pd(), st);
+  else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_F32) run_fast<PASS, bf16_t, float>(p, shape, fast_pd(), st);
+  else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F16) run_fast<PASS, f16_t, f16_t>(p, shape, fast_pd(), st);
+  else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F32) run_fast<PASS, f16_t, float>(p, shape, fast_pd(), st);
   else return false;
   return true;
 }

@@ -196,8 +196,10 @@ __global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p) {
   int a_q0[AR], a_q1[AR], a_q2[AR];  // spatial origin of the row's window
   bool a_ok[AR];
   // A operand, pass 2 (MN-contiguous over m = (tap, ci)); B operand pass 0/2.
-  const int rg = tid % TL::MN_RG;
-  const int kgp = tid / TL::MN_RG;
+  // k-groups vary fastest across lanes: the transposed LDS writes of one
+  // instruction then fill consecutive bytes of a row (no bank conflicts).
+  const int kgp = tid % TL::MN_KGR;
+  const int rg = tid / TL::MN_KGR;
 
   if (PASS == 0 || PASS == 1) {
 #pragma unroll

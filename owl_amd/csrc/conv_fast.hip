@@ -53,15 +53,29 @@ __device__ __forceinline__ int xcd_swizzle(int bid, int nblocks) {
   return (bid % 8) * (nblocks / 8) + bid / 8;
 }
 
-// PASS 0: forward.  PASS 1: backward input with unit strides.
-template <int PASS, typename TI, typename TO, int FM, int FN, int WGM, int WGN>
+// Raw buffer resource over [base, base + bytes): loads at offsets >= bytes
+// return zeros, which is how padding taps read their zeros.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+constexpr uint32_t OOB = 0x80000000u;  // >= every num_records used here (tensors < 2 GiB)
+
+__device__ __forceinline__ uint4 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+  return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0));
+}
+
+// PASS 0: forward.  PASS 1: backward input with unit strides.  PD: how many
+// K-steps of A fragments are in flight ahead of the MFMAs (register ring of
+// PD + 1 slots; the loop is unrolled by PD + 1 so slots are static).
+template <int PASS, typename TI, typename TO, int FM, int FN, int WGM, int WGN, int PD>
 __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
   typedef FastMfma<TI> MF;
   constexpr int BK = MF::BK;
-  constexpr int LK = BK / 2;                   // k values per lane per fragment per K-step (32 bytes)
+  constexpr int LK = BK / 2;                      // k values per lane per fragment per K-step (32 bytes)
   constexpr int BM = WGM * FM * 32, BN = WGN * FN * 32;
   constexpr int LDR = BK + 16 / (int)sizeof(TI);  // LDS row, padded by 16 bytes
-  constexpr int BELEMS = BK * BN / 256;        // B elements per thread per K-step
+  constexpr int BELEMS = BK * BN / 256;           // B elements per thread per K-step
+  constexpr int ES = (int)sizeof(TI);
   static_assert(WGM * WGN == 4, "four waves");
   static_assert(LK * sizeof(TI) == 32, "32-byte fragments");
 
@@ -70,11 +84,10 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WGN, wn = wave % WGN;
   const int N = p.N;
-  const int64_t M = p.M;
+  const int M = (int)p.M;
   const int n_tiles = (N + BN - 1) / BN;
-  const int nblocks = gridDim.x;
-  const int bid = xcd_swizzle(blockIdx.x, nblocks);
-  const int64_t m0 = (int64_t)(bid / n_tiles) * BM;
+  const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int m0 = (bid / n_tiles) * BM;
   const int n0 = (bid % n_tiles) * BN;
 
   const int C = PASS == 0 ? p.Ci : p.Co;  // contracted channels per tap
@@ -82,39 +95,43 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
   const int csteps = C / BK;
   const int nk = T * csteps;
 
-  const TI *__restrict__ Ag = (const TI *)p.a;
+  // A source: x (forward) or dy (backward input), addressed in bytes
+  const uint32_t a_bytes = PASS == 0 ? (uint32_t)((int64_t)p.B * p.H * p.W * p.D * p.Ci * ES)
+                                     : (uint32_t)((int64_t)p.B * p.Ho * p.Wo * p.Do * p.Co * ES);
+  const __amdgpu_buffer_rsrc_t arsrc = make_rsrc(p.a, a_bytes);
   const TI *__restrict__ Bg = (const TI *)p.b;
 
   // ---- per-lane A rows (one per M fragment) ----
+  // row_off: element offset of the row's window origin (may be negative for
+  // padded origins; only in-range taps are ever dereferenced).
   const int frow = lane & 31, fkg = lane >> 5;
-  int64_t a_base[FM];
-  int a_q0[FM], a_q1[FM], a_q2[FM];
-  bool a_ok[FM];
+  int row_off[FM], q0[FM], q1[FM], q2[FM];
 #pragma unroll
   for (int f = 0; f < FM; ++f) {
-    int64_t m = m0 + wm * FM * 32 + f * 32 + frow;
-    a_ok[f] = m < M;
-    if (!a_ok[f]) m = 0;
-    int64_t t = m;
+    int m = m0 + wm * FM * 32 + f * 32 + frow;
+    const bool ok = m < M;
+    if (!ok) m = 0;
+    int t = m;
     if (PASS == 0) {
-      int l = (int)(t % p.Do); t /= p.Do;
-      int j = (int)(t % p.Wo); t /= p.Wo;
-      int i = (int)(t % p.Ho);
-      int64_t b = t / p.Ho;
-      a_base[f] = b * (int64_t)p.H * p.W * p.D * p.Ci + fkg * LK;
-      a_q0[f] = i * p.sh - p.ph;
-      a_q1[f] = j * p.sw - p.pw;
-      a_q2[f] = l * p.sd - p.pd;
+      int l = t % p.Do; t /= p.Do;
+      int j = t % p.Wo; t /= p.Wo;
+      int i = t % p.Ho;
+      int b = t / p.Ho;
+      q0[f] = i * p.sh - p.ph;
+      q1[f] = j * p.sw - p.pw;
+      q2[f] = l * p.sd - p.pd;
+      row_off[f] = ((b * p.H + q0[f]) * p.W + q1[f]) * p.D * p.Ci + q2[f] * p.Ci + fkg * LK;
     } else {
-      int d = (int)(t % p.D); t /= p.D;
-      int w = (int)(t % p.W); t /= p.W;
-      int h = (int)(t % p.H);
-      int64_t b = t / p.H;
-      a_base[f] = b * (int64_t)p.Ho * p.Wo * p.Do * p.Co + fkg * LK;
-      a_q0[f] = h + p.ph;
-      a_q1[f] = w + p.pw;
-      a_q2[f] = d + p.pd;
+      int d = t % p.D; t /= p.D;
+      int w = t % p.W; t /= p.W;
+      int h = t % p.H;
+      int b = t / p.H;
+      q0[f] = h + p.ph;
+      q1[f] = w + p.pw;
+      q2[f] = d + p.pd;
+      row_off[f] = ((b * p.Ho + q0[f]) * p.Wo + q1[f]) * p.Do * p.Co + q2[f] * p.Co + fkg * LK;
     }
+    if (!ok) q0[f] = -(1 << 28);  // never in range
   }
 
   // ---- B staging thread layout ----
@@ -122,98 +139,113 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
   //   loads BKV k-rows of 4 consecutive n and writes 4 n-rows of BKV k.
   // backward input: B[k][n] = w[(t*Ci + n)*Co + c0 + k] is k-contiguous:
   //   each thread loads 16-byte vectors along k.
-  constexpr int BKV = BELEMS / 4;                 // forward: k values per thread
-  constexpr int VEC = 16 / (int)sizeof(TI);
-  constexpr int BNV = BELEMS / VEC;               // backward input: vectors per thread
+  constexpr int BKV = BELEMS / 4;
+  constexpr int VEC = 16 / ES;
+  constexpr int BNV = BELEMS / VEC;
   const int f_ng = tid % (BN / 4), f_kg = tid / (BN / 4);
   const int b_kv = (tid % (BK / VEC)) * VEC, b_row = tid / (BK / VEC);
   constexpr int B_ROWS_PER_PASS = 256 / (BK / VEC);
 
-  TI rb[BELEMS];
-  uint4 ra[2][FM][2];
+  constexpr int NS = PD + 1;
+  TI rb[NS][BELEMS];
+  uint4 ra[NS][FM][2];
 
-  // incremental (tap, channel block) walk
-  int ta = 0, tc = 0, te = 0, c0 = 0, tap = 0;
+  // B walk (one step ahead) and A walk (PD steps ahead)
+  int tap = 0, c0 = 0;
+  int la = 0, lc = 0, le = 0, lc0 = 0;
+  uint32_t voff[FM];  // byte offset of the current A tap for each row, or OOB
 
-  auto load_a = [&](int buf, int ta_, int tc_, int te_, int c0_) {
+  auto set_tap = [&]() {
+    const int dh = la * p.rh, dw = lc * p.rw, dd = le * p.rd;
 #pragma unroll
     for (int f = 0; f < FM; ++f) {
-      bool ok = a_ok[f];
-      int64_t off;
+      int h, w, d, off;
+      bool ok;
       if (PASS == 0) {
-        int h = a_q0[f] + ta_ * p.rh, w = a_q1[f] + tc_ * p.rw, d = a_q2[f] + te_ * p.rd;
-        ok = ok && (unsigned)h < (unsigned)p.H && (unsigned)w < (unsigned)p.W && (unsigned)d < (unsigned)p.D;
-        off = a_base[f] + (((int64_t)h * p.W + w) * p.D + d) * p.Ci + c0_;
+        h = q0[f] + dh; w = q1[f] + dw; d = q2[f] + dd;
+        ok = (unsigned)h < (unsigned)p.H && (unsigned)w < (unsigned)p.W && (unsigned)d < (unsigned)p.D;
+        off = row_off[f] + ((dh * p.W + dw) * p.D + dd) * p.Ci;
       } else {
-        int oi = a_q0[f] - ta_ * p.rh, oj = a_q1[f] - tc_ * p.rw, ol = a_q2[f] - te_ * p.rd;
-        ok = ok && (unsigned)oi < (unsigned)p.Ho && (unsigned)oj < (unsigned)p.Wo && (unsigned)ol < (unsigned)p.Do;
-        off = a_base[f] + (((int64_t)oi * p.Wo + oj) * p.Do + ol) * p.Co + c0_;
+        h = q0[f] - dh; w = q1[f] - dw; d = q2[f] - dd;
+        ok = (unsigned)h < (unsigned)p.Ho && (unsigned)w < (unsigned)p.Wo && (unsigned)d < (unsigned)p.Do;
+        off = row_off[f] - ((dh * p.Wo + dw) * p.Do + dd) * p.Co;
       }
-      if (ok) {
-        const uint4 *src = (const uint4 *)(Ag + off);
-        ra[buf][f][0] = src[0];
-        ra[buf][f][1] = src[1];
-      } else {
-        ra[buf][f][0] = make_uint4(0, 0, 0, 0);
-        ra[buf][f][1] = make_uint4(0, 0, 0, 0);
-      }
+      voff[f] = ok ? (uint32_t)off * ES : OOB;
     }
   };
 
-  auto load_b = [&](int tap_, int c0_) {
+  auto load_a = [&](int buf) {
+#pragma unroll
+    for (int f = 0; f < FM; ++f) {
+      const uint32_t o = voff[f] + (uint32_t)(lc0 * ES);
+      ra[buf][f][0] = bload(arsrc, o);
+      ra[buf][f][1] = bload(arsrc, o + 16);
+    }
+  };
+
+  auto advance_a = [&]() {
+    lc0 += BK;
+    if (lc0 == C) {
+      lc0 = 0;
+      if (++le == p.kd) {
+        le = 0;
+        if (++lc == p.kw) {
+          lc = 0;
+          ++la;
+        }
+      }
+      if (la < p.kh) set_tap();
+    }
+  };
+
+  auto load_b = [&](int slot, int tap_, int c0_) {
+    TI *rbs = rb[slot];
     if (PASS == 0) {
       const int n = n0 + f_ng * 4;
       const bool nok = n < N;
+      const TI *src = Bg + (tap_ * p.Ci + c0_ + f_kg * BKV) * p.Co + n;
 #pragma unroll
       for (int kv = 0; kv < BKV; ++kv) {
-        int k = f_kg * BKV + kv;
         TI tmp[4];
         if (nok) {
-          const TI *src = Bg + ((int64_t)tap_ * p.Ci + c0_ + k) * p.Co + n;
-          __builtin_memcpy(tmp, src, 4 * sizeof(TI));
+          __builtin_memcpy(tmp, src + kv * p.Co, 4 * sizeof(TI));
         } else {
 #pragma unroll
           for (int v = 0; v < 4; ++v) tmp[v] = (TI)0.0f;
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) rb[v * BKV + kv] = tmp[v];
+        for (int v = 0; v < 4; ++v) rbs[v * BKV + kv] = tmp[v];
       }
     } else {
 #pragma unroll
       for (int i = 0; i < BNV; ++i) {
         int n = n0 + b_row + i * B_ROWS_PER_PASS;
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (n < N) v = *(const uint4 *)(Bg + ((int64_t)tap_ * p.Ci + n) * p.Co + c0_ + b_kv);
-        __builtin_memcpy(&rb[i * VEC], &v, 16);
+        if (n < N) v = *(const uint4 *)(Bg + (tap_ * p.Ci + n) * p.Co + c0_ + b_kv);
+        __builtin_memcpy(&rbs[i * VEC], &v, 16);
       }
     }
   };
 
-  auto store_b = [&](int buf) {
+  auto store_b = [&](int buf, int slot) {
     TI *dst = Bs[buf];
+    const TI *rbs = rb[slot];
     if (PASS == 0) {
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        __builtin_memcpy(&dst[(f_ng * 4 + v) * LDR + f_kg * BKV], &rb[v * BKV], BKV * sizeof(TI));
+        __builtin_memcpy(&dst[(f_ng * 4 + v) * LDR + f_kg * BKV], &rbs[v * BKV], BKV * sizeof(TI));
     } else {
 #pragma unroll
       for (int i = 0; i < BNV; ++i)
-        __builtin_memcpy(&dst[(b_row + i * B_ROWS_PER_PASS) * LDR + b_kv], &rb[i * VEC], 16);
+        __builtin_memcpy(&dst[(b_row + i * B_ROWS_PER_PASS) * LDR + b_kv], &rbs[i * VEC], 16);
     }
   };
 
-  auto advance = [&]() {
+  auto advance_b = [&]() {
     c0 += BK;
     if (c0 == C) {
       c0 = 0;
       ++tap;
-      if (++te == p.kd) {
-        te = 0;
-        if (++tc == p.kw) {
-          tc = 0;
-          ++ta;
-        }
-      }
     }
   };
 
@@ -225,97 +257,132 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-  // prologue: step 0 into buffers 0
-  load_a(0, 0, 0, 0, 0);
-  load_b(0, 0);
-  store_b(0);
+  // prologue: A and B for steps 0 .. PD-1 into ring slots 0 .. PD-1; the B
+  // tile of step 0 goes to LDS now, step s+1's at the end of iteration s.
+  set_tap();
+#pragma unroll
+  for (int u = 0; u < PD; ++u) {
+    if (u < nk) {
+      load_a(u);
+      advance_a();
+      load_b(u, tap, c0);
+      advance_b();
+    }
+  }
+  store_b(0, 0);
   __syncthreads();
 
-  for (int ks = 0; ks < nk; ++ks) {
-    const int cur = ks & 1;
-    const bool more = ks + 1 < nk;
-    if (more) {
-      advance();
-      load_a(cur ^ 1, ta, tc, te, c0);
-      load_b(tap, c0);
-    }
-    // MFMAs on the current step
-    const TI *bs = Bs[cur];
-    uint4 fb[FN][2];
+  const int fb_off = (wn * FN * 32 + frow) * LDR + fkg * LK;
+  for (int kb = 0; kb < nk; kb += NS) {
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const uint4 *src = (const uint4 *)&bs[(wn * FN * 32 + j * 32 + frow) * LDR + fkg * LK];
-      fb[j][0] = src[0];
-      fb[j][1] = src[1];
-    }
-    if constexpr (sizeof(TI) == 2) {
+    for (int u = 0; u < NS; ++u) {
+      const int ks = kb + u;
+      if (ks < nk) {
+        const int bcur = ks & 1;
+        const bool more = ks + 1 < nk;
+        if (ks + PD < nk) {
+          load_a((u + PD) % NS);
+          advance_a();
+          load_b((u + PD) % NS, tap, c0);
+          advance_b();
+        }
+        const TI *bs = Bs[bcur] + fb_off;
+        uint4 fb[FN][2];
 #pragma unroll
-      for (int s = 0; s < 2; ++s)
+        for (int j = 0; j < FN; ++j) {
+          const uint4 *src = (const uint4 *)&bs[j * 32 * LDR];
+          fb[j][0] = src[0];
+          fb[j][1] = src[1];
+        }
+        if constexpr (sizeof(TI) == 2) {
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+          for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(ra[cur][i][s], fb[j][s], acc[i][j]);
-    } else {
-      float af[FM][8], bf[FN][8];
+            for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int i = 0; i < FM; ++i) {
-        __builtin_memcpy(&af[i][0], &ra[cur][i][0], 16);
-        __builtin_memcpy(&af[i][4], &ra[cur][i][1], 16);
+              for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(ra[u][i][s2], fb[j][s2], acc[i][j]);
+        } else {
+          float af[FM][8], bf[FN][8];
+#pragma unroll
+          for (int i = 0; i < FM; ++i) {
+            __builtin_memcpy(&af[i][0], &ra[u][i][0], 16);
+            __builtin_memcpy(&af[i][4], &ra[u][i][1], 16);
+          }
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            __builtin_memcpy(&bf[j][0], &fb[j][0], 16);
+            __builtin_memcpy(&bf[j][4], &fb[j][1], 16);
+          }
+#pragma unroll
+          for (int s2 = 0; s2 < 8; ++s2)
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+              for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[i][s2], bf[j][s2], acc[i][j], 0, 0, 0);
+        }
+        if (more) store_b(bcur ^ 1, (u + 1) % NS);
+        __syncthreads();
       }
-#pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        __builtin_memcpy(&bf[j][0], &fb[j][0], 16);
-        __builtin_memcpy(&bf[j][4], &fb[j][1], 16);
-      }
-#pragma unroll
-      for (int s = 0; s < 8; ++s)
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[i][s], bf[j][s], acc[i][j], 0, 0, 0);
     }
-    if (more) store_b(cur ^ 1);
-    __syncthreads();
   }
 
   // ---- epilogue: lane holds column (lane % 32) of each fragment ----
+  // Raw buffer stores: the uniform row offset goes in soffset, the lane's
+  // column (and its 4-row group) in a 32-bit voffset; rows past M get an
+  // out-of-range voffset and the hardware drops the store.
+  const uint32_t c_bytes = (uint32_t)((int64_t)M * p.ldc * sizeof(TO));
+  const __amdgpu_buffer_rsrc_t crsrc = make_rsrc(p.c, c_bytes);
+  const bool full = m0 + BM <= M;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int n = n0 + wn * FN * 32 + j * 32 + frow;
-      if (n >= N) continue;
+      const int mrow = m0 + wm * FM * 32 + i * 32 + 4 * fkg;  // + 8*(r>>2) + (r&3)
+      const uint32_t lane_off = (uint32_t)(mrow * p.ldc + n) * (uint32_t)sizeof(TO);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int64_t m = m0 + wm * FM * 32 + i * 32 + 8 * (r >> 2) + 4 * fkg + (r & 3);
-        if (m < M) ((TO *)p.c)[m * p.ldc + n] = (TO)acc[i][j][r];
+        const int dr = 8 * (r >> 2) + (r & 3);
+        const int so = dr * p.ldc * (int)sizeof(TO);
+        uint32_t vo = lane_off;
+        if (!full && mrow + dr >= M) vo = OOB;
+        if (n >= N) vo = OOB;
+        if constexpr (sizeof(TO) == 2) {
+          TO h = (TO)acc[i][j][r];
+          __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), crsrc, (int)vo, so, 0);
+        } else {
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, (float)acc[i][j][r]), crsrc,
+                                                (int)vo, so, 0);
+        }
       }
     }
 }
 
 template <int PASS, typename TI, typename TO, int FM, int FN, int WGM, int WGN>
-static void launch_fast(const ConvParams &p, hipStream_t st) {
+static void launch_fast(const ConvParams &p, int pd, hipStream_t st) {
   constexpr int BM = WGM * FM * 32, BN = WGN * FN * 32;
   int64_t tiles = ((p.M + BM - 1) / BM) * (int64_t)((p.N + BN - 1) / BN);
-  hipLaunchKernelGGL((conv_fast_kernel<PASS, TI, TO, FM, FN, WGM, WGN>), dim3((unsigned)tiles), dim3(256), 0,
-                     st, p);
+  dim3 g((unsigned)tiles), b(256);
+  if (pd <= 1) hipLaunchKernelGGL((conv_fast_kernel<PASS, TI, TO, FM, FN, WGM, WGN, 1>), g, b, 0, st, p);
+  else if (pd == 2) hipLaunchKernelGGL((conv_fast_kernel<PASS, TI, TO, FM, FN, WGM, WGN, 2>), g, b, 0, st, p);
+  else hipLaunchKernelGGL((conv_fast_kernel<PASS, TI, TO, FM, FN, WGM, WGN, 3>), g, b, 0, st, p);
 }
 
 // Tile shapes: 0 -> 256 x 64, 1 -> 256 x 128, 2 -> 128 x 256, 3 -> 128 x 128.
 template <int PASS, typename TI, typename TO>
-void run_fast(const ConvParams &p, int shape, hipStream_t st) {
+void run_fast(const ConvParams &p, int shape, int pd, hipStream_t st) {
   switch (shape) {
-    case 0: launch_fast<PASS, TI, TO, 2, 2, 4, 1>(p, st); break;
-    case 1: launch_fast<PASS, TI, TO, 2, 4, 4, 1>(p, st); break;
-    case 2: launch_fast<PASS, TI, TO, 2, 4, 2, 2>(p, st); break;
-    default: launch_fast<PASS, TI, TO, 2, 2, 2, 2>(p, st); break;
+    case 0: launch_fast<PASS, TI, TO, 2, 2, 4, 1>(p, pd, st); break;
+    case 1: launch_fast<PASS, TI, TO, 2, 4, 4, 1>(p, pd, st); break;
+    case 2: launch_fast<PASS, TI, TO, 2, 4, 2, 2>(p, pd, st); break;
+    default: launch_fast<PASS, TI, TO, 2, 2, 2, 2>(p, pd, st); break;
   }
 }
 
 int fast_bk(int elem_bytes) { return elem_bytes == 4 ? 16 : 32; }
 
-#define OWL_FAST(PASS, TI, TO) template void run_fast<PASS, TI, TO>(const ConvParams &, int, hipStream_t);
+#define OWL_FAST(PASS, TI, TO) template void run_fast<PASS, TI, TO>(const ConvParams &, int, int, hipStream_t);
 OWL_FAST(0, bf16_t, bf16_t)
 OWL_FAST(0, bf16_t, float)
 OWL_FAST(1, bf16_t, bf16_t)

@@ -34,9 +34,9 @@ void run_fp8_quantize(const T *src, int64_t n, unsigned char *dst, float *scale,
 void run_e4m3_encode(const float *src, int64_t n, unsigned char *dst, hipStream_t st);
 // conv_fast.hip: forward / unit-stride backward input with A fragments read
 // straight from global memory.  shape: 0 -> 256x64, 1 -> 256x128,
-// 2 -> 128x256, 3 -> 128x128 (BM x BN).
+// 2 -> 128x256, 3 -> 128x128 (BM x BN); pd = K-steps of A in flight (1-3).
 template <int PASS, typename TI, typename TO>
-void run_fast(const ConvParams &p, int shape, hipStream_t st);
+void run_fast(const ConvParams &p, int shape, int pd, hipStream_t st);
 int fast_bk(int elem_bytes);
 template <typename TI> int tile_bm();
 template <typename TI> int tile_bk();
