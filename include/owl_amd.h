@@ -108,6 +108,28 @@ int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int out_t,
                             const void *x, const void *dy, void *dw,
                             void *workspace, size_t ws_bytes, void *stream);
 
+/* Ragged batches of forward convolutions in one persistent launch (no
+ * counterpart in Owl, which loops over calls; this is the batched form of
+ * owl_amd_conv_fwd).  create: validates the n descriptors and picks a tile
+ * edge per shape class (from the autotuner's cache, else a size heuristic);
+ * autotune: times both tile edges per shape class and caches the faster;
+ * run: one launch over all problems, then waits; capture / replay: record
+ * that launch in a hipGraph on a non-default stream and replay it.  in_t /
+ * out_t: BF16 -> BF16 or F32, F16 -> F16, F32 -> F32. */
+typedef struct owl_amd_conv_group owl_amd_conv_group;
+int owl_amd_conv_group_create(owl_amd_conv_group **g, int n,
+                              const owl_amd_conv_desc *descs, int in_t, int out_t);
+int owl_amd_conv_group_autotune(owl_amd_conv_group *g, const void *const *xs,
+                                const void *const *ws, void *const *ys, void *stream);
+int owl_amd_conv_group_run(owl_amd_conv_group *g, const void *const *xs,
+                           const void *const *ws, void *const *ys, void *stream);
+int owl_amd_conv_group_capture(owl_amd_conv_group *g, const void *const *xs,
+                               const void *const *ws, void *const *ys, void *stream);
+int owl_amd_conv_group_replay(owl_amd_conv_group *g, void *stream);
+int owl_amd_conv_group_info(const owl_amd_conv_group *g, int *tile_edges,
+                            int64_t *total_tiles);
+void owl_amd_conv_group_destroy(owl_amd_conv_group *g);
+
 /* Pooling (owl_ndarray_pool_impl.h): results bit-identical to Owl's C. */
 int owl_amd_pool_fwd(const owl_amd_pool_desc *d, int t, int is_max,
                      const void *x, void *y, void *stream);

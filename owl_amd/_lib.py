@@ -99,6 +99,13 @@ def lib():
             "owl_amd_conv_fwd_fp8": ([cd, i32, vp, vp, vp, vp, vp, vp], i32),
             "owl_amd_fp8_quantize": ([i32, vp, i64, vp, vp, vp], i32),
             "owl_amd_fp8_encode": ([vp, i64, vp, vp], i32),
+            "owl_amd_conv_group_create": ([ctypes.POINTER(vp), i32, cd, i32, i32], i32),
+            "owl_amd_conv_group_autotune": ([vp, vp, vp, vp, vp], i32),
+            "owl_amd_conv_group_run": ([vp, vp, vp, vp, vp], i32),
+            "owl_amd_conv_group_capture": ([vp, vp, vp, vp, vp], i32),
+            "owl_amd_conv_group_replay": ([vp, vp], i32),
+            "owl_amd_conv_group_info": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i64)], i32),
+            "owl_amd_conv_group_destroy": ([vp], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name, None)

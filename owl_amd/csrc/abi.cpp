@@ -5,7 +5,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "conv_kernels.h"
@@ -921,3 +923,254 @@ int pool3d_bwd_dropin(bool is_max, T *input, T *output_back, T *input_back, int6
 
 DEFINE_DROPINS(float32, float)
 DEFINE_DROPINS(float64, double)
+
+// ------------------------------------------------------------------ ragged groups
+// A plan for running many forward convolutions (different shapes) as one
+// persistent launch.  Tile edge per problem comes from the shape-class cache
+// the autotuner fills (or a size heuristic until it has run).
+namespace {
+
+typedef std::tuple<int, int, int, int, int, int> ClassKey;  // in_t, out_t, aligned, log2 M, log2 N, log2 K
+std::map<ClassKey, int> g_tune_cache;
+
+int ilog2(int64_t v) {
+  int r = 0;
+  while (v > 1) {
+    v >>= 1;
+    ++r;
+  }
+  return r;
+}
+
+}  // namespace
+
+struct owl_amd_conv_group {
+  int n = 0, in_t = 0, out_t = 0;
+  std::vector<owl_amd_conv_desc> descs;
+  std::vector<GroupEntry> tab;
+  std::vector<ClassKey> keys;
+  GroupEntry *dev_tab = nullptr;
+  int64_t total = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static int group_edge(int in_t, int cfg) {
+  if (in_t == OWL_AMD_F32) return group_tile_edge<float>(cfg);
+  if (in_t == OWL_AMD_F16) return group_tile_edge<f16_t>(cfg);
+  return group_tile_edge<bf16_t>(cfg);
+}
+
+static void group_layout(owl_amd_conv_group *g) {
+  int64_t t = 0;
+  for (int i = 0; i < g->n; ++i) {
+    GroupEntry &e = g->tab[i];
+    const int edge = group_edge(g->in_t, e.cfg);
+    e.tile_begin = t;
+    t += ((e.p.M + edge - 1) / edge) * (int64_t)((e.p.N + edge - 1) / edge);
+  }
+  g->total = t;
+}
+
+static int group_launch(owl_amd_conv_group *g, const GroupEntry *tab, int nprob, int64_t total, hipStream_t st) {
+  if (total == 0) return OWL_AMD_OK;
+  int grid = (int)(total < 2048 ? total : 2048);  // persistent: ~8 workgroups per CU
+  if (g->in_t == OWL_AMD_BF16 && g->out_t == OWL_AMD_BF16) run_group_fwd<bf16_t, bf16_t>(tab, nprob, total, grid, st);
+  else if (g->in_t == OWL_AMD_BF16 && g->out_t == OWL_AMD_F32) run_group_fwd<bf16_t, float>(tab, nprob, total, grid, st);
+  else if (g->in_t == OWL_AMD_F16 && g->out_t == OWL_AMD_F16) run_group_fwd<f16_t, f16_t>(tab, nprob, total, grid, st);
+  else run_group_fwd<float, float>(tab, nprob, total, grid, st);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+static void group_set_ptrs(owl_amd_conv_group *g, const void *const *xs, const void *const *ws, void *const *ys) {
+  for (int i = 0; i < g->n; ++i) {
+    GroupEntry &e = g->tab[i];
+    e.p.a = xs[i];
+    e.p.b = ws[i];
+    e.p.c = ys[i];
+    const int v = vec_of(g->in_t);
+    e.aligned = e.p.Ci % v == 0 && e.p.Co % v == 0 && aligned16(xs[i], ws[i], nullptr);
+  }
+}
+
+extern "C" int owl_amd_conv_group_create(owl_amd_conv_group **out, int n, const owl_amd_conv_desc *descs,
+                                         int in_t, int out_t) {
+  if (!out || n < 0 || (n > 0 && !descs)) ARG_FAIL("bad group arguments");
+  const bool ok_t = (in_t == OWL_AMD_BF16 && (out_t == OWL_AMD_BF16 || out_t == OWL_AMD_F32)) ||
+                    (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F16) || (in_t == OWL_AMD_F32 && out_t == OWL_AMD_F32);
+  if (!ok_t) {
+    owl_amd_set_error("group: unsupported dtype pair in=%d out=%d", in_t, out_t);
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  for (int i = 0; i < n; ++i) {
+    int rc = check_conv(&descs[i]);
+    if (rc) return rc;
+  }
+  owl_amd_conv_group *g = new owl_amd_conv_group();
+  g->n = n;
+  g->in_t = in_t;
+  g->out_t = out_t;
+  g->descs.assign(descs, descs + n);
+  g->tab.resize(n);
+  g->keys.resize(n);
+  for (int i = 0; i < n; ++i) {
+    GroupEntry &e = g->tab[i];
+    memset(&e, 0, sizeof(e));
+    e.p = make_params(&descs[i]);
+    e.p.M = descs[i].B * descs[i].Ho * descs[i].Wo * descs[i].Do;
+    e.p.N = e.p.Co;
+    e.p.K = descs[i].kh * descs[i].kw * descs[i].kd * descs[i].Ci;
+    e.p.ldc = e.p.Co;
+    const int v = vec_of(in_t);
+    const int al = e.p.Ci % v == 0 && e.p.Co % v == 0;
+    g->keys[i] = ClassKey(in_t, out_t, al, ilog2(e.p.M), ilog2(e.p.N), ilog2(e.p.K));
+    auto it = g_tune_cache.find(g->keys[i]);
+    // heuristic until tuned: half-size tiles when the full tile would leave
+    // most of the chip idle or mostly empty
+    const int full = group_edge(in_t, 0);
+    const int64_t full_tiles = ((e.p.M + full - 1) / full) * (int64_t)((e.p.N + full - 1) / full);
+    e.cfg = it != g_tune_cache.end() ? it->second : (full_tiles < 256 || e.p.N <= full / 2 ? 1 : 0);
+  }
+  group_layout(g);
+  hipError_t err = hipMalloc(&g->dev_tab, sizeof(GroupEntry) * (n > 0 ? n : 1));
+  if (err != hipSuccess) {
+    delete g;
+    owl_amd_set_error("hipMalloc group table: %s", hipGetErrorString(err));
+    return OWL_AMD_ERR_NOMEM;
+  }
+  *out = g;
+  return OWL_AMD_OK;
+}
+
+extern "C" int owl_amd_conv_group_info(const owl_amd_conv_group *g, int *tile_edges, int64_t *total_tiles) {
+  if (!g) ARG_FAIL("null group");
+  if (tile_edges)
+    for (int i = 0; i < g->n; ++i) tile_edges[i] = group_edge(g->in_t, g->tab[i].cfg);
+  if (total_tiles) *total_tiles = g->total;
+  return OWL_AMD_OK;
+}
+
+extern "C" int owl_amd_conv_group_run(owl_amd_conv_group *g, const void *const *xs, const void *const *ws,
+                                      void *const *ys, void *stream) {
+  if (!g || (g->n > 0 && (!xs || !ws || !ys))) ARG_FAIL("bad group run arguments");
+  for (int i = 0; i < g->n; ++i)
+    if (!xs[i] || !ws[i] || !ys[i]) ARG_FAIL("null pointer for problem %d", i);
+  group_set_ptrs(g, xs, ws, ys);
+  hipStream_t st = (hipStream_t)stream;
+  // the table copy is ordered before the launch on the same stream; the host
+  // vector is not touched again until the next run, which first waits here
+  OWL_CHECK_HIP(hipMemcpyAsync(g->dev_tab, g->tab.data(), sizeof(GroupEntry) * g->n, hipMemcpyHostToDevice, st));
+  int rc = group_launch(g, g->dev_tab, g->n, g->total, st);
+  if (rc) return rc;
+  OWL_CHECK_HIP(hipStreamSynchronize(st));
+  return OWL_AMD_OK;
+}
+
+// Time each shape class with both tile edges (separate grouped launches of
+// just that class's problems) and keep the faster in the process-wide cache.
+extern "C" int owl_amd_conv_group_autotune(owl_amd_conv_group *g, const void *const *xs, const void *const *ws,
+                                           void *const *ys, void *stream) {
+  if (!g) ARG_FAIL("null group");
+  group_set_ptrs(g, xs, ws, ys);
+  hipStream_t st = (hipStream_t)stream;
+  std::map<ClassKey, std::vector<int>> classes;
+  for (int i = 0; i < g->n; ++i) classes[g->keys[i]].push_back(i);
+  GroupEntry *tmp = nullptr;
+  OWL_CHECK_HIP(hipMalloc(&tmp, sizeof(GroupEntry) * (g->n > 0 ? g->n : 1)));
+  hipEvent_t e0, e1;
+  OWL_CHECK_HIP(hipEventCreate(&e0));
+  OWL_CHECK_HIP(hipEventCreate(&e1));
+  int rc = OWL_AMD_OK;
+  for (auto &kv : classes) {
+    float best = 1e30f;
+    int best_cfg = 0;
+    for (int cfg = 0; cfg < 2 && rc == OWL_AMD_OK; ++cfg) {
+      std::vector<GroupEntry> sub;
+      int64_t t = 0;
+      const int edge = group_edge(g->in_t, cfg);
+      for (int i : kv.second) {
+        GroupEntry e = g->tab[i];
+        e.cfg = cfg;
+        e.tile_begin = t;
+        t += ((e.p.M + edge - 1) / edge) * (int64_t)((e.p.N + edge - 1) / edge);
+        sub.push_back(e);
+      }
+      if (hipMemcpy(tmp, sub.data(), sizeof(GroupEntry) * sub.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = OWL_AMD_ERR_HIP;
+        break;
+      }
+      rc = group_launch(g, tmp, (int)sub.size(), t, st);  // warm-up
+      if (rc) break;
+      (void)hipEventRecord(e0, st);
+      for (int r = 0; r < 3 && rc == OWL_AMD_OK; ++r) rc = group_launch(g, tmp, (int)sub.size(), t, st);
+      (void)hipEventRecord(e1, st);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        best_cfg = cfg;
+      }
+    }
+    if (rc) break;
+    {
+      std::lock_guard<std::recursive_mutex> lk(g_mu);
+      g_tune_cache[kv.first] = best_cfg;
+    }
+    for (int i : kv.second) g->tab[i].cfg = best_cfg;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(tmp);
+  group_layout(g);
+  return rc;
+}
+
+// Capture one grouped launch into a hipGraph (the problem table is uploaded
+// before capture, so the graph holds a single kernel node).
+extern "C" int owl_amd_conv_group_capture(owl_amd_conv_group *g, const void *const *xs, const void *const *ws,
+                                          void *const *ys, void *stream) {
+  if (!g || !stream) ARG_FAIL("group capture needs a group and a non-default stream");
+  group_set_ptrs(g, xs, ws, ys);
+  OWL_CHECK_HIP(hipMemcpy(g->dev_tab, g->tab.data(), sizeof(GroupEntry) * g->n, hipMemcpyHostToDevice));
+  hipStream_t st = (hipStream_t)stream;
+  if (g->exec) {
+    (void)hipGraphExecDestroy(g->exec);
+    g->exec = nullptr;
+  }
+  if (g->graph) {
+    (void)hipGraphDestroy(g->graph);
+    g->graph = nullptr;
+  }
+  OWL_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = group_launch(g, g->dev_tab, g->n, g->total, st);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &graph);
+  if (rc) return rc;
+  if (e != hipSuccess) {
+    owl_amd_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+    return OWL_AMD_ERR_HIP;
+  }
+  g->graph = graph;
+  OWL_CHECK_HIP(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
+  return OWL_AMD_OK;
+}
+
+extern "C" int owl_amd_conv_group_replay(owl_amd_conv_group *g, void *stream) {
+  if (!g || !g->exec) ARG_FAIL("group has no captured graph");
+  OWL_CHECK_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return OWL_AMD_OK;
+}
+
+extern "C" void owl_amd_conv_group_destroy(owl_amd_conv_group *g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->dev_tab) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(g->dev_tab);
+  }
+  delete g;
+}

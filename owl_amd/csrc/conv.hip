@@ -112,9 +112,9 @@ template <> __device__ __forceinline__ int acc_row<16>(int lane, int r) {
 
 constexpr int NT = 256;  // threads per workgroup (4 waves, 2 x 2)
 
-template <typename TI> struct Tile {
+template <typename TI, int BMX = Mfma<TI>::BM> struct Tile {
   typedef Mfma<TI> MF;
-  static constexpr int BM = MF::BM, BN = MF::BN, BK = MF::BK;
+  static constexpr int BM = BMX, BN = BMX, BK = MF::BK;
   static constexpr int VEC = 16 / (int)sizeof(TI);     // elements per 16-byte vector
   static constexpr int PAD = 16 / (int)sizeof(TI);     // 16 bytes of row padding
   static constexpr int LDR = BK + PAD;                 // LDS row length (elements)
@@ -154,14 +154,23 @@ __device__ __forceinline__ void tap_decode(int t, int kw, int kd, int &a, int &c
 // ALIGNED: channel counts are multiples of VEC and pointers 16-byte aligned,
 // so operands move as 16-byte vectors; otherwise element by element.
 // ---------------------------------------------------------------------------
-template <int PASS, typename TI, typename TO, bool ALIGNED>
-__global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p) {
-  typedef Tile<TI> TL;
+// LDS bytes one tile of the body below uses (two stages of A and B).
+template <typename TI, int BMX> constexpr int igemm_lds_bytes() {
+  return 2 * (2 * BMX) * Tile<TI, BMX>::LDR * (int)sizeof(TI);
+}
+
+// One output tile (index `tile` in M-major, N-minor order) of one split of
+// the GEMM, computed by the calling workgroup.  `lds_raw` must hold
+// igemm_lds_bytes<TI, BMX>() bytes.
+template <int PASS, typename TI, typename TO, bool ALIGNED, int BMX>
+__device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, int split_idx, char *lds_raw) {
+  typedef Tile<TI, BMX> TL;
   typedef Mfma<TI> MF;
   constexpr int BM = TL::BM, BN = TL::BN, BK = TL::BK, VEC = TL::VEC, LDR = TL::LDR;
   constexpr int ELEMS = TL::ELEMS;
 
-  __shared__ __attribute__((aligned(16))) TI lds[2][(BM + BN) * LDR];
+  typedef TI stage_t[(BM + BN) * LDR];
+  stage_t *lds = (stage_t *)lds_raw;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -173,13 +182,12 @@ __global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p) {
   const int64_t Kfull = p.K;
 
   const int n_tiles = (N + BN - 1) / BN;
-  const int64_t tile = blockIdx.x;
   const int64_t m0 = (tile / n_tiles) * BM;
   const int n0 = (int)(tile % n_tiles) * BN;
 
   int64_t kbeg = 0, kend = Kfull;
   if (PASS == 2) {
-    kbeg = (int64_t)blockIdx.y * p.k_per_split;
+    kbeg = (int64_t)split_idx * p.k_per_split;
     kend = kbeg + p.k_per_split;
     if (kend > Kfull) kend = Kfull;
   }
@@ -529,7 +537,7 @@ __global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p) {
         if (m >= M) continue;
         if (PASS == 2 && p.split > 1) {
           typedef typename MF::acc_elem TA;
-          TA *ws = (TA *)p.c + (int64_t)blockIdx.y * M * N;
+          TA *ws = (TA *)p.c + (int64_t)split_idx * M * N;
           ws[m * N + n] = (TA)acc[i][j][r];
         } else {
           if (p.scale_a)
@@ -539,6 +547,13 @@ __global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p) {
         }
       }
     }
+}
+
+template <int PASS, typename TI, typename TO, bool ALIGNED>
+__global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p) {
+  constexpr int BMX = Mfma<TI>::BM;
+  __shared__ __attribute__((aligned(16))) char lds[igemm_lds_bytes<TI, BMX>()];
+  igemm_tile<PASS, TI, TO, ALIGNED, BMX>(p, blockIdx.x, blockIdx.y, lds);
 }
 
 // Deterministic split-K reduction: dw[e] = sum over splits in split order,
@@ -586,6 +601,57 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const TA *__restrict
     }
   }
 }
+
+// ---------------------------------------------------------------------------
+// Ragged batches: many forward convolutions of different shapes in one
+// persistent launch.  The host builds a table of problems (each with its
+// own tile edge, chosen per shape class) and the prefix sum of their tile
+// counts; workgroups stride over the concatenated tile list, find their
+// problem by binary search and run the same tile body as the single-problem
+// kernel, so a grouped result equals the separate launch's bit for bit.
+// ---------------------------------------------------------------------------
+template <typename TI, typename TO>
+__global__ __launch_bounds__(NT) void conv_group_fwd_kernel(const GroupEntry *__restrict__ tab, int nprob,
+                                                            int64_t total) {
+  constexpr int L0 = igemm_lds_bytes<TI, Mfma<TI>::BM>();
+  constexpr int L1 = igemm_lds_bytes<TI, Mfma<TI>::BM / 2>();
+  __shared__ __attribute__((aligned(16))) char lds[L0 > L1 ? L0 : L1];
+  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (tab[mid].tile_begin <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const ConvParams p = tab[lo].p;
+    const int64_t local = t - tab[lo].tile_begin;
+    const int cfg = tab[lo].cfg, al = tab[lo].aligned;
+    if (cfg == 0) {
+      if (al) igemm_tile<0, TI, TO, true, Mfma<TI>::BM>(p, local, 0, lds);
+      else igemm_tile<0, TI, TO, false, Mfma<TI>::BM>(p, local, 0, lds);
+    } else {
+      if (al) igemm_tile<0, TI, TO, true, Mfma<TI>::BM / 2>(p, local, 0, lds);
+      else igemm_tile<0, TI, TO, false, Mfma<TI>::BM / 2>(p, local, 0, lds);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename TI, typename TO>
+void run_group_fwd(const GroupEntry *dev_tab, int nprob, int64_t total, int grid, hipStream_t st) {
+  hipLaunchKernelGGL((conv_group_fwd_kernel<TI, TO>), dim3((unsigned)grid), dim3(NT), 0, st, dev_tab, nprob,
+                     total);
+}
+
+template <typename TI> int group_tile_edge(int cfg) { return cfg == 0 ? Mfma<TI>::BM : Mfma<TI>::BM / 2; }
+
+template void run_group_fwd<bf16_t, bf16_t>(const GroupEntry *, int, int64_t, int, hipStream_t);
+template void run_group_fwd<bf16_t, float>(const GroupEntry *, int, int64_t, int, hipStream_t);
+template void run_group_fwd<f16_t, f16_t>(const GroupEntry *, int, int64_t, int, hipStream_t);
+template void run_group_fwd<float, float>(const GroupEntry *, int, int64_t, int, hipStream_t);
+template int group_tile_edge<bf16_t>(int);
+template int group_tile_edge<f16_t>(int);
+template int group_tile_edge<float>(int);
 
 // ---------------------------------------------------------------------------
 // Launchers

@@ -27,6 +27,17 @@ struct ConvParams {
 
 template <int PASS, typename TI, typename TO>
 void run_igemm(const ConvParams &p, bool aligned, hipStream_t st);
+
+// One problem of a grouped (ragged) forward launch.
+struct GroupEntry {
+  ConvParams p;
+  int64_t tile_begin;  // first tile of this problem in the concatenated list
+  int cfg;             // 0: full tile edge, 1: half (for small problems)
+  int aligned;
+};
+template <typename TI, typename TO>
+void run_group_fwd(const GroupEntry *dev_tab, int nprob, int64_t total, int grid, hipStream_t st);
+template <typename TI> int group_tile_edge(int cfg);
 template <typename TA, typename TO>
 void run_splitk_reduce(const TA *ws, TO *out, int64_t n, int splits, hipStream_t st);
 template <typename T>

@@ -122,6 +122,20 @@ def _pads_from_shapes(sp_in, ksp, sp_out, stride, rate):
     return [c_pad(s, o, k_eff(k, r), n) for n, k, o, s, r in zip(sp_in, ksp, sp_out, stride, rate)]
 
 
+def conv_problem(x_shape, k_shape, stride, rate=None, padding=SAME):
+    """Geometry (a ``_Conv``) of ``dilated_conv{2,3}d ~padding x w stride rate``
+    exactly as the forward wrappers derive it."""
+    nd_ = len(x_shape) - 2
+    rate = list(rate) if rate is not None else [1] * nd_
+    B, sp_in, Ci = x_shape[0], list(x_shape[1:-1]), x_shape[-1]
+    ksp, Co = list(k_shape[:-2]), k_shape[-1]
+    _verify(Ci == k_shape[-2], "input channels must equal the kernel's third dimension")
+    sp_out = [conv_out_dim(padding, n, k_eff(kk, r), s) for n, kk, s, r in zip(sp_in, ksp, stride, rate)]
+    _verify(all(o > 0 for o in sp_out), f"empty output shape {sp_out}")
+    pads = _pads_from_shapes(sp_in, ksp, sp_out, stride, rate) if padding == SAME else [0] * nd_
+    return _Conv(B, sp_in, Ci, ksp, Co, sp_out, stride, rate, pads)
+
+
 def _dev_conv(pass_, g, a, b, out, out_t=None):
     """Run one pass on device tensors.  pass_: fwd(a=x, b=w), bi(a=w, b=dy), bk(a=x, b=dy)."""
     st = _stream()

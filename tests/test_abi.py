@@ -35,6 +35,8 @@ def test_dropin_table_matches_header():
     decl = set(declared_functions())
     assert set(_lib.DROPINS) <= decl
     assert {d for d in decl if "_ndarray_" in d} == set(_lib.DROPINS)
+    for name in ("owl_amd_conv_group_create", "owl_amd_conv_group_run", "owl_amd_conv_group_replay"):
+        assert name in decl
 
 
 def test_library_loads_and_binds():
