@@ -166,6 +166,11 @@ int dispatch_igemm(const ConvParams &p, int in_t, int out_t, bool aligned, hipSt
   return OWL_AMD_OK;
 }
 
+int env_int(const char *name, int dflt) {
+  const char *e = getenv(name);
+  return e ? atoi(e) : dflt;
+}
+
 int vec_of(int t) { return 16 / (int)dtype_size(t); }
 
 // Kernel family selection: OWL_AMD_CONV_PATH=generic forces the LDS-staged
@@ -181,10 +186,7 @@ bool fast_path_enabled() {
 
 int fast_shape_for(int N) { return N <= 64 ? 0 : (N <= 128 ? 1 : 2); }
 
-int env_int(const char *name, int dflt) {
-  const char *e = getenv(name);
-  return e ? atoi(e) : dflt;
-}
+
 
 // A-prefetch depth of the fast kernels (OWL_AMD_FAST_PD overrides).
 int fast_pd() {
@@ -245,6 +247,8 @@ void plan_splitk(const ConvParams &p, int in_t, int *split, int64_t *kps) {
   int64_t tiles = ((p.M + bm - 1) / bm) * (int64_t)((p.N + bm - 1) / bm);
   int64_t want = (512 + tiles - 1) / tiles;
   int64_t max_split = (p.K + 16 * bk - 1) / (16 * bk);
+  static const int cap = env_int("OWL_AMD_MAX_SPLIT", 1 << 30);  // diagnostics / tuning
+  if (max_split > cap) max_split = cap;
   if (want > max_split) want = max_split;
   if (want < 1) want = 1;
   int64_t k = (p.K + want - 1) / want;
@@ -374,8 +378,23 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   p.a = x; p.b = dy; p.ldc = p.Co;
   const int v = vec_of(in_t);
   bool aligned = dtype_size(in_t) && p.Ci % v == 0 && p.Co % v == 0 && aligned16(x, dy, nullptr);
+  // 16-bit storage: the transposing-LDS kernel of conv_fast.hip (32-bit byte
+  // offsets, so operands below 2 GiB)
+  const bool tr = fast_path_enabled() && aligned && (in_t == OWL_AMD_BF16 || in_t == OWL_AMD_F16) &&
+                  (int64_t)p.B * p.H * p.W * p.D * p.Ci * 2 < (1LL << 31) && p.K * (int64_t)p.Co * 2 < (1LL << 31) &&
+                  p.M * (int64_t)p.N * 4 < (1LL << 31) && p.K < (1LL << 31) &&
+                  (out_t == in_t || out_t == OWL_AMD_F32);
   if (p.split <= 1) {
     p.c = dw;
+    if (tr) {
+      if (in_t == OWL_AMD_BF16) {
+        if (out_t == OWL_AMD_BF16) run_wgrad<bf16_t, bf16_t>(p, st); else run_wgrad<bf16_t, float>(p, st);
+      } else {
+        if (out_t == OWL_AMD_F16) run_wgrad<f16_t, f16_t>(p, st); else run_wgrad<f16_t, float>(p, st);
+      }
+      OWL_CHECK_HIP(hipGetLastError());
+      return OWL_AMD_OK;
+    }
     return dispatch_igemm<2>(p, in_t, out_t, aligned, st);
   }
   size_t need = owl_amd_conv_bwd_kernel_workspace(d, in_t);
@@ -392,8 +411,14 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
     ARG_FAIL("workspace too small: %zu < %zu bytes", ws_bytes, need);
   }
   p.c = ws;
-  rc = dispatch_igemm<2>(p, in_t, out_t, aligned, st);
-  if (rc) return rc;
+  if (tr) {
+    if (in_t == OWL_AMD_BF16) run_wgrad<bf16_t, float>(p, st);
+    else run_wgrad<f16_t, float>(p, st);
+    OWL_CHECK_HIP(hipGetLastError());
+  } else {
+    rc = dispatch_igemm<2>(p, in_t, out_t, aligned, st);
+    if (rc) return rc;
+  }
   const int64_t n = p.M * p.N;
   if (in_t == OWL_AMD_F64) run_splitk_reduce<double, double>((const double *)ws, (double *)dw, n, p.split, st);
   else if (out_t == OWL_AMD_F32) run_splitk_reduce<float, float>((const float *)ws, (float *)dw, n, p.split, st);

@@ -359,6 +359,207 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Backward kernel (weight gradient) for 16-bit storage:
+//   dw[m = (tap, ci)][n = co] = sum over output pixels k of
+//       x[pixel k seen through tap][ci] * dy[k][co]
+// The contraction runs over pixels, which are strided in NHWC for both
+// operands, so the tiles are copied to LDS in their natural layout
+// ([pixel][channel], 16-byte vectors, no register transposes) and the MFMA
+// operands are read with gfx950's transposing LDS load: in each group of 16
+// lanes, ds_read_b64_tr_b16 gives lane i halfword (i % 4) of the 8 bytes
+// addressed by lane 4e + i/4 as its element e.  Pointing lane j at
+// (pixel k0 + j/4, channels c0 + 4 (j % 4) ..) therefore hands lane i the 4
+// consecutive pixels k0..k0+3 of channel c0 + i -- one MFMA operand quarter.
+// Rows are padded by 64 bytes so the 32 lanes of a read phase (4 pixels x 64
+// bytes) touch every bank once.
+// Tile: 128 (tap, ci) x 128 co x 32 pixels, 4 waves of 64 x 64; split-K
+// over pixels as in the LDS-staged kernel.
+// ---------------------------------------------------------------------------
+typedef short v4s16 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4s16 lds_v4s16;
+
+template <typename TI> __device__ __forceinline__ f32x16 mfma16(const v4s16 &a0, const v4s16 &a1, const v4s16 &b0,
+                                                               const v4s16 &b1, f32x16 c);
+template <> __device__ __forceinline__ f32x16 mfma16<bf16_t>(const v4s16 &a0, const v4s16 &a1, const v4s16 &b0,
+                                                             const v4s16 &b1, f32x16 c) {
+  typedef short v8s16 __attribute__((ext_vector_type(8)));
+  v8s16 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+  v8s16 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                 0, 0);
+}
+template <> __device__ __forceinline__ f32x16 mfma16<f16_t>(const v4s16 &a0, const v4s16 &a1, const v4s16 &b0,
+                                                            const v4s16 &b1, f32x16 c) {
+  typedef short v8s16 __attribute__((ext_vector_type(8)));
+  v8s16 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+  v8s16 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0,
+                                                0);
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
+  constexpr int BM = 128, BN = 128, BK = 32;
+  constexpr int RS = BM * 2 + 64;  // LDS row stride in bytes (128 16-bit values + 64 bytes pad)
+  __shared__ __attribute__((aligned(16))) char lds[2][2][BK * RS];  // [stage][A|B]
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int M = (int)p.M, N = p.N;
+  const int n_tiles = (N + BN - 1) / BN;
+  const int tile = blockIdx.x;
+  const int m0 = (tile / n_tiles) * BM, n0 = (tile % n_tiles) * BN;
+  const int64_t kbeg = (int64_t)blockIdx.y * p.k_per_split;
+  int64_t kend = kbeg + p.k_per_split;
+  if (kend > p.K) kend = p.K;
+  const int nk = (int)((kend - kbeg + BK - 1) / BK);
+
+  const uint32_t x_bytes = (uint32_t)((int64_t)p.B * p.H * p.W * p.D * p.Ci * 2);
+  const uint32_t g_bytes = (uint32_t)((int64_t)p.K * p.Co * 2);
+  const __amdgpu_buffer_rsrc_t xr = make_rsrc(p.a, x_bytes);
+  const __amdgpu_buffer_rsrc_t gr = make_rsrc(p.b, g_bytes);
+
+  // ---- loader assignment: 512 A vectors + 512 B vectors per K-step, 2 + 2 per thread
+  // vector v: pixel k = v / 16, 8-channel chunk c = v % 16
+  const int chunk = tid & 15, kpix0 = tid >> 4;  // pixels kpix0 and kpix0 + 16
+  // A rows m0 + 8*chunk .. +7 share one tap (Ci % 8 == 0)
+  const int am = m0 + chunk * 8;
+  const bool a_row_ok = am < M;
+  const int a_tap = a_row_ok ? am / p.Ci : 0;
+  const int a_ci = am - a_tap * p.Ci;
+  int ta, tc, te;
+  {
+    int kwd = p.kw * p.kd;
+    ta = a_tap / kwd;
+    int r = a_tap - ta * kwd;
+    tc = r / p.kd;
+    te = r - tc * p.kd;
+  }
+  const int dh = ta * p.rh - p.ph, dw = tc * p.rw - p.pw, dd = te * p.rd - p.pd;
+  const int bn = n0 + chunk * 8;
+  const bool b_col_ok = bn < N;
+
+  uint4 ra[2], rb[2];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t k = k0 + kpix0 + 16 * h;
+      uint32_t vo = OOB, go = OOB;
+      if (k < kend) {
+        int t = (int)k;
+        const int l = t % p.Do; t /= p.Do;
+        const int j = t % p.Wo; t /= p.Wo;
+        const int i = t % p.Ho;
+        const int b = t / p.Ho;
+        const int hh = i * p.sh + dh, ww = j * p.sw + dw, d = l * p.sd + dd;
+        if (a_row_ok && (unsigned)hh < (unsigned)p.H && (unsigned)ww < (unsigned)p.W && (unsigned)d < (unsigned)p.D)
+          vo = (uint32_t)((((b * p.H + hh) * p.W + ww) * p.D + d) * p.Ci + a_ci) * 2u;
+        if (b_col_ok) go = (uint32_t)((int)k * p.Co + bn) * 2u;
+      }
+      ra[h] = bload(xr, vo);
+      rb[h] = bload(gr, go);
+    }
+  };
+  auto store = [&](int st) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int row = kpix0 + 16 * h;
+      *(uint4 *)(&lds[st][0][row * RS + chunk * 16]) = ra[h];
+      *(uint4 *)(&lds[st][1][row * RS + chunk * 16]) = rb[h];
+    }
+  };
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  // per-lane byte offset of its transposed-read source within a tile:
+  // pixel row (lane%16)/4 (+8 for the upper k-group, +4 for the second half),
+  // column base (lane/16 %2)*16 + 4*(lane%4) within the fragment's 32 columns
+  const int j16 = lane & 15;
+  const int tr_off = ((j16 >> 2) + 8 * (lane >> 5)) * RS + 2 * (16 * ((lane >> 4) & 1) + 4 * (j16 & 3));
+
+  if (nk > 0) {
+    load(kbeg);
+    store(0);
+    __syncthreads();
+    for (int ks = 0; ks < nk; ++ks) {
+      const int cur = ks & 1;
+      const bool more = ks + 1 < nk;
+      if (more) load(kbeg + (int64_t)(ks + 1) * BK);
+      const char *As = lds[cur][0], *Bs = lds[cur][1];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {  // two 16-pixel MFMA steps
+        v4s16 a[2][2], b[2][2];
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+          for (int hf = 0; hf < 2; ++hf) {
+            const int off = tr_off + (16 * s + 4 * hf) * RS;
+            a[f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+                (lds_v4s16 *)(As + off + 2 * (wm * 64 + f * 32)));
+            b[f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+                (lds_v4s16 *)(Bs + off + 2 * (wn * 64 + f * 32)));
+          }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][j] = mfma16<TI>(a[i][0], a[i][1], b[j][0], b[j][1], acc[i][j]);
+      }
+      if (more) store(cur ^ 1);
+      __syncthreads();
+    }
+  }
+
+  // ---- epilogue (same layout as the other MFMA kernels) ----
+  const int frow = lane & 31, fkg = lane >> 5;
+  const bool partial = p.split > 1;
+  const uint32_t c_bytes = partial ? (uint32_t)((int64_t)M * N * 4) : (uint32_t)((int64_t)M * N * sizeof(TO));
+  char *cbase = (char *)p.c + (partial ? (int64_t)blockIdx.y * M * N * 4 : 0);
+  const __amdgpu_buffer_rsrc_t cr = make_rsrc(cbase, c_bytes);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = n0 + wn * 64 + j * 32 + frow;
+      const int mrow = m0 + wm * 64 + i * 32 + 4 * fkg;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = mrow + 8 * (r >> 2) + (r & 3);
+        const bool ok = m < M && n < N;
+        // take the element as a scalar first: bit-casting the vector element
+        // directly was compiled into a dynamically indexed read that stored
+        // element 0 for every r on this path
+        const float v = acc[i][j][r];
+        if constexpr (sizeof(TO) == 2) {
+          if (!partial) {
+            const uint32_t vo = ok ? (uint32_t)(m * N + n) * 2u : OOB;
+            const TO h = (TO)v;
+            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), cr, (int)vo, 0, 0);
+            continue;
+          }
+        }
+        const uint32_t vo = ok ? (uint32_t)(m * N + n) * 4u : OOB;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), cr, (int)vo, 0, 0);
+      }
+    }
+}
+
+template <typename TI, typename TO>
+void run_wgrad(const ConvParams &p, hipStream_t st) {
+  dim3 grid((unsigned)(((p.M + 127) / 128) * ((p.N + 127) / 128)), (unsigned)p.split);
+  hipLaunchKernelGGL((conv_wgrad_kernel<TI, TO>), grid, dim3(256), 0, st, p);
+}
+template void run_wgrad<bf16_t, bf16_t>(const ConvParams &, hipStream_t);
+template void run_wgrad<bf16_t, float>(const ConvParams &, hipStream_t);
+template void run_wgrad<f16_t, f16_t>(const ConvParams &, hipStream_t);
+template void run_wgrad<f16_t, float>(const ConvParams &, hipStream_t);
+
 template <int PASS, typename TI, typename TO, int FM, int FN, int WGM, int WGN>
 static void launch_fast(const ConvParams &p, int pd, hipStream_t st) {
   constexpr int BM = WGM * FM * 32, BN = WGN * FN * 32;
