@@ -184,6 +184,13 @@ bool fast_path_enabled() {
   return v == 1;
 }
 
+// OWL_AMD_HALO=0 disables the halo kernel (cross-checks, tuning).
+bool halo_enabled() {
+  static int v = -1;
+  if (v < 0) v = env_int("OWL_AMD_HALO", 1);
+  return v != 0;
+}
+
 int fast_shape_for(int N) { return N <= 64 ? 0 : (N <= 128 ? 1 : 2); }
 
 
@@ -212,6 +219,15 @@ bool try_fast(const ConvParams &p, int in_t, int out_t, bool aligned, hipStream_
   if (a_elems * es >= (1LL << 31) || (int64_t)p.kh * p.kw * p.kd * p.Ci * p.Co * es >= (1LL << 31) ||
       c_elems * (int64_t)dtype_size(out_t) >= (1LL << 31))
     return false;
+  // stride-1 2-D problems with 16-bit storage: the halo kernel
+  if (halo_enabled()) {
+    bool done = false;
+    if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) done = run_halo<PASS, bf16_t, bf16_t>(p, st);
+    else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_F32) done = run_halo<PASS, bf16_t, float>(p, st);
+    else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F16) done = run_halo<PASS, f16_t, f16_t>(p, st);
+    else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F32) done = run_halo<PASS, f16_t, float>(p, st);
+    if (done) return true;
+  }
   const int shape = fast_shape_for(p.N);
   if (in_t == OWL_AMD_F32 && out_t == OWL_AMD_F32) run_fast<PASS, float, float>(p, shape, fast_pd(), st);
   else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) run_fast<PASS, bf16_t, bf16_t>(p, shape, fast_pd(), st);

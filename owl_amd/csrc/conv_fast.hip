@@ -560,6 +560,266 @@ template void run_wgrad<bf16_t, float>(const ConvParams &, hipStream_t);
 template void run_wgrad<f16_t, f16_t>(const ConvParams &, hipStream_t);
 template void run_wgrad<f16_t, float>(const ConvParams &, hipStream_t);
 
+// ---------------------------------------------------------------------------
+// Stride-1 2-D convolution with an LDS-staged input halo.
+//
+// A workgroup owns TR full output rows of one image (TR * Wo <= 256 pixels,
+// 8 fragments of 32) and a BN-wide slice of output channels.  For each block
+// of BK input channels it stages the input rows those output rows read --
+// (TR - 1) + (kh - 1) rh + 1 rows of Wo + (kw - 1) rw columns, zero-filled
+// outside the image -- into LDS once, and every tap then reads its A
+// fragments from that halo at a fixed offset.  Each activation is fetched
+// from memory once per channel block instead of once per tap.
+//
+// PASS 0 is the forward convolution.  PASS 1 is the backward input pass
+// written as a forward pass over dy: with stride 1,
+//   dx[h] = sum_a dy[h - p' + a r] w[kh-1-a]^T,  p' = (kh-1) r - p,
+// so the same kernel runs with dy as input, the taps flipped and the kernel
+// read as w[tap][ci][co] with co contiguous.
+// ---------------------------------------------------------------------------
+struct HaloGeom {
+  int Hi, Wi, C;       // input image (x, or dy for PASS 1) and contracted channels
+  int Ho, Wo, N;       // output image and output channels
+  int kh, kw, rh, rw;  // taps and dilation
+  int ph, pw;          // leading pads (of this forward view)
+  int B, TR;           // batch, output rows per tile
+  int wCi, wCo;        // the kernel buffer's [kh][kw][wCi][wCo] sizes
+};
+
+template <int PASS, typename TI, typename TO, int FN>
+__global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI *__restrict__ in,
+                                                           const TI *__restrict__ wt, TO *__restrict__ out) {
+  typedef FastMfma<TI> MF;
+  constexpr int BK = 32, FM = 2, BN = FN * 32, PR = BK + 8;  // halo pixel row: 32 channels + 16 B pad
+  constexpr int LDR = BK + 8;
+  constexpr int MAXH = 44 * 1024 / 2;  // halo capacity in elements (44 KB)
+  __shared__ __attribute__((aligned(16))) TI halo[MAXH];
+  __shared__ __attribute__((aligned(16))) TI Bs[2][BN * LDR];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int frow = lane & 31, fkg = lane >> 5;
+  const int n_tiles = (g.N + BN - 1) / BN;
+  const int row_tiles = (g.Ho + g.TR - 1) / g.TR;
+  const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int nt = bid % n_tiles;
+  const int rt = (bid / n_tiles) % row_tiles;
+  const int b = bid / (n_tiles * row_tiles);
+  const int n0 = nt * BN, i0 = rt * g.TR;
+  const int TRv = min(g.TR, g.Ho - i0);     // output rows in this tile
+  const int P = TRv * g.Wo;                 // output pixels in this tile
+  const int HR = (TRv - 1) + (g.kh - 1) * g.rh + 1;
+  const int HC = (g.Wo - 1) + (g.kw - 1) * g.rw + 1;
+  const int hr0 = i0 - g.ph, hc0 = -g.pw;   // input coordinates of halo (0, 0)
+
+  // per-lane fragment pixels -> halo offset of the tap (0,0) window origin
+  int hoff[FM];
+#pragma unroll
+  for (int f = 0; f < FM; ++f) {
+    int p = wave * FM * 32 + f * 32 + frow;
+    if (p >= P) p = 0;
+    const int ti = p / g.Wo, tj = p - (p / g.Wo) * g.Wo;
+    hoff[f] = (ti * HC + tj) * PR + fkg * 16;
+  }
+
+  const uint32_t in_bytes = (uint32_t)((int64_t)g.B * g.Hi * g.Wi * g.C * 2);
+  const __amdgpu_buffer_rsrc_t inr = make_rsrc(in, in_bytes);
+  const int T = g.kh * g.kw;
+  const int cblocks = g.C / BK;
+
+  // halo staging: chunk q -> pixel q / 4, 16-byte channel chunk q % 4
+  const int hq_total = HR * HC * 4;
+  auto stage_halo = [&](int c0) {
+    for (int q = tid; q < hq_total; q += 256) {
+      const int pix = q >> 2, ch = q & 3;
+      const int r = pix / HC, c = pix - (pix / HC) * HC;
+      const int hi = hr0 + r, wi = hc0 + c;
+      uint32_t vo = OOB;
+      if ((unsigned)hi < (unsigned)g.Hi && (unsigned)wi < (unsigned)g.Wi)
+        vo = (uint32_t)(((b * g.Hi + hi) * g.Wi + wi) * g.C + c0 + ch * 8) * 2u;
+      *(uint4 *)&halo[pix * PR + ch * 8] = bload(inr, vo);
+    }
+  };
+
+  // B staging for (tap, c0): PASS 0 reads w[tap][c0 + k][n] (n-contiguous,
+  // transposed through registers); PASS 1 reads w[flipped tap][n][c0 + k].
+  constexpr int BELEMS = BK * BN / 256;
+  constexpr int BKV = BELEMS / 4;
+  constexpr int BNV = BELEMS / 8;
+  const int f_ng = tid % (BN / 4), f_kg = tid / (BN / 4);
+  const int b_kv = (tid % 4) * 8, b_row = tid / 4;
+  TI rb[BELEMS];
+  auto load_b = [&](int tap, int c0) {
+    if (PASS == 0) {
+      const int n = n0 + f_ng * 4;
+      const TI *src = wt + ((int64_t)tap * g.wCi + c0 + f_kg * BKV) * g.wCo + n;
+#pragma unroll
+      for (int kv = 0; kv < BKV; ++kv) {
+        TI tmp[4];
+        if (n < g.N) {
+          __builtin_memcpy(tmp, src + kv * g.wCo, 4 * sizeof(TI));
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) tmp[v] = (TI)0.0f;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rb[v * BKV + kv] = tmp[v];
+      }
+    } else {
+      const int ft = T - 1 - tap;  // flipped tap
+#pragma unroll
+      for (int i = 0; i < BNV; ++i) {
+        const int n = n0 + b_row + i * 64;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (n < g.N) v = *(const uint4 *)(wt + ((int64_t)ft * g.wCi + n) * g.wCo + c0 + b_kv);
+        __builtin_memcpy(&rb[i * 8], &v, 16);
+      }
+    }
+  };
+  auto store_b = [&](int buf) {
+    TI *dst = Bs[buf];
+    if (PASS == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        __builtin_memcpy(&dst[(f_ng * 4 + v) * LDR + f_kg * BKV], &rb[v * BKV], BKV * sizeof(TI));
+    } else {
+#pragma unroll
+      for (int i = 0; i < BNV; ++i) __builtin_memcpy(&dst[(b_row + i * 64) * LDR + b_kv], &rb[i * 8], 16);
+    }
+  };
+
+  f32x16 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  const int nk = cblocks * T;
+  int cb = 0, tap = 0;       // current step
+  load_b(0, 0);
+  stage_halo(0);
+  store_b(0);
+  __syncthreads();
+  const int fb_off = (frow)*LDR + fkg * 16;
+  for (int ks = 0; ks < nk; ++ks) {
+    const int bcur = ks & 1;
+    // next step's (tap, channel block)
+    int ntap = tap + 1, ncb = cb;
+    if (ntap == T) {
+      ntap = 0;
+      ++ncb;
+    }
+    const bool more = ks + 1 < nk;
+    if (more) load_b(ntap, ncb * BK);
+    // A from the halo at this tap's offset, B from LDS
+    const int a = tap / g.kw, c = tap - (tap / g.kw) * g.kw;
+    const int toff = (a * g.rh * HC + c * g.rw) * PR;
+    uint4 fa[FM][2], fb[FN][2];
+#pragma unroll
+    for (int f = 0; f < FM; ++f) {
+      const uint4 *src = (const uint4 *)&halo[hoff[f] + toff];
+      fa[f][0] = src[0];
+      fa[f][1] = src[1];
+    }
+    const TI *bs = Bs[bcur] + fb_off;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const uint4 *src = (const uint4 *)&bs[j * 32 * LDR];
+      fb[j][0] = src[0];
+      fb[j][1] = src[1];
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i][s2], fb[j][s2], acc[i][j]);
+    if (more) store_b(bcur ^ 1);
+    if (more && ncb != cb) {
+      __syncthreads();  // every wave is done with this channel block's halo
+      stage_halo(ncb * BK);
+    }
+    __syncthreads();
+    tap = ntap;
+    cb = ncb;
+  }
+
+  // ---- epilogue: output pixel (b, i0 + p / Wo, p % Wo), channel n ----
+  const uint32_t o_bytes = (uint32_t)((int64_t)g.B * g.Ho * g.Wo * g.N * sizeof(TO));
+  const __amdgpu_buffer_rsrc_t orr = make_rsrc(out, o_bytes);
+  const int obase = (b * g.Ho + i0) * g.Wo;  // first output pixel of the tile
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int n = n0 + j * 32 + frow;
+      const int prow = wave * FM * 32 + i * 32 + 4 * fkg;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int pp = prow + 8 * (r >> 2) + (r & 3);
+        const bool ok = pp < P && n < g.N;
+        const uint32_t vo = ok ? (uint32_t)((obase + pp) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
+        const float v = acc[i][j][r];
+        if constexpr (sizeof(TO) == 2) {
+          const TO h = (TO)v;
+          __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), orr, (int)vo, 0, 0);
+        } else {
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), orr, (int)vo, 0, 0);
+        }
+      }
+    }
+}
+
+// Runs the halo kernel if the problem fits it; returns false otherwise.
+template <int PASS, typename TI, typename TO>
+bool run_halo(const ConvParams &p, hipStream_t st) {
+  if (sizeof(TI) != 2) return false;
+  HaloGeom g;
+  if (p.D != 1 || p.kd != 1 || p.Do != 1) return false;
+  if (p.sh != 1 || p.sw != 1) return false;
+  if (PASS == 0) {
+    g.Hi = p.H; g.Wi = p.W; g.C = p.Ci; g.Ho = p.Ho; g.Wo = p.Wo; g.N = p.Co;
+    g.ph = p.ph; g.pw = p.pw;
+  } else {
+    g.Hi = p.Ho; g.Wi = p.Wo; g.C = p.Co; g.Ho = p.H; g.Wo = p.W; g.N = p.Ci;
+    g.ph = (p.kh - 1) * p.rh - p.ph;
+    g.pw = (p.kw - 1) * p.rw - p.pw;
+    if (g.ph < 0 || g.pw < 0) return false;
+  }
+  g.kh = p.kh; g.kw = p.kw; g.rh = p.rh; g.rw = p.rw;
+  g.B = p.B; g.wCi = p.Ci; g.wCo = p.Co;
+  if (g.C % 32 != 0 || g.N % 8 != 0 || g.Wo > 256) return false;
+  g.TR = 256 / g.Wo;
+  if (g.TR > g.Ho) g.TR = g.Ho;
+  const int HR = (g.TR - 1) + (g.kh - 1) * g.rh + 1, HC = (g.Wo - 1) + (g.kw - 1) * g.rw + 1;
+  if (HR * HC * 40 > 44 * 1024 / 2) return false;  // halo must fit its LDS
+  if ((int64_t)g.B * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * sizeof(TO) >= (1LL << 31))
+    return false;
+  const int FN = g.N <= 64 ? 2 : 4;
+  const int BN = FN * 32;
+  const int64_t blocks = (int64_t)g.B * ((g.Ho + g.TR - 1) / g.TR) * ((g.N + BN - 1) / BN);
+  const TI *in = (const TI *)p.a;
+  const TI *w = (const TI *)p.b;
+  if (FN == 2)
+    hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2>), dim3((unsigned)blocks), dim3(256), 0, st, g, in, w,
+                       (TO *)p.c);
+  else
+    hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 4>), dim3((unsigned)blocks), dim3(256), 0, st, g, in, w,
+                       (TO *)p.c);
+  return true;
+}
+#define OWL_HALO(PASS, TI, TO) template bool run_halo<PASS, TI, TO>(const ConvParams &, hipStream_t);
+OWL_HALO(0, bf16_t, bf16_t)
+OWL_HALO(0, bf16_t, float)
+OWL_HALO(1, bf16_t, bf16_t)
+OWL_HALO(1, bf16_t, float)
+OWL_HALO(0, f16_t, f16_t)
+OWL_HALO(0, f16_t, float)
+OWL_HALO(1, f16_t, f16_t)
+OWL_HALO(1, f16_t, float)
+#undef OWL_HALO
+
 template <int PASS, typename TI, typename TO, int FM, int FN, int WGM, int WGN>
 static void launch_fast(const ConvParams &p, int pd, hipStream_t st) {
   constexpr int BM = WGM * FM * 32, BN = WGN * FN * 32;
