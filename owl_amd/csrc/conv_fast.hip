@@ -586,7 +586,9 @@ struct HaloGeom {
   int wCi, wCo;        // the kernel buffer's [kh][kw][wCi][wCo] sizes
 };
 
-template <int PASS, typename TI, typename TO, int FN>
+// KROWS: the kernel operand is laid out [tap][n][k] (k contiguous): the
+// backward-input view of w, or forward weights packed by pack_weights_kernel.
+template <int PASS, typename TI, typename TO, int FN, bool KROWS>
 __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI *__restrict__ in,
                                                            const TI *__restrict__ wt, TO *__restrict__ out) {
   typedef FastMfma<TI> MF;
@@ -649,7 +651,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   const int b_kv = (tid % 4) * 8, b_row = tid / 4;
   TI rb[BELEMS];
   auto load_b = [&](int tap, int c0) {
-    if (PASS == 0) {
+    if (!KROWS) {
       const int n = n0 + f_ng * 4;
       const TI *src = wt + ((int64_t)tap * g.wCi + c0 + f_kg * BKV) * g.wCo + n;
 #pragma unroll
@@ -665,19 +667,20 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
         for (int v = 0; v < 4; ++v) rb[v * BKV + kv] = tmp[v];
       }
     } else {
-      const int ft = T - 1 - tap;  // flipped tap
+      // rows of k: [tap][n][k] with N rows of C values per tap
+      const int ft = PASS == 1 ? T - 1 - tap : tap;  // backward input flips the taps
 #pragma unroll
       for (int i = 0; i < BNV; ++i) {
         const int n = n0 + b_row + i * 64;
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (n < g.N) v = *(const uint4 *)(wt + ((int64_t)ft * g.wCi + n) * g.wCo + c0 + b_kv);
+        if (n < g.N) v = *(const uint4 *)(wt + ((int64_t)ft * g.N + n) * g.C + c0 + b_kv);
         __builtin_memcpy(&rb[i * 8], &v, 16);
       }
     }
   };
   auto store_b = [&](int buf) {
     TI *dst = Bs[buf];
-    if (PASS == 0) {
+    if (!KROWS) {
 #pragma unroll
       for (int v = 0; v < 4; ++v)
         __builtin_memcpy(&dst[(f_ng * 4 + v) * LDR + f_kg * BKV], &rb[v * BKV], BKV * sizeof(TI));
@@ -771,11 +774,10 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     }
 }
 
-// Runs the halo kernel if the problem fits it; returns false otherwise.
-template <int PASS, typename TI, typename TO>
-bool run_halo(const ConvParams &p, hipStream_t st) {
-  if (sizeof(TI) != 2) return false;
-  HaloGeom g;
+// Geometry of the halo kernel for a problem, or false if it does not fit.
+template <int PASS>
+static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom &g) {
+  if (in_bytes != 2) return false;
   if (p.D != 1 || p.kd != 1 || p.Do != 1) return false;
   if (p.sh != 1 || p.sw != 1) return false;
   if (PASS == 0) {
@@ -794,22 +796,67 @@ bool run_halo(const ConvParams &p, hipStream_t st) {
   if (g.TR > g.Ho) g.TR = g.Ho;
   const int HR = (g.TR - 1) + (g.kh - 1) * g.rh + 1, HC = (g.Wo - 1) + (g.kw - 1) * g.rw + 1;
   if (HR * HC * 40 > 44 * 1024 / 2) return false;  // halo must fit its LDS
-  if ((int64_t)g.B * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * sizeof(TO) >= (1LL << 31))
+  if ((int64_t)g.B * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * out_bytes >= (1LL << 31))
     return false;
+  return true;
+}
+
+bool halo_fits(const ConvParams &p, int pass, int in_bytes, int out_bytes) {
+  HaloGeom g;
+  return pass == 0 ? halo_geom<0>(p, in_bytes, out_bytes, g) : halo_geom<1>(p, in_bytes, out_bytes, g);
+}
+
+// Forward weights [tap][ci][co] -> [tap][co][ci] (k-contiguous rows), so the
+// forward halo kernel stages them with plain 16-byte copies.
+template <typename TI>
+__global__ __launch_bounds__(256) void pack_weights_kernel(const TI *__restrict__ w, TI *__restrict__ wp, int T,
+                                                           int Ci, int Co) {
+  __shared__ TI tile[32][33];
+  const int t = blockIdx.z;
+  const int c0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int c = c0 + r, n = n0 + tx;
+    if (c < Ci && n < Co) tile[r][tx] = w[((int64_t)t * Ci + c) * Co + n];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int n = n0 + r, c = c0 + tx;
+    if (c < Ci && n < Co) wp[((int64_t)t * Co + n) * Ci + c] = tile[tx][r];
+  }
+}
+
+// Runs the halo kernel if the problem fits it; returns false otherwise.
+// wpack: scratch for the packed forward weights (T*Ci*Co elements) or null
+// to stage the forward weights through the register transpose instead.
+template <int PASS, typename TI, typename TO>
+bool run_halo(const ConvParams &p, void *wpack, hipStream_t st) {
+  HaloGeom g;
+  if (!halo_geom<PASS>(p, (int)sizeof(TI), (int)sizeof(TO), g)) return false;
   const int FN = g.N <= 64 ? 2 : 4;
   const int BN = FN * 32;
   const int64_t blocks = (int64_t)g.B * ((g.Ho + g.TR - 1) / g.TR) * ((g.N + BN - 1) / BN);
   const TI *in = (const TI *)p.a;
   const TI *w = (const TI *)p.b;
-  if (FN == 2)
-    hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2>), dim3((unsigned)blocks), dim3(256), 0, st, g, in, w,
-                       (TO *)p.c);
-  else
-    hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 4>), dim3((unsigned)blocks), dim3(256), 0, st, g, in, w,
-                       (TO *)p.c);
+  bool krows = PASS == 1;
+  if (PASS == 0 && wpack) {
+    const int T = p.kh * p.kw;
+    dim3 pg((unsigned)((p.Co + 31) / 32), (unsigned)((p.Ci + 31) / 32), (unsigned)T);
+    hipLaunchKernelGGL((pack_weights_kernel<TI>), pg, dim3(256), 0, st, w, (TI *)wpack, T, p.Ci, p.Co);
+    w = (const TI *)wpack;
+    krows = true;
+  }
+  dim3 grid((unsigned)blocks), blk(256);
+  if (FN == 2) {
+    if (krows) hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2, true>), grid, blk, 0, st, g, in, w, (TO *)p.c);
+    else hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2, false>), grid, blk, 0, st, g, in, w, (TO *)p.c);
+  } else {
+    if (krows) hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 4, true>), grid, blk, 0, st, g, in, w, (TO *)p.c);
+    else hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 4, false>), grid, blk, 0, st, g, in, w, (TO *)p.c);
+  }
   return true;
 }
-#define OWL_HALO(PASS, TI, TO) template bool run_halo<PASS, TI, TO>(const ConvParams &, hipStream_t);
+#define OWL_HALO(PASS, TI, TO) template bool run_halo<PASS, TI, TO>(const ConvParams &, void *, hipStream_t);
 OWL_HALO(0, bf16_t, bf16_t)
 OWL_HALO(0, bf16_t, float)
 OWL_HALO(1, bf16_t, bf16_t)

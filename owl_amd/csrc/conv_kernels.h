@@ -51,8 +51,10 @@ void run_fast(const ConvParams &p, int shape, int pd, hipStream_t st);
 int fast_bk(int elem_bytes);
 // conv_fast.hip: stride-1 2-D forward / backward input with an LDS-staged
 // input halo (16-bit storage); returns false when the problem does not fit.
+// wpack: scratch of T*Ci*Co elements for the packed forward weights (or null).
 template <int PASS, typename TI, typename TO>
-bool run_halo(const ConvParams &p, hipStream_t st);
+bool run_halo(const ConvParams &p, void *wpack, hipStream_t st);
+bool halo_fits(const ConvParams &p, int pass, int in_bytes, int out_bytes);
 // conv_fast.hip: backward kernel for 16-bit storage via transposing LDS reads
 // (tile 128 x 128 x 32 pixels; p.split / p.k_per_split as for run_igemm<2>).
 template <typename TI, typename TO>
