@@ -675,8 +675,64 @@ void run_igemm(const ConvParams &p, bool aligned, hipStream_t st) {
   launch_igemm<PASS, TI, TO>(p, aligned, st);
 }
 
+// Wide form for n % 4 == 0: a block covers 64 outputs (16 groups of 4) and
+// spreads the splits over 16 thread groups (group g sums splits g, g + 16,
+// ...); the 16 group sums are then combined in group order, so the result
+// is fixed for a given (n, splits) and every launch gives the same bits.
+template <typename TA, typename TO>
+__global__ __launch_bounds__(256) void splitk_reduce_wide_kernel(const TA *__restrict__ ws, TO *__restrict__ out,
+                                                                 int64_t n, int splits) {
+  __shared__ TA red[2][16][16 * 4];
+  const int c = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int64_t e0 = ((int64_t)blockIdx.x * 16 + c) * 4;
+  const bool live = e0 < n;
+  TA s[4] = {0, 0, 0, 0}, comp[4] = {0, 0, 0, 0};
+  if (live) {
+    int k = g;
+    for (; k + 48 < splits; k += 64) {
+      TA v[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) __builtin_memcpy(v[u], ws + (int64_t)(k + 16 * u) * n + e0, 4 * sizeof(TA));
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) neumaier(s[q], comp[q], v[u][q]);
+    }
+    for (; k < splits; k += 16) {
+      TA v[4];
+      __builtin_memcpy(v, ws + (int64_t)k * n + e0, 4 * sizeof(TA));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) neumaier(s[q], comp[q], v[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    red[0][g][c * 4 + q] = s[q];
+    red[1][g][c * 4 + q] = comp[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int cc = threadIdx.x >> 2, q = threadIdx.x & 3;
+    const int64_t e = ((int64_t)blockIdx.x * 16 + cc) * 4 + q;
+    if (e < n) {
+      TA t = 0, tc = 0;
+      for (int gg = 0; gg < 16; ++gg) {
+        neumaier(t, tc, red[0][gg][threadIdx.x]);
+        tc += red[1][gg][threadIdx.x];
+      }
+      out[e] = (TO)(t + tc);
+    }
+  }
+}
+
 template <typename TA, typename TO>
 void run_splitk_reduce(const TA *ws, TO *out, int64_t n, int splits, hipStream_t st) {
+  if (n % 4 == 0) {
+    const int64_t blocks = (n + 63) / 64;
+    hipLaunchKernelGGL((splitk_reduce_wide_kernel<TA, TO>), dim3((unsigned)blocks), dim3(256), 0, st, ws, out, n,
+                       splits);
+    return;
+  }
   int64_t blocks = (n + 1023) / 1024;
   hipLaunchKernelGGL((splitk_reduce_kernel<TA, TO>), dim3((unsigned)blocks), dim3(256), 0, st,
                      ws, out, n, splits);

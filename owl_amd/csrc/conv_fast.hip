@@ -440,25 +440,52 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   const int bn = n0 + chunk * 8;
   const bool b_col_ok = bn < N;
 
+  // Output-pixel coordinates (b, i, j, l) of the thread's two loader pixels
+  // for the next load.  They advance by BK pixels per K-step: BK is split
+  // once into the same mixed radix (Do, Wo, Ho) and added digit by digit
+  // with carries, so the loop needs no integer division.
+  int sl, sj, si, sb;
+  {
+    int t = BK;
+    sl = t % p.Do; t /= p.Do;
+    sj = t % p.Wo; t /= p.Wo;
+    si = t % p.Ho; sb = t / p.Ho;
+  }
+  int pk[2], pl[2], pj[2], pi[2], pb[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    int t = (int)kbeg + kpix0 + 16 * h;
+    pk[h] = t;
+    pl[h] = t % p.Do; t /= p.Do;
+    pj[h] = t % p.Wo; t /= p.Wo;
+    pi[h] = t % p.Ho; pb[h] = t / p.Ho;
+  }
   uint4 ra[2], rb[2];
-  auto load = [&](int64_t k0) {
+  auto load = [&]() {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int64_t k = k0 + kpix0 + 16 * h;
       uint32_t vo = OOB, go = OOB;
-      if (k < kend) {
-        int t = (int)k;
-        const int l = t % p.Do; t /= p.Do;
-        const int j = t % p.Wo; t /= p.Wo;
-        const int i = t % p.Ho;
-        const int b = t / p.Ho;
-        const int hh = i * p.sh + dh, ww = j * p.sw + dw, d = l * p.sd + dd;
+      if (pk[h] < kend) {
+        const int hh = pi[h] * p.sh + dh, ww = pj[h] * p.sw + dw, d = pl[h] * p.sd + dd;
         if (a_row_ok && (unsigned)hh < (unsigned)p.H && (unsigned)ww < (unsigned)p.W && (unsigned)d < (unsigned)p.D)
-          vo = (uint32_t)((((b * p.H + hh) * p.W + ww) * p.D + d) * p.Ci + a_ci) * 2u;
-        if (b_col_ok) go = (uint32_t)((int)k * p.Co + bn) * 2u;
+          vo = (uint32_t)((((pb[h] * p.H + hh) * p.W + ww) * p.D + d) * p.Ci + a_ci) * 2u;
+        if (b_col_ok) go = (uint32_t)(pk[h] * p.Co + bn) * 2u;
       }
       ra[h] = bload(xr, vo);
       rb[h] = bload(gr, go);
+      // advance to the pixel BK further on
+      pk[h] += BK;
+      int c;
+      pl[h] += sl;
+      c = pl[h] >= p.Do;
+      if (c) pl[h] -= p.Do;
+      pj[h] += sj + c;
+      c = pj[h] >= p.Wo;
+      if (c) pj[h] -= p.Wo;
+      pi[h] += si + c;
+      c = pi[h] >= p.Ho;
+      if (c) pi[h] -= p.Ho;
+      pb[h] += sb + c;
     }
   };
   auto store = [&](int st) {
@@ -485,13 +512,13 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   const int tr_off = ((j16 >> 2) + 8 * (lane >> 5)) * RS + 2 * (16 * ((lane >> 4) & 1) + 4 * (j16 & 3));
 
   if (nk > 0) {
-    load(kbeg);
+    load();
     store(0);
     __syncthreads();
     for (int ks = 0; ks < nk; ++ks) {
       const int cur = ks & 1;
       const bool more = ks + 1 < nk;
-      if (more) load(kbeg + (int64_t)(ks + 1) * BK);
+      if (more) load();
       const char *As = lds[cur][0], *Bs = lds[cur][1];
 #pragma unroll
       for (int s = 0; s < 2; ++s) {  // two 16-pixel MFMA steps
@@ -594,8 +621,12 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   typedef FastMfma<TI> MF;
   constexpr int BK = 32, FM = 2, BN = FN * 32, PR = BK + 8;  // halo pixel row: 32 channels + 16 B pad
   constexpr int LDR = BK + 8;
-  constexpr int MAXH = 44 * 1024 / 2;  // halo capacity in elements (44 KB)
-  __shared__ __attribute__((aligned(16))) TI halo[MAXH];
+  // Two halo buffers of HBUF pixels: while one channel block's taps run,
+  // the next block's halo is loaded into registers (HQ 16-byte chunks per
+  // thread) and written to the other buffer.  A halo above HBUF pixels uses
+  // both buffers as one and is restaged between channel blocks.
+  constexpr int HQ = 6, HBUF = 352;
+  __shared__ __attribute__((aligned(16))) TI halo[2 * HBUF * PR];
   __shared__ __attribute__((aligned(16))) TI Bs[2][BN * LDR];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -626,19 +657,48 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   const uint32_t in_bytes = (uint32_t)((int64_t)g.B * g.Hi * g.Wi * g.C * 2);
   const __amdgpu_buffer_rsrc_t inr = make_rsrc(in, in_bytes);
   const int T = g.kh * g.kw;
+  const __amdgpu_buffer_rsrc_t wr = make_rsrc(wt, (uint32_t)(T * g.N * g.C * 2));
   const int cblocks = g.C / BK;
 
-  // halo staging: chunk q -> pixel q / 4, 16-byte channel chunk q % 4
+  // halo staging: chunk q -> pixel q / 4, 16-byte channel chunk q % 4.
+  // Per-thread source offsets (channel 0 of the block) are fixed for the
+  // tile; OOB marks padding, which the buffer loads fill with zeros.
   const int hq_total = HR * HC * 4;
-  auto stage_halo = [&](int c0) {
-    for (int q = tid; q < hq_total; q += 256) {
-      const int pix = q >> 2, ch = q & 3;
-      const int r = pix / HC, c = pix - (pix / HC) * HC;
-      const int hi = hr0 + r, wi = hc0 + c;
-      uint32_t vo = OOB;
-      if ((unsigned)hi < (unsigned)g.Hi && (unsigned)wi < (unsigned)g.Wi)
-        vo = (uint32_t)(((b * g.Hi + hi) * g.Wi + wi) * g.C + c0 + ch * 8) * 2u;
-      *(uint4 *)&halo[pix * PR + ch * 8] = bload(inr, vo);
+  const bool dbl = HR * HC <= HBUF;
+  const int nbatch = (hq_total + HQ * 256 - 1) / (HQ * 256);
+  auto chunk_src = [&](int q) -> uint32_t {
+    const int pix = q >> 2, ch = q & 3;
+    const int r = pix / HC, c = pix - (pix / HC) * HC;
+    const int hi = hr0 + r, wi = hc0 + c;
+    if (q < hq_total && (unsigned)hi < (unsigned)g.Hi && (unsigned)wi < (unsigned)g.Wi)
+      return (uint32_t)(((b * g.Hi + hi) * g.Wi + wi) * g.C + ch * 8) * 2u;
+    return OOB;
+  };
+  uint32_t hsrc[HQ];
+#pragma unroll
+  for (int u = 0; u < HQ; ++u) hsrc[u] = chunk_src(tid + u * 256);
+  uint4 hreg[HQ];
+  auto halo_issue = [&](int c0, int batch) {
+#pragma unroll
+    for (int u = 0; u < HQ; ++u) {
+      const int q = batch * HQ * 256 + tid + u * 256;
+      const uint32_t so = batch == 0 ? hsrc[u] : chunk_src(q);
+      hreg[u] = bload(inr, so == OOB ? OOB : so + (uint32_t)c0 * 2u);  // q >= hq_total: OOB
+    }
+  };
+  auto halo_commit = [&](TI *hb, int batch) {
+#pragma unroll
+    for (int u = 0; u < HQ; ++u) {
+      const int q = batch * HQ * 256 + tid + u * 256;
+      if (q < hq_total) *(uint4 *)&hb[(q >> 2) * PR + (q & 3) * 8] = hreg[u];
+    }
+  };
+  auto stage_halo = [&](TI *hb, int c0) {  // every batch (at most two), loads first
+    halo_issue(c0, 0);
+    halo_commit(hb, 0);
+    if (nbatch > 1) {
+      halo_issue(c0, 1);
+      halo_commit(hb, 1);
     }
   };
 
@@ -649,8 +709,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   constexpr int BNV = BELEMS / 8;
   const int f_ng = tid % (BN / 4), f_kg = tid / (BN / 4);
   const int b_kv = (tid % 4) * 8, b_row = tid / 4;
-  TI rb[BELEMS];
-  auto load_b = [&](int tap, int c0) {
+  TI rb0[BELEMS], rb1[BELEMS];  // register ring: B tiles two steps ahead
+  auto load_b = [&](TI(&rb)[BELEMS], int tap, int c0) {
     if (!KROWS) {
       const int n = n0 + f_ng * 4;
       const TI *src = wt + ((int64_t)tap * g.wCi + c0 + f_kg * BKV) * g.wCo + n;
@@ -671,14 +731,16 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       const int ft = PASS == 1 ? T - 1 - tap : tap;  // backward input flips the taps
 #pragma unroll
       for (int i = 0; i < BNV; ++i) {
+        // unconditional loads (OOB rows read as zeros) keep the number of
+        // loads in flight static, so waits stay precise across the loop
         const int n = n0 + b_row + i * 64;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (n < g.N) v = *(const uint4 *)(wt + ((int64_t)ft * g.N + n) * g.C + c0 + b_kv);
+        const uint32_t wo = n < g.N ? (uint32_t)((ft * g.N + n) * g.C + c0 + b_kv) * 2u : OOB;
+        const uint4 v = bload(wr, wo);
         __builtin_memcpy(&rb[i * 8], &v, 16);
       }
     }
   };
-  auto store_b = [&](int buf) {
+  auto store_b = [&](const TI(&rb)[BELEMS], int buf) {
     TI *dst = Bs[buf];
     if (!KROWS) {
 #pragma unroll
@@ -699,29 +761,49 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
   const int nk = cblocks * T;
-  int cb = 0, tap = 0;       // current step
-  load_b(0, 0);
-  stage_halo(0);
-  store_b(0);
+  // load cursor: (tap, channel block) of the next B tile to fetch.  Past the
+  // last step it fetches tile (0, 0) again, so every step issues the same
+  // loads and the waits on them stay exact.
+  int ltap = 0, lcb = 0;
+  auto next_load = [&](TI(&rb)[BELEMS]) {
+    const bool in = lcb < cblocks;
+    load_b(rb, in ? ltap : 0, in ? lcb * BK : 0);
+    if (++ltap == T) {
+      ltap = 0;
+      ++lcb;
+    }
+  };
+  next_load(rb0);  // step 0
+  next_load(rb1);  // step 1
+  stage_halo(halo, 0);
+  store_b(rb0, 0);
+  next_load(rb0);  // step 2
   __syncthreads();
   const int fb_off = (frow)*LDR + fkg * 16;
-  for (int ks = 0; ks < nk; ++ks) {
+  int cb = 0, tap = 0, ta = 0, tc = 0;  // current step: channel block, tap = (ta, tc)
+  // one K-step; rnext holds B(ks + 1) and is refilled with B(ks + 3)
+  auto step = [&](int ks, TI(&rnext)[BELEMS]) {
     const int bcur = ks & 1;
     // next step's (tap, channel block)
-    int ntap = tap + 1, ncb = cb;
+    int ntap = tap + 1, ncb = cb, nta = ta, ntc = tc + 1;
+    if (ntc == g.kw) {
+      ntc = 0;
+      ++nta;
+    }
     if (ntap == T) {
       ntap = 0;
+      nta = 0;
       ++ncb;
     }
     const bool more = ks + 1 < nk;
-    if (more) load_b(ntap, ncb * BK);
+    if (dbl && tap == 0 && cb + 1 < cblocks) halo_issue((cb + 1) * BK, 0);
     // A from the halo at this tap's offset, B from LDS
-    const int a = tap / g.kw, c = tap - (tap / g.kw) * g.kw;
-    const int toff = (a * g.rh * HC + c * g.rw) * PR;
+    const TI *hcur = halo + (dbl ? (cb & 1) * HBUF * PR : 0);
+    const int toff = (ta * g.rh * HC + tc * g.rw) * PR;
     uint4 fa[FM][2], fb[FN][2];
 #pragma unroll
     for (int f = 0; f < FM; ++f) {
-      const uint4 *src = (const uint4 *)&halo[hoff[f] + toff];
+      const uint4 *src = (const uint4 *)&hcur[hoff[f] + toff];
       fa[f][0] = src[0];
       fa[f][1] = src[1];
     }
@@ -738,14 +820,27 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i][s2], fb[j][s2], acc[i][j]);
-    if (more) store_b(bcur ^ 1);
+    if (more) store_b(rnext, bcur ^ 1);
+    next_load(rnext);
     if (more && ncb != cb) {
-      __syncthreads();  // every wave is done with this channel block's halo
-      stage_halo(ncb * BK);
+      if (dbl) {
+        // the other buffer was last read before the previous block's final
+        // barrier; the barrier below publishes the new halo
+        halo_commit(halo + (ncb & 1) * HBUF * PR, 0);
+      } else {
+        __syncthreads();  // every wave is done with this channel block's halo
+        stage_halo(halo, ncb * BK);
+      }
     }
     __syncthreads();
     tap = ntap;
+    ta = nta;
+    tc = ntc;
     cb = ncb;
+  };
+  for (int ks = 0; ks < nk; ks += 2) {
+    step(ks, rb1);
+    if (ks + 1 < nk) step(ks + 1, rb0);
   }
 
   // ---- epilogue: output pixel (b, i0 + p / Wo, p % Wo), channel n ----
@@ -795,9 +890,10 @@ static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom
   g.TR = 256 / g.Wo;
   if (g.TR > g.Ho) g.TR = g.Ho;
   const int HR = (g.TR - 1) + (g.kh - 1) * g.rh + 1, HC = (g.Wo - 1) + (g.kw - 1) * g.rw + 1;
-  if (HR * HC * 40 > 44 * 1024 / 2) return false;  // halo must fit its LDS
-  if ((int64_t)g.B * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * out_bytes >= (1LL << 31))
-    return false;
+  if (HR * HC > 2 * 352) return false;  // halo must fit its LDS (two 352-pixel buffers)
+  if ((int64_t)g.B * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * out_bytes >= (1LL << 31) ||
+      (int64_t)g.kh * g.kw * g.C * g.N * 2 >= (1LL << 31))
+    return false;  // 32-bit buffer offsets
   return true;
 }
 
