@@ -149,6 +149,37 @@ __device__ __forceinline__ void tap_decode(int t, int kw, int kd, int &a, int &c
   e = r - c * kd;
 }
 
+// A mixed-radix counter: digits d0 (fastest), d1, d2 with radices r0..r2
+// and an unbounded top digit.  step() adds a fixed amount, given as digits
+// of the same radices, with one carry per digit -- the loaders advance their
+// k position by a K-step this way instead of dividing every step.
+struct Radix3 {
+  int d0, d1, d2, r0, r1, r2, s0, s1, s2;
+  int64_t top, stop;
+  __device__ __forceinline__ void init(int64_t v, int64_t step, int ra, int rb, int rc) {
+    r0 = ra; r1 = rb; r2 = rc;
+    d0 = (int)(v % r0); v /= r0;
+    d1 = (int)(v % r1); v /= r1;
+    d2 = (int)(v % r2); top = v / r2;
+    s0 = (int)(step % r0); step /= r0;
+    s1 = (int)(step % r1); step /= r1;
+    s2 = (int)(step % r2); stop = step / r2;
+  }
+  __device__ __forceinline__ void step() {
+    int c;
+    d0 += s0;
+    c = d0 >= r0;
+    if (c) d0 -= r0;
+    d1 += s1 + c;
+    c = d1 >= r1;
+    if (c) d1 -= r1;
+    d2 += s2 + c;
+    c = d2 >= r2;
+    if (c) d2 -= r2;
+    top += stop + c;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // The kernel.  PASS: 0 forward, 1 backward input, 2 backward kernel.
 // ALIGNED: channel counts are multiples of VEC and pointers 16-byte aligned,
@@ -251,6 +282,13 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
     tap_decode(t, p.kw, p.kd, w_tap_a, w_tap_c, w_tap_e);
   }
 
+  // k positions of this thread's loads, stepped by BK per K-step:
+  // passes 0/1: k = ((ta * kw + tc) * kd + te) * C + c  -> (c, te, tc; ta)
+  // pass 2:     k = ((b * Ho + i) * Wo + j) * Do + l    -> (l, j, i; b)
+  Radix3 kc;
+  if (PASS == 0 || PASS == 1) kc.init(kbeg + kvA, BK, PASS == 0 ? p.Ci : p.Co, p.kd, p.kw);
+  else kc.init(kbeg + kgp * TL::MN_KV, BK, p.Do, p.Wo, p.Ho);
+
   TI ra[ELEMS], rb[ELEMS];
 
   // ---------------- global -> registers ----------------
@@ -260,10 +298,7 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
       if (ALIGNED) {
         int64_t k = k0 + kvA;
         bool kok = k < kend;
-        int t = kok ? (int)(k / C) : 0;
-        int c = (int)(k - (int64_t)t * C);
-        int ta, tc, te;
-        tap_decode(t, p.kw, p.kd, ta, tc, te);
+        const int c = kc.d0, te = kc.d1, tc = kc.d2, ta = (int)kc.top;
 #pragma unroll
         for (int i = 0; i < AR; ++i) {
           bool ok = kok && a_ok[i];
@@ -317,12 +352,9 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
       // position for the tap; thread loads VEC rows x MN_KV pixels.
       constexpr int KV = TL::MN_KV;
       int64_t k = k0 + kgp * KV;
-      // decode the first pixel, then step
-      int64_t t = k < kend ? k : 0;
-      int l = (int)(t % p.Do); t /= p.Do;
-      int j = (int)(t % p.Wo); t /= p.Wo;
-      int ii = (int)(t % p.Ho);
-      int64_t b = t / p.Ho;
+      // the first pixel's coordinates, then step pixel by pixel
+      int l = kc.d0, j = kc.d1, ii = kc.d2;
+      int64_t b = kc.top;
 #pragma unroll
       for (int kv = 0; kv < KV; ++kv) {
         bool kok = (k + kv) < kend;
@@ -385,9 +417,9 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
           bool ok = k < kend && n < N;
           uint4 v = make_uint4(0, 0, 0, 0);
           if (ok) {
-            int t = (int)(k / p.Co);
-            int co = (int)(k - (int64_t)t * p.Co);
-            v = *(const uint4 *)(Bg + ((int64_t)t * p.Ci + n) * p.Co + co);
+            // same k as the A load: tap and channel from its counter
+            const int t = ((int)kc.top * p.kw + kc.d2) * p.kd + kc.d1;
+            v = *(const uint4 *)(Bg + ((int64_t)t * p.Ci + n) * p.Co + kc.d0);
           }
           __builtin_memcpy(&rb[i * VEC], &v, 16);
         } else {
@@ -506,6 +538,7 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
   if (nk > 0) {
     load_a(kbeg);
     load_b(kbeg);
+    kc.step();
     store_tiles(0);
     __syncthreads();
     for (int64_t kt = 0; kt < nk; ++kt) {
@@ -514,6 +547,7 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
       if (more) {
         load_a(kbeg + (kt + 1) * BK);
         load_b(kbeg + (kt + 1) * BK);
+        kc.step();
       }
       compute(cur);
       if (more) store_tiles(cur ^ 1);
