@@ -26,6 +26,15 @@ B, H, W, CI, CO, K = 64, 56, 56, 64, 128, 3
 FLOP_PASS = 2.0 * B * H * W * CO * K * K * CI  # 29.595 GFLOP
 PEAK = {"bf16": 2.5e15, "f16": 2.5e15, "f32": 157.3e12}
 HBM_PEAK = 8.0e12
+# HBM bytes per launch of the BF16 step's kernels, from two rocprofv3 --pmc
+# runs (FETCH_SIZE, then WRITE_SIZE) scaled by what the same counters read
+# for a plain copy of known size (x2.0 and x1.0); the runs and their command
+# are in profiles/conv2d_vgg_layer_bf16_hbm_traffic.md
+TRAFFIC = {
+    ("conv_halo_kernel", "fwd"): 13634.1 * 2048 + 50199.2 * 1024,
+    ("conv_halo_kernel", "bwd_input"): 26166.2 * 2048 + 25088.4 * 1024,
+    ("conv_wgrad_kernel", "bwd_kernel"): 187679.5 * 2048 + 29664.0 * 1024,
+}
 
 
 def parse():
@@ -46,6 +55,7 @@ def main():
         sys.exit(2)
 
     import torch
+    from owl_amd import _lib
     from owl_amd import ndarray as nd
     from owl_amd.shapes import SAME, VALID
 
@@ -63,13 +73,22 @@ def main():
         dw = nd.conv2d_backward_kernel(x, w, [1, 1], dy, out_dtype=out_w)
         return y, dx, dw
 
-    def timed(x, w, dy, out_w, steps, warmup):
+    def timed(x, w, dy, out_w, steps, warmup, kernel_times=False):
+        """Wall-clock K steps (the measured value), then -- outside that
+        region -- K more steps with per-pass HIP events, and (BF16) K more
+        with the library's HIP events around every kernel it launches.
+        Event records add a release on the stream between kernels, so they
+        are kept out of the region the throughput is taken from."""
         for _ in range(warmup):
             step(x, w, dy, out_w)
-        # per-pass HIP events on the stream the library launches on
-        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        for _ in range(steps):
+            step(x, w, dy, out_w)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        # per-pass events on the stream the library launches on
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
         for i in range(steps):
             e = ev[i]
             e[0].record()
@@ -80,12 +99,23 @@ def main():
             nd.conv2d_backward_kernel(x, w, [1, 1], dy, out_dtype=out_w)
             e[3].record()
         torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
         per = [statistics.mean(ev[i][j].elapsed_time(ev[i][j + 1]) for i in range(steps)) for j in range(3)]
-        return wall, per
+        kt = None
+        if kernel_times:
+            _lib.kernel_timing(True)
+            for _ in range(steps):
+                step(x, w, dy, out_w)
+            torch.cuda.synchronize()
+            kt = {}
+            for cls, tag in ((_lib.KT_FWD, "fwd"), (_lib.KT_BWD_INPUT, "bwd_input"),
+                             (_lib.KT_BWD_KERNEL, "bwd_kernel"), (_lib.KT_AUX, "aux")):
+                ms, n, name = _lib.kernel_time(cls)
+                kt[tag] = {"kernel": name, "launches": n, "avg_ms": ms / max(n, 1)}
+            _lib.kernel_timing(False)
+        return wall, per, kt
 
-    wall_b, per_b = timed(xb, wb, dyb, torch.float32, args.steps, args.warmup)
-    wall_f, per_f = timed(x32, w32, dy32, torch.float32, args.steps, args.warmup)
+    wall_b, per_b, kt_b = timed(xb, wb, dyb, torch.float32, args.steps, args.warmup, kernel_times=True)
+    wall_f, per_f, _ = timed(x32, w32, dy32, torch.float32, args.steps, args.warmup)
     tf_b = 3 * FLOP_PASS * args.steps / wall_b / 1e12
     tf_f = 3 * FLOP_PASS * args.steps / wall_f / 1e12
 
@@ -117,45 +147,51 @@ def main():
             "fwd_frac_hbm": round((ny + no) / t_pf / HBM_PEAK, 3),
             "bwd_frac_hbm": round((ny + no + ny) / t_pb / HBM_PEAK, 3)}
 
-    # errors against the FP64 oracle on a 2-image slice (conv is per-image for
-    # fwd / bwd_input; bwd_kernel is re-run on the slice)
+    # errors against an FP64 reference on a 2-image slice: torch's FP64
+    # convolution and its gradients on the unrounded FP32 inputs (NCHW views
+    # of the NHWC tensors; SAME for a 3x3 stride-1 kernel pads 1 on each side)
     errs = None
     if not args.no_check:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_ndarray as on
+        import torch.nn.functional as F
         n = 2
-        xs64 = xb[:n].double().cpu().numpy()
-        ws64 = wb.double().cpu().numpy()
-        gs64 = dyb[:n].double().cpu().numpy()
+        x64 = x32[:n].double().permute(0, 3, 1, 2).contiguous()
+        w64 = w32.double().permute(3, 2, 0, 1).contiguous()          # [co, ci, kh, kw]
+        g64 = dy32[:n].double().permute(0, 3, 1, 2).contiguous()
+        ry = F.conv2d(x64, w64, padding=1).permute(0, 2, 3, 1)
+        rdx = torch.nn.grad.conv2d_input(x64.shape, w64, g64, padding=1).permute(0, 2, 3, 1)
+        rdw = torch.nn.grad.conv2d_weight(x64, w64.shape, g64, padding=1).permute(2, 3, 1, 0)
+        rel = lambda a, b: float(torch.linalg.vector_norm(a.double() - b) / torch.linalg.vector_norm(b))
         yb, dxb, _ = step(xb, wb, dyb, torch.float32)
         dwb = nd.conv2d_backward_kernel(xb[:n].contiguous(), wb, [1, 1], dyb[:n].contiguous(), out_dtype=torch.float32)
-        x64u = x32[:n].double().cpu().numpy()
-        w64u = w32.double().cpu().numpy()
-        g64u = dy32[:n].double().cpu().numpy()
-        ry = on.conv2d(x64u, w64u, [1, 1], SAME)
-        rdx = on.conv2d_backward_input(x64u, w64u, [1, 1], g64u)
-        rdw = on.conv2d_backward_kernel(x64u, w64u, [1, 1], g64u)
-        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
         yf, dxf, _ = step(x32, w32, dy32, torch.float32)
         dwf = nd.conv2d_backward_kernel(x32[:n].contiguous(), w32, [1, 1], dy32[:n].contiguous())
         errs = {
-            "bf16_rel_l2_vs_fp64": {"fwd": rel(yb[:n].double().cpu().numpy(), ry),
-                                    "bwd_input": rel(dxb[:n].double().cpu().numpy(), rdx),
-                                    "bwd_kernel": rel(dwb.double().cpu().numpy(), rdw)},
-            "fp32_rel_l2_vs_fp64": {"fwd": rel(yf[:n].double().cpu().numpy(), ry),
-                                    "bwd_input": rel(dxf[:n].double().cpu().numpy(), rdx),
-                                    "bwd_kernel": rel(dwf.double().cpu().numpy(), rdw)},
+            "bf16_rel_l2_vs_fp64": {"fwd": rel(yb[:n], ry), "bwd_input": rel(dxb[:n], rdx),
+                                    "bwd_kernel": rel(dwb, rdw)},
+            "fp32_rel_l2_vs_fp64": {"fwd": rel(yf[:n], ry), "bwd_input": rel(dxf[:n], rdx),
+                                    "bwd_kernel": rel(dwf, rdw)},
+            "reference": "torch FP64 conv2d / conv2d_input / conv2d_weight on the FP32 inputs",
             "slice": f"first {n} images (bwd_kernel recomputed on the slice)",
         }
-        del xs64, ws64, gs64
+        del x64, w64, g64, ry, rdx, rdw
 
-    # roofline of the dominant BF16 kernel
+    # roofline of the dominant BF16 kernel: the main kernel with the longest
+    # average launch, timed by the library's HIP events on its own stream
+    # over the timed region
     names = ["fwd", "bwd_input", "bwd_kernel"]
-    j = int(np.argmax(per_b))
-    roof = {"kernel": f"conv_igemm_kernel<{j},bf16> ({names[j]})", "bound": "mfma",
-            "achieved": round(FLOP_PASS / (per_b[j] * 1e-3) / 1e12, 2), "peak": PEAK["bf16"] / 1e12,
-            "unit": "TFLOP/s", "traffic": None}
+    dom = max(names, key=lambda k: kt_b[k]["avg_ms"])
+    kname, kavg = kt_b[dom]["kernel"], kt_b[dom]["avg_ms"]
+    traffic = TRAFFIC.get((kname, dom))
+    traffic = None if traffic is None else int(traffic)
+    roof = {"kernel": f"{kname} ({dom})", "bound": "mfma",
+            "achieved": round(FLOP_PASS / (kavg * 1e-3) / 1e12, 2), "peak": PEAK["bf16"] / 1e12,
+            "unit": "TFLOP/s", "traffic": traffic, "avg_launch_ms": round(kavg, 4),
+            "launches": kt_b[dom]["launches"]}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+    if traffic is not None:
+        roof["traffic_unit"] = "bytes per launch (HBM, rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"
+    kernels = {k: {"kernel": v["kernel"], "avg_ms": round(v["avg_ms"], 4), "launches": v["launches"]}
+               for k, v in kt_b.items()}
 
     cpu = None
     if not args.no_cpu:
@@ -185,22 +221,23 @@ def main():
                         "fp32": dict(zip(names, [round(v, 4) for v in per_f]))},
         "errors": errs,
         "pool_max2x2s2": pool,
+        "kernels_bf16": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
 
-def cpu_baseline():
-    """Owl's CPU algorithm (im2col + OpenBLAS SGEMM, restated in oracle/) on a
-    16-image slice of the same layer, all host threads, median of 5."""
+def cpu_baseline(budget_s=12.0):
+    """Owl's CPU algorithm (im2col + OpenBLAS SGEMM, restated in oracle/) on
+    the whole layer (all 64 images), all host threads: full steps repeated
+    until about budget_s seconds of CPU work, median step time."""
     import oracle
     rng = np.random.default_rng(0)
-    n = 16
-    x = rng.standard_normal([n, H, W, CI]).astype(np.float32)
+    x = rng.standard_normal([B, H, W, CI]).astype(np.float32)
     w = (rng.standard_normal([K, K, CI, CO]) * (2.0 / (K * K * CI)) ** 0.5).astype(np.float32)
-    dy = rng.standard_normal([n, H, W, CO]).astype(np.float32)
-    g = oracle.conv_geom([n, H, W, CI], [K, K, CI, CO], [H, W], [1, 1], [1, 1], [1, 1])
+    dy = rng.standard_normal([B, H, W, CO]).astype(np.float32)
+    g = oracle.conv_geom([B, H, W, CI], [K, K, CI, CO], [H, W], [1, 1], [1, 1], [1, 1])
     lib, _, path = oracle.openblas_sgemm()
     threads = None
     for sym in ("scipy_openblas_get_num_threads", "openblas_get_num_threads"):
@@ -208,22 +245,25 @@ def cpu_baseline():
         if f is not None:
             threads = int(f())
             break
+
     def run():
         oracle.conv_fwd_owl_f32(g, x, w)
         oracle.conv_bwd_input_owl_f32(g, w, dy)
         oracle.conv_bwd_kernel_owl_f32(g, x, dy)
     run()
     ts = []
-    for _ in range(5):
+    t_start = time.perf_counter()
+    while time.perf_counter() - t_start < budget_s or len(ts) < 3:
         t0 = time.perf_counter()
         run()
         ts.append(time.perf_counter() - t0)
     t = statistics.median(ts)
-    flop = 3 * FLOP_PASS * n / B
+    flop = 3 * FLOP_PASS
     return {"value": round(flop / t / 1e12, 4), "unit": "TFLOP/s", "cores": threads or os.cpu_count(),
-            "kind": "port", "sample": f"{n}-image slice of the layer, fwd+bwd_input+bwd_kernel, FP32 "
-                                      f"im2col + OpenBLAS sgemm ({os.path.basename(path)}), median of 5",
-            "seconds_per_sample": round(t, 3)}
+            "kind": "port", "sample": f"the whole layer (64 images), fwd+bwd_input+bwd_kernel, FP32 im2col + "
+                                      f"OpenBLAS sgemm ({os.path.basename(path)}), {len(ts)} steps in "
+                                      f"{sum(ts):.1f} s, median step",
+            "seconds_per_step": round(t, 3)}
 
 
 if __name__ == "__main__":

@@ -72,6 +72,24 @@ const char *owl_amd_last_error(void);
 int owl_amd_cache_stats(int64_t *n_buffers, int64_t *bytes);
 void owl_amd_cache_release(void);
 
+/* Kernel timing (diagnostics; no counterpart in the reference).  While
+ * enabled, each convolution call brackets its main compute kernel with a
+ * pair of HIP events on the stream it launches on; the auxiliary kernels
+ * (weight packing, split-K reduction) are timed as their own class.
+ * owl_amd_kernel_timing(1) clears the totals and starts recording, (0)
+ * stops.  owl_amd_kernel_time waits for the recorded events and returns the
+ * total milliseconds and launch count of one class, and the name of the
+ * kernel that class last launched. */
+enum {
+  OWL_AMD_KT_FWD = 0,        /* main kernel of owl_amd_conv_fwd */
+  OWL_AMD_KT_BWD_INPUT = 1,  /* main kernel of owl_amd_conv_bwd_input */
+  OWL_AMD_KT_BWD_KERNEL = 2, /* main kernel of owl_amd_conv_bwd_kernel */
+  OWL_AMD_KT_AUX = 3,        /* weight packing and split-K reduction */
+  OWL_AMD_KT_COUNT = 4
+};
+int owl_amd_kernel_timing(int enable);
+int owl_amd_kernel_time(int kclass, double *total_ms, int64_t *launches, const char **name);
+
 /* ---------------------------------------------------------------- device API */
 /* Forward convolution y = x (*) w.  Replaces the arithmetic of
  * owl_ndarray_conv_impl.h:246-453 (spatial), :1211-1319 (cuboid_im2col),

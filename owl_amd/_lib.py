@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libowl_amd.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM = 0, 1, 2, 3, 4
 F32, F64, BF16, F16, FP8E4M3 = 0, 1, 2, 3, 4
+KT_FWD, KT_BWD_INPUT, KT_BWD_KERNEL, KT_AUX = 0, 1, 2, 3
 
 
 class OwlAmdError(RuntimeError):
@@ -89,6 +90,9 @@ def lib():
             "owl_amd_cache_stats": ([ctypes.POINTER(i64), ctypes.POINTER(i64)], i32),
             "owl_amd_cache_release": ([], None),
             "owl_amd_complex_supported": ([], i32),
+            "owl_amd_kernel_timing": ([i32], i32),
+            "owl_amd_kernel_time": ([i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64),
+                                     ctypes.POINTER(ctypes.c_char_p)], i32),
             "owl_amd_conv_fwd": ([cd, i32, i32, vp, vp, vp, vp], i32),
             "owl_amd_conv_bwd_input": ([cd, i32, i32, vp, vp, vp, vp], i32),
             "owl_amd_conv_bwd_kernel_workspace": ([cd, i32], ctypes.c_size_t),
@@ -129,3 +133,16 @@ def check(rc):
 
 def call(name, *args):
     return check(getattr(lib(), name)(*args))
+
+
+def kernel_timing(enable):
+    """Start (clearing the totals) or stop recording kernel times."""
+    check(lib().owl_amd_kernel_timing(1 if enable else 0))
+
+
+def kernel_time(kclass):
+    """(total ms, launches, kernel name) recorded for one kernel class
+    (KT_FWD, KT_BWD_INPUT, KT_BWD_KERNEL, KT_AUX); waits for the events."""
+    ms, n, name = ctypes.c_double(), ctypes.c_int64(), ctypes.c_char_p()
+    check(lib().owl_amd_kernel_time(kclass, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(name)))
+    return ms.value, n.value, (name.value or b"").decode()

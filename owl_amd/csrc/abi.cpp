@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -33,6 +34,125 @@ extern "C" const char *owl_amd_last_error(void) { return g_err; }
     owl_amd_set_error(__VA_ARGS__);  \
     return OWL_AMD_ERR_ARG;          \
   } while (0)
+
+// ------------------------------------------------------------------ kernel timing
+namespace owl_amd {
+namespace {
+struct KRec {
+  int kclass;
+  const char *name;
+  hipEvent_t a, b;
+};
+std::mutex g_kt_mu;
+std::atomic<bool> g_kt_on{false};
+std::vector<KRec *> g_kt_pending;
+double g_kt_ms[OWL_AMD_KT_COUNT];
+int64_t g_kt_n[OWL_AMD_KT_COUNT];
+const char *g_kt_name[OWL_AMD_KT_COUNT];
+
+// Events are pooled (created when timing is enabled, reused after each
+// read) so that recording adds no event creation to the launch path.
+std::vector<hipEvent_t> g_kt_pool;
+
+hipEvent_t kt_event() {
+  hipEvent_t e = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_kt_mu);
+    if (!g_kt_pool.empty()) {
+      e = g_kt_pool.back();
+      g_kt_pool.pop_back();
+      return e;
+    }
+  }
+  if (hipEventCreate(&e) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return e;
+}
+
+// caller holds g_kt_mu
+void kt_free(KRec *r) {
+  if (r->a) g_kt_pool.push_back(r->a);
+  if (r->b) g_kt_pool.push_back(r->b);
+  delete r;
+}
+}  // namespace
+
+void ktime_begin(int kclass, const char *name, hipStream_t st, void **token) {
+  *token = nullptr;
+  if (!g_kt_on.load(std::memory_order_relaxed)) return;
+  KRec *r = new KRec{kclass, name, kt_event(), kt_event()};
+  if (!r->a || !r->b || hipEventRecord(r->a, st) != hipSuccess) {
+    (void)hipGetLastError();
+    std::lock_guard<std::mutex> lk(g_kt_mu);
+    kt_free(r);
+    return;
+  }
+  *token = r;
+}
+
+void ktime_end(void *token, hipStream_t st) {
+  KRec *r = (KRec *)token;
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  if (hipEventRecord(r->b, st) != hipSuccess) {
+    (void)hipGetLastError();
+    kt_free(r);
+    return;
+  }
+  g_kt_pending.push_back(r);
+}
+}  // namespace owl_amd
+
+extern "C" int owl_amd_kernel_timing(int enable) {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  g_kt_on.store(enable != 0);
+  if (enable) {
+    for (KRec *r : g_kt_pending) kt_free(r);
+    g_kt_pending.clear();
+    while (g_kt_pool.size() < 512) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        break;
+      }
+      g_kt_pool.push_back(e);
+    }
+    for (int i = 0; i < OWL_AMD_KT_COUNT; ++i) {
+      g_kt_ms[i] = 0.0;
+      g_kt_n[i] = 0;
+      g_kt_name[i] = nullptr;
+    }
+  }
+  return OWL_AMD_OK;
+}
+
+extern "C" int owl_amd_kernel_time(int kclass, double *total_ms, int64_t *launches, const char **name) {
+  if (kclass < 0 || kclass >= OWL_AMD_KT_COUNT) ARG_FAIL("kernel class %d out of range", kclass);
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  int rc = OWL_AMD_OK;
+  for (KRec *r : g_kt_pending) {
+    float ms = 0.0f;
+    if (rc == OWL_AMD_OK) {
+      hipError_t e = hipEventSynchronize(r->b);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, r->a, r->b);
+      if (e != hipSuccess) {
+        owl_amd_set_error("kernel timing: %s", hipGetErrorString(e));
+        rc = OWL_AMD_ERR_HIP;
+      } else {
+        g_kt_ms[r->kclass] += ms;
+        g_kt_n[r->kclass] += 1;
+        g_kt_name[r->kclass] = r->name;
+      }
+    }
+    kt_free(r);
+  }
+  g_kt_pending.clear();
+  if (total_ms) *total_ms = g_kt_ms[kclass];
+  if (launches) *launches = g_kt_n[kclass];
+  if (name) *name = g_kt_name[kclass] ? g_kt_name[kclass] : "";
+  return rc;
+}
 
 // ------------------------------------------------------------------ per-device state
 namespace {
@@ -148,6 +268,7 @@ bool aligned16(const void *a, const void *b, const void *c) {
 
 template <int PASS>
 int dispatch_igemm(const ConvParams &p, int in_t, int out_t, bool aligned, hipStream_t st) {
+  KTimer kt(PASS, "conv_igemm_kernel", st);
   if (in_t == OWL_AMD_F32 && out_t == OWL_AMD_F32) run_igemm<PASS, float, float>(p, aligned, st);
   else if (in_t == OWL_AMD_F64 && out_t == OWL_AMD_F64) run_igemm<PASS, double, double>(p, aligned, st);
   else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) run_igemm<PASS, bf16_t, bf16_t>(p, aligned, st);
@@ -252,6 +373,7 @@ bool try_fast(const ConvParams &p, int in_t, int out_t, bool aligned, hipStream_
     if (done) return true;
   }
   const int shape = fast_shape_for(p.N);
+  KTimer kt(PASS, "conv_fast_kernel", st);
   if (in_t == OWL_AMD_F32 && out_t == OWL_AMD_F32) run_fast<PASS, float, float>(p, shape, fast_pd(), st);
   else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) run_fast<PASS, bf16_t, bf16_t>(p, shape, fast_pd(), st);
   else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_F32) run_fast<PASS, bf16_t, float>(p, shape, fast_pd(), st);
@@ -426,6 +548,7 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   if (p.split <= 1) {
     p.c = dw;
     if (tr) {
+      KTimer kt(OWL_AMD_KT_BWD_KERNEL, "conv_wgrad_kernel", st);
       if (in_t == OWL_AMD_BF16) {
         if (out_t == OWL_AMD_BF16) run_wgrad<bf16_t, bf16_t>(p, st); else run_wgrad<bf16_t, float>(p, st);
       } else {
@@ -451,6 +574,7 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   }
   p.c = ws;
   if (tr) {
+    KTimer kt(OWL_AMD_KT_BWD_KERNEL, "conv_wgrad_kernel", st);
     if (in_t == OWL_AMD_BF16) run_wgrad<bf16_t, float>(p, st);
     else run_wgrad<f16_t, float>(p, st);
     OWL_CHECK_HIP(hipGetLastError());
@@ -459,6 +583,7 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
     if (rc) return rc;
   }
   const int64_t n = p.M * p.N;
+  KTimer kt(OWL_AMD_KT_AUX, "splitk_reduce_kernel", st);
   if (in_t == OWL_AMD_F64) run_splitk_reduce<double, double>((const double *)ws, (double *)dw, n, p.split, st);
   else if (out_t == OWL_AMD_F32) run_splitk_reduce<float, float>((const float *)ws, (float *)dw, n, p.split, st);
   else if (out_t == OWL_AMD_BF16) run_splitk_reduce<float, bf16_t>((const float *)ws, (bf16_t *)dw, n, p.split, st);
@@ -506,6 +631,7 @@ extern "C" int owl_amd_conv_fwd_fp8(const owl_amd_conv_desc *d, int out_t, const
   p.scale_b = w_scale;
   bool aligned = p.Ci % 16 == 0 && p.Co % 16 == 0 && aligned16(xq, wq, nullptr);
   hipStream_t st = (hipStream_t)stream;
+  KTimer kt(OWL_AMD_KT_FWD, "conv_igemm_kernel", st);
   if (out_t == OWL_AMD_F32) run_igemm<0, fp8e4m3_t, float>(p, aligned, st);
   else if (out_t == OWL_AMD_BF16) run_igemm<0, fp8e4m3_t, bf16_t>(p, aligned, st);
   else {

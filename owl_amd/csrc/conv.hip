@@ -583,11 +583,13 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
     }
 }
 
+// 1-D grid of tiles x splits, split-major in launch order.
 template <int PASS, typename TI, typename TO, bool ALIGNED>
-__global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p) {
+__global__ __launch_bounds__(NT) void conv_igemm_kernel(ConvParams p, int tiles) {
   constexpr int BMX = Mfma<TI>::BM;
   __shared__ __attribute__((aligned(16))) char lds[igemm_lds_bytes<TI, BMX>()];
-  igemm_tile<PASS, TI, TO, ALIGNED, BMX>(p, blockIdx.x, blockIdx.y, lds);
+  const int lid = blockIdx.x;
+  igemm_tile<PASS, TI, TO, ALIGNED, BMX>(p, lid % tiles, lid / tiles, lds);
 }
 
 // Deterministic split-K reduction: dw[e] = sum over splits in split order,
@@ -694,11 +696,12 @@ template <int PASS, typename TI, typename TO>
 static void launch_igemm(const ConvParams &p, bool aligned, hipStream_t st) {
   typedef Tile<TI> TL;
   int64_t tiles = ((p.M + TL::BM - 1) / TL::BM) * (int64_t)((p.N + TL::BN - 1) / TL::BN);
-  dim3 grid((unsigned)tiles, PASS == 2 ? (unsigned)p.split : 1u);
+  const int64_t blocks = tiles * (PASS == 2 ? p.split : 1);
+  dim3 grid((unsigned)blocks);
   if (aligned)
-    hipLaunchKernelGGL((conv_igemm_kernel<PASS, TI, TO, true>), grid, dim3(NT), 0, st, p);
+    hipLaunchKernelGGL((conv_igemm_kernel<PASS, TI, TO, true>), grid, dim3(NT), 0, st, p, (int)tiles);
   else
-    hipLaunchKernelGGL((conv_igemm_kernel<PASS, TI, TO, false>), grid, dim3(NT), 0, st, p);
+    hipLaunchKernelGGL((conv_igemm_kernel<PASS, TI, TO, false>), grid, dim3(NT), 0, st, p, (int)tiles);
 }
 
 template <typename TI> int tile_bm() { return Tile<TI>::BM; }

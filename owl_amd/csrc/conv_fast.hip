@@ -46,12 +46,6 @@ template <> struct FastMfma<float> {
   static constexpr int BK = 16, KPL = 1;
 };
 
-__device__ __forceinline__ int xcd_swizzle(int bid, int nblocks) {
-  // Hardware dispatches workgroup i to XCD i % 8.  Give XCD x the contiguous
-  // range of tiles [x * nblocks / 8, (x + 1) * nblocks / 8).
-  if (nblocks % 8 != 0) return bid;
-  return (bid % 8) * (nblocks / 8) + bid / 8;
-}
 
 // Raw buffer resource over [base, base + bytes): loads at offsets >= bytes
 // return zeros, which is how padding taps read their zeros.
@@ -408,9 +402,15 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   const int wm = wave >> 1, wn = wave & 1;
   const int M = (int)p.M, N = p.N;
   const int n_tiles = (N + BN - 1) / BN;
-  const int tile = blockIdx.x;
+  const int mn_tiles = ((M + BM - 1) / BM) * n_tiles;
+  // 1-D grid of mn_tiles x split in launch order: the tiles of one split
+  // (which read the same pixels) spread over the XCDs.  Grouping them on one
+  // XCD cut the L2 fetch 5x on a 64x56x56x64 -> 128 layer but ran slower
+  // (the Infinity Cache already serves the re-reads).
+  const int lid = blockIdx.x;
+  const int tile = lid % mn_tiles, split_idx = lid / mn_tiles;
   const int m0 = (tile / n_tiles) * BM, n0 = (tile % n_tiles) * BN;
-  const int64_t kbeg = (int64_t)blockIdx.y * p.k_per_split;
+  const int64_t kbeg = (int64_t)split_idx * p.k_per_split;
   int64_t kend = kbeg + p.k_per_split;
   if (kend > p.K) kend = p.K;
   const int nk = (int)((kend - kbeg + BK - 1) / BK);
@@ -547,7 +547,7 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   const int frow = lane & 31, fkg = lane >> 5;
   const bool partial = p.split > 1;
   const uint32_t c_bytes = partial ? (uint32_t)((int64_t)M * N * 4) : (uint32_t)((int64_t)M * N * sizeof(TO));
-  char *cbase = (char *)p.c + (partial ? (int64_t)blockIdx.y * M * N * 4 : 0);
+  char *cbase = (char *)p.c + (partial ? (int64_t)split_idx * M * N * 4 : 0);
   const __amdgpu_buffer_rsrc_t cr = make_rsrc(cbase, c_bytes);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -579,7 +579,7 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
 
 template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st) {
-  dim3 grid((unsigned)(((p.M + 127) / 128) * ((p.N + 127) / 128)), (unsigned)p.split);
+  dim3 grid((unsigned)(((p.M + 127) / 128) * ((p.N + 127) / 128) * p.split));
   hipLaunchKernelGGL((conv_wgrad_kernel<TI, TO>), grid, dim3(256), 0, st, p);
 }
 template void run_wgrad<bf16_t, bf16_t>(const ConvParams &, hipStream_t);
@@ -938,11 +938,13 @@ bool run_halo(const ConvParams &p, void *wpack, hipStream_t st) {
   if (PASS == 0 && wpack) {
     const int T = p.kh * p.kw;
     dim3 pg((unsigned)((p.Co + 31) / 32), (unsigned)((p.Ci + 31) / 32), (unsigned)T);
+    KTimer kp(OWL_AMD_KT_AUX, "pack_weights_kernel", st);
     hipLaunchKernelGGL((pack_weights_kernel<TI>), pg, dim3(256), 0, st, w, (TI *)wpack, T, p.Ci, p.Co);
     w = (const TI *)wpack;
     krows = true;
   }
   dim3 grid((unsigned)blocks), blk(256);
+  KTimer kt(PASS, "conv_halo_kernel", st);
   if (FN == 2) {
     if (krows) hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2, true>), grid, blk, 0, st, g, in, w, (TO *)p.c);
     else hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2, false>), grid, blk, 0, st, g, in, w, (TO *)p.c);

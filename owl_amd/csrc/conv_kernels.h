@@ -62,4 +62,19 @@ void run_wgrad(const ConvParams &p, hipStream_t st);
 template <typename TI> int tile_bm();
 template <typename TI> int tile_bk();
 
+// Kernel timing (owl_amd_kernel_timing): a KTimer brackets one launch with
+// HIP events on its stream while timing is enabled, and does nothing else.
+void ktime_begin(int kclass, const char *name, hipStream_t st, void **token);
+void ktime_end(void *token, hipStream_t st);
+struct KTimer {
+  void *tok = nullptr;
+  hipStream_t st;
+  KTimer(int kclass, const char *name, hipStream_t s) : st(s) { ktime_begin(kclass, name, s, &tok); }
+  ~KTimer() {
+    if (tok) ktime_end(tok, st);
+  }
+  KTimer(const KTimer &) = delete;
+  KTimer &operator=(const KTimer &) = delete;
+};
+
 }  // namespace owl_amd

@@ -33,3 +33,15 @@ extern "C" void owl_amd_set_error(const char *fmt, ...);
 template <typename T> __device__ __forceinline__ float to_f32(T v) { return (float)v; }
 template <typename T> __device__ __forceinline__ T from_f32(float v) { return (T)v; }
 template <> __device__ __forceinline__ double from_f32<double>(float v) { return (double)v; }
+
+namespace owl_amd {
+// Hardware dispatches workgroup i of a 1-D grid to XCD i % 8, and each XCD
+// has its own L2.  Map workgroup i to a logical tile so that XCD x runs a
+// contiguous range of logical tiles (the first nblocks % 8 XCDs one more
+// than the others): tiles that share operands then share an L2.  A bijection
+// on [0, nblocks) for every nblocks.
+__device__ __forceinline__ int xcd_swizzle(int bid, int nblocks) {
+  const int q = nblocks >> 3, r = nblocks & 7, x = bid & 7;
+  return x * q + (x < r ? x : r) + (bid >> 3);
+}
+}  // namespace owl_amd
