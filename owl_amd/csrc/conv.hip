@@ -292,7 +292,7 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
   TI ra[ELEMS], rb[ELEMS];
 
   // ---------------- global -> registers ----------------
-  auto load_a = [&](int64_t k0) {
+  auto load_a = [&](int64_t k0) __attribute__((always_inline)) {
     if (PASS == 0 || PASS == 1) {
       const int C = PASS == 0 ? p.Ci : p.Co;
       if (ALIGNED) {
@@ -404,7 +404,7 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
     }
   };
 
-  auto load_b = [&](int64_t k0) {
+  auto load_b = [&](int64_t k0) __attribute__((always_inline)) {
     if (PASS == 1) {
       // B[k = (tap, co)][n = ci] = w[tap][n][co]: K-contiguous.
       constexpr int BR = ELEMS / VEC;
@@ -469,7 +469,7 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
   };
 
   // ---------------- registers -> LDS ----------------
-  auto store_tiles = [&](int buf) {
+  auto store_tiles = [&](int buf) __attribute__((always_inline)) {
     TI *As = lds[buf];
     TI *Bs = lds[buf] + BM * LDR;
     if (PASS == 0 || PASS == 1) {
@@ -511,7 +511,7 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
   const int frow = lane % MF::TM;
   const int fkg = lane / MF::TM;
 
-  auto compute = [&](int buf) {
+  auto compute = [&](int buf) __attribute__((always_inline)) {
     const TI *As = lds[buf];
     const TI *Bs = lds[buf] + BM * LDR;
     TI fa[TL::FM][TL::LK], fb[TL::FN][TL::LK];

@@ -54,6 +54,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, ui
 }
 constexpr uint32_t OOB = 0x80000000u;  // >= every num_records used here (tensors < 2 GiB)
 
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint4 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
   return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0));
 }
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
   int la = 0, lc = 0, le = 0, lc0 = 0;
   uint32_t voff[FM];  // byte offset of the current A tap for each row, or OOB
 
-  auto set_tap = [&]() {
+  auto set_tap = [&]() __attribute__((always_inline)) {
     const int dh = la * p.rh, dw = lc * p.rw, dd = le * p.rd;
 #pragma unroll
     for (int f = 0; f < FM; ++f) {
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
     }
   };
 
-  auto load_a = [&](int buf) {
+  auto load_a = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
     for (int f = 0; f < FM; ++f) {
       const uint32_t o = voff[f] + (uint32_t)(lc0 * ES);
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
     }
   };
 
-  auto advance_a = [&]() {
+  auto advance_a = [&]() __attribute__((always_inline)) {
     lc0 += BK;
     if (lc0 == C) {
       lc0 = 0;
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
     }
   };
 
-  auto load_b = [&](int slot, int tap_, int c0_) {
+  auto load_b = [&](int slot, int tap_, int c0_) __attribute__((always_inline)) {
     TI *rbs = rb[slot];
     if (PASS == 0) {
       const int n = n0 + f_ng * 4;
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
     }
   };
 
-  auto store_b = [&](int buf, int slot) {
+  auto store_b = [&](int buf, int slot) __attribute__((always_inline)) {
     TI *dst = Bs[buf];
     const TI *rbs = rb[slot];
     if (PASS == 0) {
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
     }
   };
 
-  auto advance_b = [&]() {
+  auto advance_b = [&]() __attribute__((always_inline)) {
     c0 += BK;
     if (c0 == C) {
       c0 = 0;
@@ -461,7 +462,7 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
     pi[h] = t % p.Ho; pb[h] = t / p.Ho;
   }
   uint4 ra[2], rb[2];
-  auto load = [&]() {
+  auto load = [&]() __attribute__((always_inline)) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       uint32_t vo = OOB, go = OOB;
@@ -488,7 +489,7 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
       pb[h] += sb + c;
     }
   };
-  auto store = [&](int st) {
+  auto store = [&](int st) __attribute__((always_inline)) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int row = kpix0 + 16 * h;
@@ -590,8 +591,10 @@ template void run_wgrad<f16_t, float>(const ConvParams &, hipStream_t);
 // ---------------------------------------------------------------------------
 // Stride-1 2-D convolution with an LDS-staged input halo.
 //
-// A workgroup owns TR full output rows of one image (TR * Wo <= 256 pixels,
-// 8 fragments of 32) and a BN-wide slice of output channels.  For each block
+// A workgroup owns TR full output rows of one image and a BN-wide slice of
+// output channels: 4 waves of FM x FN 32x32 fragments, so TR * Wo <= 512
+// pixels for 64-wide slices (FM 4) and <= 256 for 128-wide ones (FM 2).
+// For each block
 // of BK input channels it stages the input rows those output rows read --
 // (TR - 1) + (kh - 1) rh + 1 rows of Wo + (kw - 1) rw columns, zero-filled
 // outside the image -- into LDS once, and every tap then reads its A
@@ -615,19 +618,24 @@ struct HaloGeom {
 
 // KROWS: the kernel operand is laid out [tap][n][k] (k contiguous): the
 // backward-input view of w, or forward weights packed by pack_weights_kernel.
-template <int PASS, typename TI, typename TO, int FN, bool KROWS>
+template <int PASS, typename TI, typename TO, int FM, int FN, bool KROWS>
 __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI *__restrict__ in,
                                                            const TI *__restrict__ wt, TO *__restrict__ out) {
   typedef FastMfma<TI> MF;
-  constexpr int BK = 32, FM = 2, BN = FN * 32, PR = BK + 8;  // halo pixel row: 32 channels + 16 B pad
+  constexpr int BK = 32, BN = FN * 32, PR = BK + 8;  // halo pixel row: 32 channels + 16 B pad
   constexpr int LDR = BK + 8;
   // Two halo buffers of HBUF pixels: while one channel block's taps run,
   // the next block's halo is loaded into registers (HQ 16-byte chunks per
   // thread) and written to the other buffer.  A halo above HBUF pixels uses
   // both buffers as one and is restaged between channel blocks.
-  constexpr int HQ = 6, HBUF = 352;
-  __shared__ __attribute__((aligned(16))) TI halo[2 * HBUF * PR];
-  __shared__ __attribute__((aligned(16))) TI Bs[2][BN * LDR];
+  constexpr int HQ = FM == 4 ? 10 : 6, HBUF = 352;
+  // one LDS block: the two halo buffers, then the two kernel tiles; after
+  // the K loop it holds the output tile for the 16-byte store epilogue
+  constexpr int HALO_BYTES = 2 * HBUF * PR * (int)sizeof(TI);
+  constexpr int LDS_BYTES = HALO_BYTES + 2 * BN * LDR * (int)sizeof(TI);
+  __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES];
+  TI *halo = (TI *)smem;
+  TI(*Bs)[BN * LDR] = (TI(*)[BN * LDR])(smem + HALO_BYTES);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int frow = lane & 31, fkg = lane >> 5;
@@ -660,13 +668,14 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   const __amdgpu_buffer_rsrc_t wr = make_rsrc(wt, (uint32_t)(T * g.N * g.C * 2));
   const int cblocks = g.C / BK;
 
-  // halo staging: chunk q -> pixel q / 4, 16-byte channel chunk q % 4.
-  // Per-thread source offsets (channel 0 of the block) are fixed for the
-  // tile; OOB marks padding, which the buffer loads fill with zeros.
+  // halo staging: chunk q -> pixel q / 4, 16-byte channel chunk q % 4;
+  // OOB marks padding, which the buffer loads fill with zeros.
   const int hq_total = HR * HC * 4;
   const bool dbl = HR * HC <= HBUF;
-  const int nbatch = (hq_total + HQ * 256 - 1) / (HQ * 256);
-  auto chunk_src = [&](int q) -> uint32_t {
+  // a single-buffer halo can still be prefetched into registers when it fits
+  // HQ chunks per thread; it is then written after the block's last barrier
+  const bool pf = hq_total <= HQ * 256;
+  auto chunk_src = [&](int q) __attribute__((always_inline)) -> uint32_t {
     const int pix = q >> 2, ch = q & 3;
     const int r = pix / HC, c = pix - (pix / HC) * HC;
     const int hi = hr0 + r, wi = hc0 + c;
@@ -674,33 +683,45 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       return (uint32_t)(((b * g.Hi + hi) * g.Wi + wi) * g.C + ch * 8) * 2u;
     return OOB;
   };
-  uint32_t hsrc[HQ];
-#pragma unroll
-  for (int u = 0; u < HQ; ++u) hsrc[u] = chunk_src(tid + u * 256);
+  // This thread's chunks q = tid + 256 u sit 64 pixels apart: the pixel's
+  // (row, column) in the halo steps by 64 in radix HC, one carry per step.
+  const int hs_r = 64 / HC, hs_c = 64 - (64 / HC) * HC;
+  const int hpix0 = tid >> 2, hch = tid & 3;
+  const int hr_0 = hpix0 / HC, hc_0 = hpix0 - (hpix0 / HC) * HC;
   uint4 hreg[HQ];
-  auto halo_issue = [&](int c0, int batch) {
+  // the first HQ * 256 chunks go through registers (loads issued ahead of
+  // their LDS writes); halos beyond that (rows wider than 64 HQ pixels
+  // allow) copy their remaining chunks load by load
+  auto halo_issue = [&](int c0) __attribute__((always_inline)) {
+    int r = hr_0, c = hc_0;
 #pragma unroll
     for (int u = 0; u < HQ; ++u) {
-      const int q = batch * HQ * 256 + tid + u * 256;
-      const uint32_t so = batch == 0 ? hsrc[u] : chunk_src(q);
-      hreg[u] = bload(inr, so == OOB ? OOB : so + (uint32_t)c0 * 2u);  // q >= hq_total: OOB
+      const int hi = hr0 + r, wi = hc0 + c;
+      const bool ok = tid + u * 256 < hq_total && (unsigned)hi < (unsigned)g.Hi && (unsigned)wi < (unsigned)g.Wi;
+      const uint32_t so = (uint32_t)(((b * g.Hi + hi) * g.Wi + wi) * g.C + c0 + hch * 8) * 2u;
+      hreg[u] = bload(inr, ok ? so : OOB);
+      c += hs_c;
+      const int cy = c >= HC;
+      if (cy) c -= HC;
+      r += hs_r + cy;
     }
   };
-  auto halo_commit = [&](TI *hb, int batch) {
+  auto halo_commit = [&](TI *hb) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < HQ; ++u) {
-      const int q = batch * HQ * 256 + tid + u * 256;
+      const int q = tid + u * 256;
       if (q < hq_total) *(uint4 *)&hb[(q >> 2) * PR + (q & 3) * 8] = hreg[u];
     }
   };
-  auto stage_halo = [&](TI *hb, int c0) {  // every batch (at most two), loads first
-    halo_issue(c0, 0);
-    halo_commit(hb, 0);
-    if (nbatch > 1) {
-      halo_issue(c0, 1);
-      halo_commit(hb, 1);
+  auto stage_halo = [&](TI *hb, int c0) __attribute__((always_inline)) {
+    halo_issue(c0);
+    halo_commit(hb);
+    for (int q = HQ * 256 + tid; q < hq_total; q += 256) {
+      const uint32_t so = chunk_src(q);
+      *(uint4 *)&hb[(q >> 2) * PR + (q & 3) * 8] = bload(inr, so == OOB ? OOB : so + (uint32_t)c0 * 2u);
     }
   };
+
 
   // B staging for (tap, c0): PASS 0 reads w[tap][c0 + k][n] (n-contiguous,
   // transposed through registers); PASS 1 reads w[flipped tap][n][c0 + k].
@@ -796,39 +817,60 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       ++ncb;
     }
     const bool more = ks + 1 < nk;
-    if (dbl && tap == 0 && cb + 1 < cblocks) halo_issue((cb + 1) * BK, 0);
+    if (pf && tap == 0 && cb + 1 < cblocks) halo_issue((cb + 1) * BK);
     // A from the halo at this tap's offset, B from LDS
     const TI *hcur = halo + (dbl ? (cb & 1) * HBUF * PR : 0);
     const int toff = (ta * g.rh * HC + tc * g.rw) * PR;
-    uint4 fa[FM][2], fb[FN][2];
-#pragma unroll
-    for (int f = 0; f < FM; ++f) {
-      const uint4 *src = (const uint4 *)&hcur[hoff[f] + toff];
-      fa[f][0] = src[0];
-      fa[f][1] = src[1];
-    }
     const TI *bs = Bs[bcur] + fb_off;
+    if constexpr (FM == 4) {
+      // 128 accumulators leave no room for both K halves' fragments: read
+      // and consume one half at a time
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const uint4 *src = (const uint4 *)&bs[j * 32 * LDR];
-      fb[j][0] = src[0];
-      fb[j][1] = src[1];
+      for (int s2 = 0; s2 < 2; ++s2) {
+        uint4 fa[FM], fb[FN];
+#pragma unroll
+        for (int f = 0; f < FM; ++f) fa[f] = ((const uint4 *)&hcur[hoff[f] + toff])[s2];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) fb[j] = ((const uint4 *)&bs[j * 32 * LDR])[s2];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i], fb[j], acc[i][j]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else {
+      uint4 fa[FM][2], fb[FN][2];
+#pragma unroll
+      for (int f = 0; f < FM; ++f) {
+        const uint4 *src = (const uint4 *)&hcur[hoff[f] + toff];
+        fa[f][0] = src[0];
+        fa[f][1] = src[1];
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const uint4 *src = (const uint4 *)&bs[j * 32 * LDR];
+        fb[j][0] = src[0];
+        fb[j][1] = src[1];
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i][s2], fb[j][s2], acc[i][j]);
     }
-#pragma unroll
-    for (int s2 = 0; s2 < 2; ++s2)
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i][s2], fb[j][s2], acc[i][j]);
     if (more) store_b(rnext, bcur ^ 1);
     next_load(rnext);
     if (more && ncb != cb) {
       if (dbl) {
         // the other buffer was last read before the previous block's final
         // barrier; the barrier below publishes the new halo
-        halo_commit(halo + (ncb & 1) * HBUF * PR, 0);
-      } else {
+        halo_commit(halo + (ncb & 1) * HBUF * PR);
+      } else if (pf) {
         __syncthreads();  // every wave is done with this channel block's halo
+        halo_commit(halo);
+      } else {
+        __syncthreads();
         stage_halo(halo, ncb * BK);
       }
     }
@@ -847,27 +889,60 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   const uint32_t o_bytes = (uint32_t)((int64_t)g.B * g.Ho * g.Wo * g.N * sizeof(TO));
   const __amdgpu_buffer_rsrc_t orr = make_rsrc(out, o_bytes);
   const int obase = (b * g.Ho + i0) * g.Wo;  // first output pixel of the tile
+  // staged output row: BN values, plus 16 B of padding when that still fits
+  constexpr int ORS_PAD = BN * (int)sizeof(TO) + 16;
+  constexpr int ORS = FM * 128 * ORS_PAD <= LDS_BYTES ? ORS_PAD : BN * (int)sizeof(TO);
+  if constexpr (FM * 128 * ORS <= LDS_BYTES) {
+    // Stage the tile in LDS as [pixel][channel] rows (the K loop ended on a
+    // barrier, so the LDS is free), then store whole 16-byte chunks.
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int n = n0 + j * 32 + frow;
-      const int prow = wave * FM * 32 + i * 32 + 4 * fkg;
+      for (int j = 0; j < FN; ++j) {
+        char *rowp = smem + (wave * FM * 32 + i * 32 + 4 * fkg) * ORS + (j * 32 + frow) * (int)sizeof(TO);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int pp = prow + 8 * (r >> 2) + (r & 3);
-        const bool ok = pp < P && n < g.N;
-        const uint32_t vo = ok ? (uint32_t)((obase + pp) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
-        const float v = acc[i][j][r];
-        if constexpr (sizeof(TO) == 2) {
-          const TO h = (TO)v;
-          __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), orr, (int)vo, 0, 0);
-        } else {
-          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), orr, (int)vo, 0, 0);
+        for (int r = 0; r < 16; ++r) {
+          const float v = acc[i][j][r];
+          *(TO *)(rowp + (8 * (r >> 2) + (r & 3)) * ORS) = (TO)v;
         }
       }
+    __syncthreads();
+    constexpr int CPR = BN * (int)sizeof(TO) / 16;  // 16-byte chunks per row
+    constexpr int EPC = 16 / (int)sizeof(TO);       // values per chunk
+    for (int c = tid; c < P * CPR; c += 256) {
+      const int pix = c / CPR, ch = c - (c / CPR) * CPR;
+      const int n = n0 + ch * EPC;
+      const uint4 v = *(const uint4 *)(smem + pix * ORS + ch * 16);
+      const uint32_t vo = n < g.N ? (uint32_t)((obase + pix) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), orr, (int)vo, 0, 0);
     }
+  } else {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int n = n0 + j * 32 + frow;
+        const int prow = wave * FM * 32 + i * 32 + 4 * fkg;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int pp = prow + 8 * (r >> 2) + (r & 3);
+          const bool ok = pp < P && n < g.N;
+          const uint32_t vo = ok ? (uint32_t)((obase + pp) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
+          const float v = acc[i][j][r];
+          if constexpr (sizeof(TO) == 2) {
+            const TO h = (TO)v;
+            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), orr, (int)vo, 0, 0);
+          } else {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), orr, (int)vo, 0, 0);
+          }
+        }
+      }
+  }
 }
+
+// Fragments per wave along the pixels: 4 (128 pixels) for 64-wide channel
+// slices, 2 (64 pixels) for 128-wide ones; either way 128 accumulators.
+static int halo_fm(int N) { return N <= 64 ? 4 : 2; }
 
 // Geometry of the halo kernel for a problem, or false if it does not fit.
 template <int PASS>
@@ -887,7 +962,8 @@ static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom
   g.kh = p.kh; g.kw = p.kw; g.rh = p.rh; g.rw = p.rw;
   g.B = p.B; g.wCi = p.Ci; g.wCo = p.Co;
   if (g.C % 32 != 0 || g.N % 8 != 0 || g.Wo > 256) return false;
-  g.TR = 256 / g.Wo;
+  // 128 pixels per wave (FM = 4) when the channel slice is 64 wide, else 64
+  g.TR = halo_fm(g.N) * 128 / g.Wo;
   if (g.TR > g.Ho) g.TR = g.Ho;
   const int HR = (g.TR - 1) + (g.kh - 1) * g.rh + 1, HC = (g.Wo - 1) + (g.kw - 1) * g.rw + 1;
   if (HR * HC > 2 * 352) return false;  // halo must fit its LDS (two 352-pixel buffers)
@@ -930,7 +1006,7 @@ bool run_halo(const ConvParams &p, void *wpack, hipStream_t st) {
   HaloGeom g;
   if (!halo_geom<PASS>(p, (int)sizeof(TI), (int)sizeof(TO), g)) return false;
   const int FN = g.N <= 64 ? 2 : 4;
-  const int BN = FN * 32;
+  const int BN = FN * 32;  // FM = halo_fm(N): 4 with FN 2, 2 with FN 4
   const int64_t blocks = (int64_t)g.B * ((g.Ho + g.TR - 1) / g.TR) * ((g.N + BN - 1) / BN);
   const TI *in = (const TI *)p.a;
   const TI *w = (const TI *)p.b;
@@ -946,11 +1022,11 @@ bool run_halo(const ConvParams &p, void *wpack, hipStream_t st) {
   dim3 grid((unsigned)blocks), blk(256);
   KTimer kt(PASS, "conv_halo_kernel", st);
   if (FN == 2) {
-    if (krows) hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2, true>), grid, blk, 0, st, g, in, w, (TO *)p.c);
-    else hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2, false>), grid, blk, 0, st, g, in, w, (TO *)p.c);
+    if (krows) hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 4, 2, true>), grid, blk, 0, st, g, in, w, (TO *)p.c);
+    else hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 4, 2, false>), grid, blk, 0, st, g, in, w, (TO *)p.c);
   } else {
-    if (krows) hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 4, true>), grid, blk, 0, st, g, in, w, (TO *)p.c);
-    else hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 4, false>), grid, blk, 0, st, g, in, w, (TO *)p.c);
+    if (krows) hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2, 4, true>), grid, blk, 0, st, g, in, w, (TO *)p.c);
+    else hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2, 4, false>), grid, blk, 0, st, g, in, w, (TO *)p.c);
   }
   return true;
 }
