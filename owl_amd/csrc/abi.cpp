@@ -406,7 +406,8 @@ int tile_m(int t) {
 void plan_splitk(const ConvParams &p, int in_t, int *split, int64_t *kps) {
   const int bm = tile_m(in_t), bk = tile_k(in_t);
   int64_t tiles = ((p.M + bm - 1) / bm) * (int64_t)((p.N + bm - 1) / bm);
-  int64_t want = (512 + tiles - 1) / tiles;
+  static const int target = env_int("OWL_AMD_SPLIT_TARGET", 512);  // workgroups to aim for (tuning)
+  int64_t want = (target + tiles - 1) / tiles;
   int64_t max_split = (p.K + 16 * bk - 1) / (16 * bk);
   static const int cap = env_int("OWL_AMD_MAX_SPLIT", 1 << 30);  // diagnostics / tuning
   if (max_split > cap) max_split = cap;

@@ -395,9 +395,10 @@ template <> __device__ __forceinline__ f32x16 mfma16<f16_t>(const v4s16 &a0, con
 
 template <typename TI, typename TO>
 __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
-  constexpr int BM = 128, BN = 128, BK = 32;
+  // BK 64 (80 KB of LDS) ran slower than BK 32: one workgroup per CU
+  constexpr int BM = 128, BN = 128, BK = 32, KP = BK / 32;  // KP pixels per loader thread
   constexpr int RS = BM * 2 + 64;  // LDS row stride in bytes (128 16-bit values + 64 bytes pad)
-  __shared__ __attribute__((aligned(16))) char lds[2][2][BK * RS];  // [stage][A|B]
+  __shared__ __attribute__((aligned(16))) char lds[2][2][BK * RS];  // [stage][A|B], 40 KB
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -421,81 +422,118 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   const __amdgpu_buffer_rsrc_t xr = make_rsrc(p.a, x_bytes);
   const __amdgpu_buffer_rsrc_t gr = make_rsrc(p.b, g_bytes);
 
-  // ---- loader assignment: 512 A vectors + 512 B vectors per K-step, 2 + 2 per thread
-  // vector v: pixel k = v / 16, 8-channel chunk c = v % 16
-  const int chunk = tid & 15, kpix0 = tid >> 4;  // pixels kpix0 and kpix0 + 16
-  // A rows m0 + 8*chunk .. +7 share one tap (Ci % 8 == 0)
-  const int am = m0 + chunk * 8;
-  const bool a_row_ok = am < M;
-  const int a_tap = a_row_ok ? am / p.Ci : 0;
-  const int a_ci = am - a_tap * p.Ci;
-  int ta, tc, te;
-  {
-    int kwd = p.kw * p.kd;
-    ta = a_tap / kwd;
-    int r = a_tap - ta * kwd;
-    tc = r / p.kd;
-    te = r - tc * p.kd;
+  // ---- loader assignment: per K-step the A tile (BK pixels x 128 rows m)
+  // and the B tile (BK pixels x 128 columns n) are 16 16-byte chunks per
+  // pixel.  Thread t loads pixels kp = t / 8 and kp + 32 of the step, and of
+  // each the chunks c8 = t % 8 and c8 + 8 of both tiles.
+  const int c8 = tid & 7, kp = tid >> 3;
+  // A chunk q covers rows m0 + 8 (c8 + 8 q) .. + 7, all in one tap (Ci % 8 == 0):
+  // tap window offset (tdh, tdw, tdd) and the chunk's element offset from
+  // the pixel's window origin
+  int tdh[2], tdw[2], tdd[2], toff[2];
+  bool aok[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int am = m0 + 8 * (c8 + 8 * q);
+    aok[q] = am < M;
+    const int tap = aok[q] ? am / p.Ci : 0;
+    const int ci = aok[q] ? am - tap * p.Ci : 0;
+    const int kwd = p.kw * p.kd;
+    const int ta = tap / kwd, r = tap - (tap / kwd) * kwd;
+    const int tc = r / p.kd, te = r - (r / p.kd) * p.kd;
+    tdh[q] = ta * p.rh - p.ph;
+    tdw[q] = tc * p.rw - p.pw;
+    tdd[q] = te * p.rd - p.pd;
+    toff[q] = ((tdh[q] * p.W + tdw[q]) * p.D + tdd[q]) * p.Ci + ci;
   }
-  const int dh = ta * p.rh - p.ph, dw = tc * p.rw - p.pw, dd = te * p.rd - p.pd;
-  const int bn = n0 + chunk * 8;
-  const bool b_col_ok = bn < N;
+  const int bn = n0 + 8 * c8;
+  const bool bok0 = bn < N, bok1 = bn + 64 < N;
 
-  // Output-pixel coordinates (b, i, j, l) of the thread's two loader pixels
-  // for the next load.  They advance by BK pixels per K-step: BK is split
-  // once into the same mixed radix (Do, Wo, Ho) and added digit by digit
-  // with carries, so the loop needs no integer division.
-  int sl, sj, si, sb;
+  // Pixel positions, advanced by BK per K-step without division: the step is
+  // split once into digits of the radix (Do, Wo, Ho); the scaled digits
+  // pld = l sd, pjw = j sw, pih = i sh carry into each other, and the
+  // element offset pbase of the pixel's window origin follows with a fixed
+  // correction per carry.
+  const int kend32 = (int)kend;
+  int pk[KP], pld[KP], pjw[KP], pih[KP], pbase[KP], gbase[KP];
+#pragma unroll
+  for (int h = 0; h < KP; ++h) {
+    int t = (int)kbeg + kp + 32 * h;
+    pk[h] = t;
+    const int l = t % p.Do; t /= p.Do;
+    const int j = t % p.Wo; t /= p.Wo;
+    const int i = t % p.Ho, b = t / p.Ho;
+    pld[h] = l * p.sd; pjw[h] = j * p.sw; pih[h] = i * p.sh;
+    pbase[h] = (((b * p.H + pih[h]) * p.W + pjw[h]) * p.D + pld[h]) * p.Ci;
+    gbase[h] = pk[h] * p.Co + bn;  // dy element offset of chunk 0
+  }
+  int inc_l, inc_j, inc_i, base_step;
+  const int dx_l = p.sd * p.Ci, dx_j = p.sw * p.D * p.Ci, dx_i = p.sh * p.W * p.D * p.Ci;
+  const int dx_b = p.H * p.W * p.D * p.Ci;
   {
     int t = BK;
-    sl = t % p.Do; t /= p.Do;
-    sj = t % p.Wo; t /= p.Wo;
-    si = t % p.Ho; sb = t / p.Ho;
+    const int sl = t % p.Do; t /= p.Do;
+    const int sj = t % p.Wo; t /= p.Wo;
+    const int si = t % p.Ho, sb = t / p.Ho;
+    inc_l = sl * p.sd; inc_j = sj * p.sw; inc_i = si * p.sh;
+    base_step = sl * dx_l + sj * dx_j + si * dx_i + sb * dx_b;
   }
-  int pk[2], pl[2], pj[2], pi[2], pb[2];
+  const int lim_l = p.Do * p.sd, lim_j = p.Wo * p.sw, lim_i = p.Ho * p.sh;
+  const int corr_l = dx_j - p.Do * dx_l, corr_j = dx_i - p.Wo * dx_j, corr_i = dx_b - p.Ho * dx_i;
+  const int gstep = BK * p.Co;
+
+  // register ring: loads run two K-steps ahead of the LDS stage they fill
+  typedef uint4 Stage[KP][4];  // per pixel: A chunks 0, 1, B chunks 0, 1
+  Stage rs0, rs1;
+  auto load = [&](Stage &rg) __attribute__((always_inline)) {
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    int t = (int)kbeg + kpix0 + 16 * h;
-    pk[h] = t;
-    pl[h] = t % p.Do; t /= p.Do;
-    pj[h] = t % p.Wo; t /= p.Wo;
-    pi[h] = t % p.Ho; pb[h] = t / p.Ho;
-  }
-  uint4 ra[2], rb[2];
-  auto load = [&]() __attribute__((always_inline)) {
+    for (int h = 0; h < KP; ++h) {
+      // branch-free (bitwise predicates, one select per load)
+      const bool kok = pk[h] < kend32;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      uint32_t vo = OOB, go = OOB;
-      if (pk[h] < kend) {
-        const int hh = pi[h] * p.sh + dh, ww = pj[h] * p.sw + dw, d = pl[h] * p.sd + dd;
-        if (a_row_ok && (unsigned)hh < (unsigned)p.H && (unsigned)ww < (unsigned)p.W && (unsigned)d < (unsigned)p.D)
-          vo = (uint32_t)((((pb[h] * p.H + hh) * p.W + ww) * p.D + d) * p.Ci + a_ci) * 2u;
-        if (b_col_ok) go = (uint32_t)(pk[h] * p.Co + bn) * 2u;
+      for (int q = 0; q < 2; ++q) {
+        const int hh = pih[h] + tdh[q], ww = pjw[h] + tdw[q], dd = pld[h] + tdd[q];
+        const bool ok = kok & aok[q] & ((unsigned)hh < (unsigned)p.H) & ((unsigned)ww < (unsigned)p.W) &
+                        ((unsigned)dd < (unsigned)p.D);
+        const uint32_t off = (uint32_t)(pbase[h] + toff[q]) * 2u;
+        rg[h][q] = bload(xr, ok ? off : OOB);
       }
-      ra[h] = bload(xr, vo);
-      rb[h] = bload(gr, go);
+      const uint32_t goff = (uint32_t)gbase[h] * 2u;
+      rg[h][2] = bload(gr, (kok & bok0) ? goff : OOB);
+      rg[h][3] = bload(gr, (kok & bok1) ? goff + 128u : OOB);
       // advance to the pixel BK further on
       pk[h] += BK;
-      int c;
-      pl[h] += sl;
-      c = pl[h] >= p.Do;
-      if (c) pl[h] -= p.Do;
-      pj[h] += sj + c;
-      c = pj[h] >= p.Wo;
-      if (c) pj[h] -= p.Wo;
-      pi[h] += si + c;
-      c = pi[h] >= p.Ho;
-      if (c) pi[h] -= p.Ho;
-      pb[h] += sb + c;
+      gbase[h] += gstep;
+      pbase[h] += base_step;
+      pld[h] += inc_l;
+      int c = pld[h] >= lim_l;
+      if (c) {
+        pld[h] -= lim_l;
+        pbase[h] += corr_l;
+      }
+      pjw[h] += inc_j + (c ? p.sw : 0);
+      c = pjw[h] >= lim_j;
+      if (c) {
+        pjw[h] -= lim_j;
+        pbase[h] += corr_j;
+      }
+      pih[h] += inc_i + (c ? p.sh : 0);
+      c = pih[h] >= lim_i;
+      if (c) {
+        pih[h] -= lim_i;
+        pbase[h] += corr_i;
+      }
     }
   };
-  auto store = [&](int st) __attribute__((always_inline)) {
+  auto store = [&](const Stage &rg, int st) __attribute__((always_inline)) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int row = kpix0 + 16 * h;
-      *(uint4 *)(&lds[st][0][row * RS + chunk * 16]) = ra[h];
-      *(uint4 *)(&lds[st][1][row * RS + chunk * 16]) = rb[h];
-    }
+    for (int h = 0; h < KP; ++h)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int row = kp + 32 * h;
+        *(uint4 *)(&lds[st][0][row * RS + (c8 + 8 * q) * 16]) = rg[h][q];
+        *(uint4 *)(&lds[st][1][row * RS + (c8 + 8 * q) * 16]) = rg[h][2 + q];
+      }
   };
 
   f32x16 acc[2][2];
@@ -512,35 +550,52 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   const int j16 = lane & 15;
   const int tr_off = ((j16 >> 2) + 8 * (lane >> 5)) * RS + 2 * (16 * ((lane >> 4) & 1) + 4 * (j16 & 3));
 
-  if (nk > 0) {
-    load();
-    store(0);
+  // one K-step: MFMAs from LDS stage ks & 1; rn holds step ks + 1 (stored
+  // to the other stage) and is refilled with step ks + 3.  Loads past the
+  // split's end read zeros, so every step issues the same loads.
+  auto kstep = [&](int ks, Stage &rn) __attribute__((always_inline)) {
+    const int cur = ks & 1;
+    const char *As = lds[cur][0], *Bs = lds[cur][1];
+    // every 16-pixel quarter's fragments in their own registers, so all the
+    // transposed reads can be in flight before the first MFMA
+    constexpr int NQ = BK / 16;
+    v4s16 a[NQ][2][2], b[NQ][2][2];  // [quarter][fragment][half]
+#pragma unroll
+    for (int s = 0; s < NQ; ++s)
+#pragma unroll
+      for (int f = 0; f < 2; ++f)
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+          const int off = tr_off + (16 * s + 4 * hf) * RS;
+          a[s][f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s16 *)(As + off + 2 * (wm * 64 + f * 32)));
+          b[s][f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s16 *)(Bs + off + 2 * (wn * 64 + f * 32)));
+        }
+#pragma unroll
+    for (int s = 0; s < NQ; ++s)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = mfma16<TI>(a[s][i][0], a[s][i][1], b[s][j][0], b[s][j][1], acc[i][j]);
+    // after the last step this fills a stage nobody reads again
+    store(rn, cur ^ 1);
+    load(rn);
+    // scheduling: the LDS reads as one group ahead of the MFMAs (an
+    // MFMA-by-MFMA interleave with the loads and writes ran slower)
+    __builtin_amdgcn_sched_group_barrier(0x100, 8 * NQ, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 4 * NQ, 0);
     __syncthreads();
-    for (int ks = 0; ks < nk; ++ks) {
-      const int cur = ks & 1;
-      const bool more = ks + 1 < nk;
-      if (more) load();
-      const char *As = lds[cur][0], *Bs = lds[cur][1];
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {  // two 16-pixel MFMA steps
-        v4s16 a[2][2], b[2][2];
-#pragma unroll
-        for (int f = 0; f < 2; ++f)
-#pragma unroll
-          for (int hf = 0; hf < 2; ++hf) {
-            const int off = tr_off + (16 * s + 4 * hf) * RS;
-            a[f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-                (lds_v4s16 *)(As + off + 2 * (wm * 64 + f * 32)));
-            b[f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-                (lds_v4s16 *)(Bs + off + 2 * (wn * 64 + f * 32)));
-          }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) acc[i][j] = mfma16<TI>(a[i][0], a[i][1], b[j][0], b[j][1], acc[i][j]);
-      }
-      if (more) store(cur ^ 1);
-      __syncthreads();
+  };
+
+  if (nk > 0) {
+    load(rs0);  // step 0
+    load(rs1);  // step 1
+    store(rs0, 0);
+    load(rs0);  // step 2
+    __syncthreads();
+    for (int ks = 0; ks < nk; ks += 2) {
+      kstep(ks, rs1);
+      if (ks + 1 < nk) kstep(ks + 1, rs0);
     }
   }
 
