@@ -98,6 +98,28 @@ int owl_amd_kernel_time(int kclass, double *total_ms, int64_t *launches, const c
 int owl_amd_conv_fwd(const owl_amd_conv_desc *d, int in_t, int out_t,
                      const void *x, const void *w, void *y, void *stream);
 
+/* Forward convolution with the conv layer's bias and activation fused into
+ * the epilogue: y = act(conv(x, w) + bias[co]).  Owl builds this layer as
+ * conv2d + b followed by an activation (owl_neural_neuron.ml:1253 Conv2D.run,
+ * owl_algodiff_ops.ml:762-778 relu); relu is fmax(v, 0) as
+ * owl_ndarray_maths_stub.c:3559-3569 (NaN -> 0).  bias: Co values in the
+ * accumulator type (float; double when out_t is F64), or null.  pre: null,
+ * or a second output receiving conv + bias before the activation (the
+ * backward pass needs it: Owl's relu adjoint looks at its input's sign). */
+enum { OWL_AMD_ACT_NONE = 0, OWL_AMD_ACT_RELU = 1 };
+int owl_amd_conv_fwd_bias_act(const owl_amd_conv_desc *d, int in_t, int out_t,
+                              const void *x, const void *w, const void *bias, int act,
+                              void *y, void *pre, void *stream);
+
+/* Adjoint of y = act(z + bias) for z laid out [npix][C] (C fastest):
+ * dz = dy * (signum(pre) + 1) / 2 for relu (owl_algodiff_ops.ml:774: 1 above
+ * zero, 1/2 at zero, 0 below, NaN propagates), dz = dy for no activation;
+ * db[c] = sum over pixels of dz[., c] when db is non-null (float; double for
+ * F64), summed in a fixed order.  dt: F32, F64, BF16, F16 for pre, dy, dz.
+ * dz may alias dy. */
+int owl_amd_bias_act_bwd(int dt, int act, int64_t npix, int64_t C, const void *pre,
+                         const void *dy, void *dz, void *db, void *stream);
+
 /* FP8 e4m3 forward with per-tensor scales: x = x_q * x_scale, w = w_q * w_scale
  * (the scales are what an amax pass produced; owl_amd_fp8_quantize below).
  * out_t: F32 or BF16. */

@@ -94,6 +94,8 @@ def lib():
             "owl_amd_kernel_time": ([i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64),
                                      ctypes.POINTER(ctypes.c_char_p)], i32),
             "owl_amd_conv_fwd": ([cd, i32, i32, vp, vp, vp, vp], i32),
+            "owl_amd_conv_fwd_bias_act": ([cd, i32, i32, vp, vp, vp, i32, vp, vp, vp], i32),
+            "owl_amd_bias_act_bwd": ([i32, i32, i64, i64, vp, vp, vp, vp, vp], i32),
             "owl_amd_conv_bwd_input": ([cd, i32, i32, vp, vp, vp, vp], i32),
             "owl_amd_conv_bwd_kernel_workspace": ([cd, i32], ctypes.c_size_t),
             "owl_amd_conv_bwd_kernel": ([cd, i32, i32, vp, vp, vp, vp, ctypes.c_size_t, vp], i32),

@@ -64,6 +64,20 @@ def build_piso(name, ff, dr_ab):
     return op
 
 
+def build_aiso(name, ff, dr):
+    """Array input, single output op (owl_algodiff_ops_builder.ml:384-430):
+    ``ff primals`` forward; ``dr idxs primals cp ca`` returns the adjoints of
+    the inputs at ``idxs`` (those on the tape), in that order."""
+    def op(*xs):
+        ps = [primal(x) for x in xs]
+        cp = ff(ps)
+        idxs = [i for i, x in enumerate(xs) if isinstance(x, Node)]
+        out = Node(cp, tuple(xs[i] for i in idxs), None, name)
+        out.adjoint_fn = lambda ca: dr(idxs, ps, cp, ca)
+        return out
+    return op
+
+
 def _topo(root):
     order, seen = [], set()
 
@@ -159,3 +173,33 @@ def finite_difference(loss, params, index, eps=1e-4):
     p[k][pos] -= 2 * eps
     dn = float(loss(*p))
     return (up - dn) / (2 * eps)
+
+
+def conv2d_bias_relu(x, w, b, stride, padding=SAME):
+    """Owl's conv layer with a relu, ``relu (conv2d ~padding x w stride + b)``
+    (owl_neural_neuron.ml:1253 and owl_algodiff_ops.ml:762-778), as one
+    ``build_aiso`` op: the forward is the fused GPU conv (bias and relu in the
+    epilogue, pre-activation kept for the adjoint); the adjoint is
+    ``bias_act_backward`` then the two conv backward passes.  The bias is
+    float32 (float64 for float64 layers) as the fused kernels take it."""
+    saved = {}
+
+    def ff(ps):
+        a, k, bb = ps
+        y, pre = nd.conv2d_bias_act(a, k, bb, stride, padding, act="relu", keep_pre=True)
+        saved["pre"] = pre
+        return y
+
+    def dr(idxs, ps, cp, ca):
+        a, k, bb = ps
+        dz, db = nd.bias_act_backward(saved["pre"], ca, act="relu", bias_grad=2 in idxs)
+        out = []
+        for i in idxs:
+            if i == 0:
+                out.append(nd.conv2d_backward_input(a, k, stride, dz))
+            elif i == 1:
+                out.append(nd.conv2d_backward_kernel(a, k, stride, dz, out_dtype=k.dtype))
+            else:
+                out.append(db.to(bb.dtype))
+        return out
+    return build_aiso("conv2d_bias_relu", ff, dr)(x, w, b)

@@ -1,6 +1,7 @@
 // C ABI of libowl_amd: argument checking, GEMM-view set-up, dtype dispatch,
 // split-K planning, and the Owl drop-in entry points that own device buffers.
 #include <stdarg.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -167,6 +168,7 @@ struct DeviceState {
   hipStream_t stream = nullptr;  // default priority, used by the drop-ins
   int *flag = nullptr;           // pooling orphan flag (one int)
   DevBuf ws;                     // split-K workspace
+  DevBuf red;                    // bias-gradient partial sums
   DevBuf slot[4];                // drop-in staging buffers
 };
 
@@ -463,8 +465,16 @@ extern "C" void owl_amd_cache_release(void) {
 // ------------------------------------------------------------------ device API: conv
 extern "C" int owl_amd_conv_fwd(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x,
                                 const void *w, void *y, void *stream) {
+  return owl_amd_conv_fwd_bias_act(d, in_t, out_t, x, w, nullptr, OWL_AMD_ACT_NONE, y, nullptr, stream);
+}
+
+extern "C" int owl_amd_conv_fwd_bias_act(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x,
+                                         const void *w, const void *bias, int act, void *y, void *pre,
+                                         void *stream) {
   int rc = check_conv(d);
   if (rc) return rc;
+  if (act != OWL_AMD_ACT_NONE && act != OWL_AMD_ACT_RELU) ARG_FAIL("unknown activation %d", act);
+  if (pre && pre == y) ARG_FAIL("pre must not alias y");
   ConvParams p = make_params(d);
   p.M = d->B * d->Ho * d->Wo * d->Do;
   p.N = p.Co;
@@ -472,8 +482,10 @@ extern "C" int owl_amd_conv_fwd(const owl_amd_conv_desc *d, int in_t, int out_t,
   if (p.M == 0 || p.N == 0) return OWL_AMD_OK;
   if (!x || !w || !y) ARG_FAIL("null pointer");
   p.a = x; p.b = w; p.c = y; p.ldc = p.Co;
+  p.bias = bias; p.pre = pre; p.act = act;
   const int v = vec_of(in_t);
-  bool aligned = dtype_size(in_t) && p.Ci % v == 0 && p.Co % v == 0 && aligned16(x, w, y);
+  bool aligned = dtype_size(in_t) && p.Ci % v == 0 && p.Co % v == 0 && aligned16(x, w, y) &&
+                 aligned16(pre, nullptr, nullptr);
   if (try_fast<0>(p, in_t, out_t, aligned, (hipStream_t)stream)) {
     OWL_CHECK_HIP(hipGetLastError());
     return OWL_AMD_OK;
@@ -589,6 +601,49 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   else if (out_t == OWL_AMD_F32) run_splitk_reduce<float, float>((const float *)ws, (float *)dw, n, p.split, st);
   else if (out_t == OWL_AMD_BF16) run_splitk_reduce<float, bf16_t>((const float *)ws, (bf16_t *)dw, n, p.split, st);
   else run_splitk_reduce<float, f16_t>((const float *)ws, (f16_t *)dw, n, p.split, st);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+// ------------------------------------------------------------------ fused epilogue adjoint
+extern "C" int owl_amd_bias_act_bwd(int dt, int act, int64_t npix, int64_t C, const void *pre, const void *dy,
+                                    void *dz, void *db, void *stream) {
+  if (act != OWL_AMD_ACT_NONE && act != OWL_AMD_ACT_RELU) ARG_FAIL("unknown activation %d", act);
+  if (npix < 0 || C <= 0 || C > INT32_MAX) ARG_FAIL("bad sizes npix=%lld C=%lld", (long long)npix, (long long)C);
+  if (dt != OWL_AMD_F32 && dt != OWL_AMD_F64 && dt != OWL_AMD_BF16 && dt != OWL_AMD_F16) {
+    owl_amd_set_error("unsupported dtype %d", dt);
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  if (!dy || !dz || (act == OWL_AMD_ACT_RELU && !pre)) ARG_FAIL("null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (npix == 0) {
+    if (db) OWL_CHECK_HIP(hipMemsetAsync(db, 0, (size_t)C * (dt == OWL_AMD_F64 ? 8 : 4), st));
+    return OWL_AMD_OK;
+  }
+  void *partial = nullptr;
+  if (db) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DeviceState *s;
+    int rc = ensure_state(&s);
+    if (rc) return rc;
+    rc = ensure_buf(s->red, (size_t)bias_act_chunks(npix) * (size_t)C * (dt == OWL_AMD_F64 ? 8 : 4), st);
+    if (rc) return rc;
+    partial = s->red.ptr;
+  }
+  KTimer kt(OWL_AMD_KT_AUX, "bias_act_bwd_kernel", st);
+  const int c = (int)C;
+  if (dt == OWL_AMD_F32)
+    run_bias_act_bwd<float, float>(act, npix, c, (const float *)pre, (const float *)dy, (float *)dz,
+                                   (float *)partial, (float *)db, st);
+  else if (dt == OWL_AMD_F64)
+    run_bias_act_bwd<double, double>(act, npix, c, (const double *)pre, (const double *)dy, (double *)dz,
+                                     (double *)partial, (double *)db, st);
+  else if (dt == OWL_AMD_BF16)
+    run_bias_act_bwd<bf16_t, float>(act, npix, c, (const bf16_t *)pre, (const bf16_t *)dy, (bf16_t *)dz,
+                                    (float *)partial, (float *)db, st);
+  else
+    run_bias_act_bwd<f16_t, float>(act, npix, c, (const f16_t *)pre, (const f16_t *)dy, (f16_t *)dz,
+                                   (float *)partial, (float *)db, st);
   OWL_CHECK_HIP(hipGetLastError());
   return OWL_AMD_OK;
 }

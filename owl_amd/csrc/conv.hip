@@ -574,10 +574,15 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
           TA *ws = (TA *)p.c + (int64_t)split_idx * M * N;
           ws[m * N + n] = (TA)acc[i][j][r];
         } else {
-          if (p.scale_a)
-            ((TO *)p.c)[m * p.ldc + n] = (TO)((float)acc[i][j][r] * oscale);
-          else
-            ((TO *)p.c)[m * p.ldc + n] = (TO)acc[i][j][r];
+          typedef typename MF::acc_elem TA;
+          TA v = p.scale_a ? (TA)((float)acc[i][j][r] * oscale) : (TA)acc[i][j][r];
+          if (PASS == 0 && (p.bias || p.act)) {
+            // fused bias and activation (Owl's relu is fmax(x, 0): NaN -> 0)
+            if (p.bias) v += ((const TA *)p.bias)[n];
+            if (p.pre) ((TO *)p.pre)[m * p.ldc + n] = (TO)v;
+            if (p.act == OWL_AMD_ACT_RELU) v = fmax(v, (TA)0);
+          }
+          ((TO *)p.c)[m * p.ldc + n] = (TO)v;
         }
       }
     }

@@ -328,7 +328,18 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
   // out-of-range voffset and the hardware drops the store.
   const uint32_t c_bytes = (uint32_t)((int64_t)M * p.ldc * sizeof(TO));
   const __amdgpu_buffer_rsrc_t crsrc = make_rsrc(p.c, c_bytes);
+  const __amdgpu_buffer_rsrc_t prsrc = make_rsrc(p.pre ? p.pre : p.c, c_bytes);
   const bool full = m0 + BM <= M;
+  // fused bias and activation of the forward (none for backward input)
+  const bool epi = PASS == 0 && (p.bias || p.act);
+  auto put = [&](__amdgpu_buffer_rsrc_t r, float v, uint32_t vo, int so) __attribute__((always_inline)) {
+    if constexpr (sizeof(TO) == 2) {
+      const TO h = (TO)v;
+      __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), r, (int)vo, so, 0);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)vo, so, 0);
+    }
+  };
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -336,6 +347,7 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
       const int n = n0 + wn * FN * 32 + j * 32 + frow;
       const int mrow = m0 + wm * FM * 32 + i * 32 + 4 * fkg;  // + 8*(r>>2) + (r&3)
       const uint32_t lane_off = (uint32_t)(mrow * p.ldc + n) * (uint32_t)sizeof(TO);
+      const float bn = (epi && p.bias && n < N) ? ((const float *)p.bias)[n] : 0.0f;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int dr = 8 * (r >> 2) + (r & 3);
@@ -343,13 +355,13 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
         uint32_t vo = lane_off;
         if (!full && mrow + dr >= M) vo = OOB;
         if (n >= N) vo = OOB;
-        if constexpr (sizeof(TO) == 2) {
-          TO h = (TO)acc[i][j][r];
-          __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), crsrc, (int)vo, so, 0);
-        } else {
-          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, (float)acc[i][j][r]), crsrc,
-                                                (int)vo, so, 0);
+        float v = acc[i][j][r];
+        if (epi) {
+          if (p.bias) v += bn;
+          if (p.pre) put(prsrc, v, vo, so);
+          if (p.act == OWL_AMD_ACT_RELU) v = fmaxf(v, 0.0f);
         }
+        put(crsrc, v, vo, so);
       }
     }
 }
@@ -669,6 +681,10 @@ struct HaloGeom {
   int ph, pw;          // leading pads (of this forward view)
   int B, TR;           // batch, output rows per tile
   int wCi, wCo;        // the kernel buffer's [kh][kw][wCi][wCo] sizes
+  // forward only: fused bias / activation and the pre-activation output
+  const float *bias;
+  void *pre;
+  int act;
 };
 
 // KROWS: the kernel operand is laid out [tap][n][k] (k contiguous): the
@@ -942,57 +958,77 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 
   // ---- epilogue: output pixel (b, i0 + p / Wo, p % Wo), channel n ----
   const uint32_t o_bytes = (uint32_t)((int64_t)g.B * g.Ho * g.Wo * g.N * sizeof(TO));
-  const __amdgpu_buffer_rsrc_t orr = make_rsrc(out, o_bytes);
   const int obase = (b * g.Ho + i0) * g.Wo;  // first output pixel of the tile
+  // fused bias and activation (forward only); relu is fmax(v, 0): NaN -> 0
+  const bool epi = PASS == 0 && (g.bias || g.act);
+  float bj[FN];
+#pragma unroll
+  for (int j = 0; j < FN; ++j) {
+    const int n = n0 + j * 32 + frow;
+    bj[j] = (epi && g.bias && n < g.N) ? g.bias[n] : 0.0f;
+  }
+  auto val = [&](int i, int j, int r, bool act) __attribute__((always_inline)) -> float {
+    float v = acc[i][j][r];
+    if (epi) {
+      if (g.bias) v += bj[j];
+      if (act && g.act == OWL_AMD_ACT_RELU) v = fmaxf(v, 0.0f);
+    }
+    return v;
+  };
   // staged output row: BN values, plus 16 B of padding when that still fits
   constexpr int ORS_PAD = BN * (int)sizeof(TO) + 16;
   constexpr int ORS = FM * 128 * ORS_PAD <= LDS_BYTES ? ORS_PAD : BN * (int)sizeof(TO);
-  if constexpr (FM * 128 * ORS <= LDS_BYTES) {
-    // Stage the tile in LDS as [pixel][channel] rows (the K loop ended on a
-    // barrier, so the LDS is free), then store whole 16-byte chunks.
+  auto emit = [&](TO *dst, bool act) __attribute__((always_inline)) {
+    const __amdgpu_buffer_rsrc_t orr = make_rsrc(dst, o_bytes);
+    if constexpr (FM * 128 * ORS <= LDS_BYTES) {
+      // Stage the tile in LDS as [pixel][channel] rows (the LDS is free after
+      // the K loop's last barrier), then store whole 16-byte chunks.
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        char *rowp = smem + (wave * FM * 32 + i * 32 + 4 * fkg) * ORS + (j * 32 + frow) * (int)sizeof(TO);
+        for (int j = 0; j < FN; ++j) {
+          char *rowp = smem + (wave * FM * 32 + i * 32 + 4 * fkg) * ORS + (j * 32 + frow) * (int)sizeof(TO);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const float v = acc[i][j][r];
-          *(TO *)(rowp + (8 * (r >> 2) + (r & 3)) * ORS) = (TO)v;
+          for (int r = 0; r < 16; ++r) *(TO *)(rowp + (8 * (r >> 2) + (r & 3)) * ORS) = (TO)val(i, j, r, act);
         }
+      __syncthreads();
+      constexpr int CPR = BN * (int)sizeof(TO) / 16;  // 16-byte chunks per row
+      constexpr int EPC = 16 / (int)sizeof(TO);       // values per chunk
+      for (int c = tid; c < P * CPR; c += 256) {
+        const int pix = c / CPR, ch = c - (c / CPR) * CPR;
+        const int n = n0 + ch * EPC;
+        const uint4 v = *(const uint4 *)(smem + pix * ORS + ch * 16);
+        const uint32_t vo = n < g.N ? (uint32_t)((obase + pix) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), orr, (int)vo, 0, 0);
       }
-    __syncthreads();
-    constexpr int CPR = BN * (int)sizeof(TO) / 16;  // 16-byte chunks per row
-    constexpr int EPC = 16 / (int)sizeof(TO);       // values per chunk
-    for (int c = tid; c < P * CPR; c += 256) {
-      const int pix = c / CPR, ch = c - (c / CPR) * CPR;
-      const int n = n0 + ch * EPC;
-      const uint4 v = *(const uint4 *)(smem + pix * ORS + ch * 16);
-      const uint32_t vo = n < g.N ? (uint32_t)((obase + pix) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
-      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), orr, (int)vo, 0, 0);
-    }
-  } else {
+    } else {
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        const int n = n0 + j * 32 + frow;
-        const int prow = wave * FM * 32 + i * 32 + 4 * fkg;
+        for (int j = 0; j < FN; ++j) {
+          const int n = n0 + j * 32 + frow;
+          const int prow = wave * FM * 32 + i * 32 + 4 * fkg;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int pp = prow + 8 * (r >> 2) + (r & 3);
-          const bool ok = pp < P && n < g.N;
-          const uint32_t vo = ok ? (uint32_t)((obase + pp) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
-          const float v = acc[i][j][r];
-          if constexpr (sizeof(TO) == 2) {
-            const TO h = (TO)v;
-            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), orr, (int)vo, 0, 0);
-          } else {
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), orr, (int)vo, 0, 0);
+          for (int r = 0; r < 16; ++r) {
+            const int pp = prow + 8 * (r >> 2) + (r & 3);
+            const bool ok = pp < P && n < g.N;
+            const uint32_t vo = ok ? (uint32_t)((obase + pp) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
+            const float v = val(i, j, r, act);
+            if constexpr (sizeof(TO) == 2) {
+              const TO h = (TO)v;
+              __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), orr, (int)vo, 0, 0);
+            } else {
+              __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), orr, (int)vo, 0, 0);
+            }
           }
         }
-      }
+    }
+  };
+  if (PASS == 0 && g.pre) {
+    emit((TO *)g.pre, false);
+    __syncthreads();  // the staging area is reused for y
   }
+  emit(out, true);
 }
 
 // Fragments per wave along the pixels: 4 (128 pixels) for 64-wide channel
@@ -1016,6 +1052,9 @@ static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom
   }
   g.kh = p.kh; g.kw = p.kw; g.rh = p.rh; g.rw = p.rw;
   g.B = p.B; g.wCi = p.Ci; g.wCo = p.Co;
+  g.bias = PASS == 0 ? (const float *)p.bias : nullptr;
+  g.pre = PASS == 0 ? p.pre : nullptr;
+  g.act = PASS == 0 ? p.act : OWL_AMD_ACT_NONE;
   if (g.C % 32 != 0 || g.N % 8 != 0 || g.Wo > 256) return false;
   // 128 pixels per wave (FM = 4) when the channel slice is 64 wide, else 64
   g.TR = halo_fm(g.N) * 128 / g.Wo;

@@ -23,6 +23,12 @@ struct ConvParams {
   const float *scale_b;
   int split;            // backward kernel: number of K splits
   int64_t k_per_split;  // multiple of the K-step
+  // forward epilogue (owl_amd_conv_fwd_bias_act): y = act(conv + bias[n]);
+  // bias in the accumulator type (float, or double for FP64), pre receives
+  // conv + bias before the activation (same layout and type as y)
+  const void *bias;
+  void *pre;
+  int act;  // OWL_AMD_ACT_*
 };
 
 template <int PASS, typename TI, typename TO>
@@ -61,6 +67,13 @@ template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st);
 template <typename TI> int tile_bm();
 template <typename TI> int tile_bk();
+
+// Adjoint of the fused bias / activation epilogue (epilogue.hip); partial
+// holds bias_act_chunks(npix) x C values of A when db is wanted.
+int bias_act_chunks(int64_t npix);
+template <typename T, typename A>
+void run_bias_act_bwd(int act, int64_t npix, int C, const T *pre, const T *dy, T *dz, A *partial, A *db,
+                      hipStream_t st);
 
 // Kernel timing (owl_amd_kernel_timing): a KTimer brackets one launch with
 // HIP events on its stream while timing is enabled, and does nothing else.

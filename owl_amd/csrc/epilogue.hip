@@ -1,0 +1,119 @@
+// Adjoint of the conv layer's fused epilogue y = act(z + bias), z = conv(x, w):
+//   dz = dy * (signum(pre) + 1) / 2   (relu; owl_algodiff_ops.ml:774 with
+//        signum(x) = x > 0 ? 1 : x < 0 ? -1 : x, owl_ndarray_maths_stub.c:2756,
+//        so the factor is 1 above zero, 1/2 at +-0, 0 below, NaN for NaN)
+//   dz = dy                           (no activation)
+//   db[c] = sum over pixels of dz[., c]
+// pre = z + bias is [npix][C] with C fastest.  The bias gradient is summed in
+// a fixed order -- within a thread in pixel order, then over the lanes of a
+// block in lane order, then over pixel chunks in chunk order -- so every
+// launch gives the same bits.
+#include "owl_amd_common.h"
+#include "conv_kernels.h"
+
+namespace owl_amd {
+
+// relu factor of Owl's adjoint, in the arithmetic type A
+template <typename A> __device__ __forceinline__ A relu_factor(A z) {
+  const A s = z > (A)0 ? (A)1 : (z < (A)0 ? (A)-1 : z);  // signum, keeping +-0 and NaN
+  return (s + (A)1) / (A)2;
+}
+
+// Block: 256 threads = PL pixel lanes x CW channel vectors of VEC channels.
+// Grid: (pixel chunks of PCH pixels, channel-vector blocks of CW).
+template <typename T, typename A, int VEC>
+__global__ __launch_bounds__(256) void bias_act_bwd_kernel(int act, int64_t npix, int C, int CW, int PCH,
+                                                           const T *__restrict__ pre, const T *dy, T *dz,
+                                                           A *__restrict__ partial) {
+  __shared__ A red[256 * VEC];
+  const int t = threadIdx.x;
+  const int cw = t % CW, pl = t / CW, PL = 256 / CW;
+  const int cv = blockIdx.y * CW + cw;  // channel vector
+  const int nv = C / VEC;
+  const int64_t p0 = (int64_t)blockIdx.x * PCH;
+  int64_t p1 = p0 + PCH;
+  if (p1 > npix) p1 = npix;
+  A s[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) s[v] = (A)0;
+  if (pl < PL && cv < nv) {
+    for (int64_t p = p0 + pl; p < p1; p += PL) {
+      const int64_t e = p * C + (int64_t)cv * VEC;
+      T g[VEC], z[VEC];
+      __builtin_memcpy(g, dy + e, sizeof(g));
+      if (act == OWL_AMD_ACT_RELU) __builtin_memcpy(z, pre + e, sizeof(z));
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        A d = (A)g[v];
+        if (act == OWL_AMD_ACT_RELU) d = d * relu_factor<A>((A)z[v]);
+        g[v] = (T)d;
+        s[v] += (A)g[v];  // the stored (rounded) gradient is what the bias sums
+      }
+      __builtin_memcpy(dz + e, g, sizeof(g));
+    }
+  }
+  if (!partial) return;
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) red[t * VEC + v] = s[v];
+  __syncthreads();
+  if (pl == 0 && cv < nv) {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      A a = (A)0;
+      for (int l = 0; l < PL; ++l) a += red[(l * CW + cw) * VEC + v];
+      partial[(int64_t)blockIdx.x * C + (int64_t)cv * VEC + v] = a;
+    }
+  }
+}
+
+// db[c] = sum over chunks (in chunk order) of partial[chunk][c]
+template <typename A>
+__global__ __launch_bounds__(256) void bias_grad_sum_kernel(const A *__restrict__ partial, int nchunks, int C,
+                                                           A *__restrict__ db) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= C) return;
+  A a = (A)0;
+  for (int k = 0; k < nchunks; ++k) a += partial[(int64_t)k * C + c];
+  db[c] = a;
+}
+
+// Number of pixel chunks the launch below uses (the partial buffer holds
+// chunks x C values of A).
+int bias_act_chunks(int64_t npix) {
+  const int64_t PCH = 256;
+  return (int)((npix + PCH - 1) / PCH);
+}
+
+template <typename T, typename A>
+void run_bias_act_bwd(int act, int64_t npix, int C, const T *pre, const T *dy, T *dz, A *partial, A *db,
+                      hipStream_t st) {
+  constexpr int VEC = 16 / sizeof(T);
+  const int PCH = 256;
+  const int nchunks = bias_act_chunks(npix);
+  const bool vec_ok = C % VEC == 0 && ((((uintptr_t)pre | (uintptr_t)dy | (uintptr_t)dz) & 15) == 0);
+  const int vw = vec_ok ? VEC : 1;
+  const int nv = C / vw;
+  int CW = nv < 256 ? nv : 256;
+  while (256 % CW != 0) --CW;  // lanes tile the block exactly
+  dim3 grid((unsigned)nchunks, (unsigned)((nv + CW - 1) / CW));
+  if (vec_ok)
+    hipLaunchKernelGGL((bias_act_bwd_kernel<T, A, VEC>), grid, dim3(256), 0, st, act, npix, C, CW, PCH, pre, dy, dz,
+                       db ? partial : (A *)nullptr);
+  else
+    hipLaunchKernelGGL((bias_act_bwd_kernel<T, A, 1>), grid, dim3(256), 0, st, act, npix, C, CW, PCH, pre, dy, dz,
+                       db ? partial : (A *)nullptr);
+  if (db)
+    hipLaunchKernelGGL((bias_grad_sum_kernel<A>), dim3((unsigned)((C + 255) / 256)), dim3(256), 0, st, partial,
+                       nchunks, C, db);
+}
+
+template void run_bias_act_bwd<float, float>(int, int64_t, int, const float *, const float *, float *, float *,
+                                             float *, hipStream_t);
+template void run_bias_act_bwd<double, double>(int, int64_t, int, const double *, const double *, double *,
+                                               double *, double *, hipStream_t);
+template void run_bias_act_bwd<bf16_t, float>(int, int64_t, int, const bf16_t *, const bf16_t *, bf16_t *,
+                                              float *, float *, hipStream_t);
+template void run_bias_act_bwd<f16_t, float>(int, int64_t, int, const f16_t *, const f16_t *, f16_t *, float *,
+                                             float *, hipStream_t);
+
+}  // namespace owl_amd

@@ -526,3 +526,74 @@ def max_pool3d_backward(padding, input, kernel, stride, output_grad):
 
 def avg_pool3d_backward(padding, input, kernel, stride, output_grad):
     return _pool_bwd("avg_pool3d_backward", 3, False, padding, input, kernel, stride, output_grad)
+
+
+# ----------------------------------------------------------------------------- fused conv layer epilogue
+_ACT = {None: 0, "none": 0, "relu": 1}
+
+
+def _act_code(act):
+    _verify(act in _ACT, f"unknown activation {act!r} (none or relu)")
+    return _ACT[act]
+
+
+def _conv_bias_act(name, nd, x, k, bias, stride, rate, padding, act, keep_pre, out_dtype):
+    _verify(_is_torch(x) and _is_torch(k), f"{name}: the fused layer ops take device tensors")
+    _check_conv_args(name, nd, x, k, stride, rate)
+    x, k = _contig(x), _contig(k)
+    g = conv_problem(list(x.shape), list(k.shape), stride, rate, padding)
+    y = _empty_like_kind(x, [g.B, *g.sp_out, g.Co], dtype=out_dtype)
+    pre = torch.empty_like(y) if keep_pre else None
+    acc_dt = torch.float64 if y.dtype == torch.float64 else torch.float32
+    if bias is not None:
+        _verify(_is_torch(bias) and tuple(bias.shape) == (g.Co,),
+                f"{name}: bias must be a device tensor of shape [{g.Co}]")
+        _verify(bias.dtype == acc_dt, f"{name}: bias must be {acc_dt} for {y.dtype} output")
+        bias = _contig(bias)
+    d = g.desc()
+    _lib.check(_lib.lib().owl_amd_conv_fwd_bias_act(
+        ctypes.byref(d), _tdt(x), _tdt(y), _ptr(x), _ptr(k), _ptr(bias) if bias is not None else None,
+        _act_code(act), _ptr(y), _ptr(pre) if pre is not None else None, _stream()))
+    return (y, pre) if keep_pre else y
+
+
+def conv2d_bias_act(input, kernel, bias, stride, padding=SAME, act="relu", keep_pre=False, out_dtype=None):
+    """Owl's 2-D conv layer forward, ``act (conv2d ~padding x w stride + b)``
+    (owl_neural_neuron.ml:1253 Conv2D.run, then the activation), with the bias
+    add and the activation in the convolution's epilogue.  relu is
+    ``fmax(v, 0)`` as owl_ndarray_maths_stub.c:3559-3569.  Device tensors
+    only; bias is [Co] in float32 (float64 for float64 output) or None.
+    Returns y, or (y, pre) with pre = conv + bias when keep_pre."""
+    return _conv_bias_act("conv2d_bias_act", 2, input, kernel, bias, stride, None, padding, act, keep_pre,
+                          out_dtype)
+
+
+def conv3d_bias_act(input, kernel, bias, stride, padding=SAME, act="relu", keep_pre=False, out_dtype=None):
+    """The 3-D form of ``conv2d_bias_act`` (Owl's Conv3D layer)."""
+    return _conv_bias_act("conv3d_bias_act", 3, input, kernel, bias, stride, None, padding, act, keep_pre,
+                          out_dtype)
+
+
+def bias_act_backward(pre, output_grad, act="relu", bias_grad=True):
+    """Adjoint of ``y = act(pre)`` with ``pre = conv + b``: returns (dz, db).
+    dz = dy * (signum(pre) + 1) / 2 for relu -- Owl's relu adjoint
+    (owl_algodiff_ops.ml:774), 1 above zero, 1/2 at zero, 0 below -- or dy
+    for no activation; db sums dz over every axis but the last (float32, or
+    float64 for float64 tensors), in a fixed order.  Device tensors only."""
+    _verify(_is_torch(output_grad), "bias_act_backward takes device tensors")
+    code = _act_code(act)
+    dy = _contig(output_grad)
+    if code:
+        _verify(pre is not None and _is_torch(pre) and pre.shape == dy.shape and pre.dtype == dy.dtype,
+                "bias_act_backward: pre must match the output gradient")
+        pre = _contig(pre)
+    C = dy.shape[-1]
+    npix = dy.numel() // C if C else 0
+    dz = torch.empty_like(dy)
+    db = None
+    if bias_grad:
+        db = torch.empty(C, dtype=torch.float64 if dy.dtype == torch.float64 else torch.float32, device=dy.device)
+    _lib.check(_lib.lib().owl_amd_bias_act_bwd(
+        _tdt(dy), code, npix, C, _ptr(pre) if code else None, _ptr(dy), _ptr(dz),
+        _ptr(db) if db is not None else None, _stream()))
+    return dz, db
