@@ -119,6 +119,31 @@ def main():
     tf_b = 3 * FLOP_PASS * args.steps / wall_b / 1e12
     tf_f = 3 * FLOP_PASS * args.steps / wall_f / 1e12
 
+    # the conv layer with its bias and relu (Owl's Conv2D + relu): fused into
+    # the BF16 forward kernel's epilogue, against conv then separate ops
+    bias = torch.randn(CO, device=dev, generator=g) * 0.1
+    zero = torch.zeros((), device=dev, dtype=torch.bfloat16)
+
+    def t_loop(f, reps=20):
+        for _ in range(3):
+            f()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+    fused = {
+        "fused_us": round(t_loop(lambda: nd.conv2d_bias_act(xb, wb, bias, [1, 1], SAME)), 1),
+        "fused_keep_pre_us": round(t_loop(lambda: nd.conv2d_bias_act(xb, wb, bias, [1, 1], SAME,
+                                                                     keep_pre=True)), 1),
+        "unfused_us": round(t_loop(lambda: torch.fmax(nd.conv2d(xb, wb, [1, 1], SAME) + bias.bfloat16(), zero)),
+                            1),
+        "layer": "BF16 relu(conv2d(x, w) + b), x[64,56,56,64] w[3,3,64,128]",
+    }
+
     # pooling on the FP32 layer output
     y32 = nd.conv2d(x32, w32, [1, 1], SAME)
     gp = torch.randn(B, H // 2, W // 2, CO, device=dev, generator=g)
@@ -221,6 +246,7 @@ def main():
                         "fp32": dict(zip(names, [round(v, 4) for v in per_f]))},
         "errors": errs,
         "pool_max2x2s2": pool,
+        "conv_bias_relu_bf16": fused,
         "kernels_bf16": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -111,6 +111,19 @@ int owl_amd_conv_fwd_bias_act(const owl_amd_conv_desc *d, int in_t, int out_t,
                               const void *x, const void *w, const void *bias, int act,
                               void *y, void *pre, void *stream);
 
+/* 2-D nearest-neighbour upsampling of NHWC tensors by integer scales
+ * (sh along H = Owl's cols, sw along W = rows).  Forward:
+ * y[b,i,j,c] = x[b, min(i/sh, H-1), min(j/sw, W-1), c] -- Owl's upsampling2d is
+ * repeat x [|1;sh;sw;1|] (owl_dense_ndarray_generic.ml:8223-8234), i.e.
+ * Ho = H*sh, Wo = W*sw.  Backward: replaces owl_ndarray_upsampling_impl.h:9-60
+ * (spatial_backward): dx[b,hi,wi,c] = sum of dy over the output cells mapping
+ * to (hi,wi), in Owl's order (bit-identical for F32 / F64).
+ * dt: F32, F64, BF16, F16. */
+int owl_amd_upsampling2d_fwd(int dt, int64_t B, int64_t H, int64_t W, int64_t C, int64_t Ho,
+                             int64_t Wo, int64_t sh, int64_t sw, const void *x, void *y, void *stream);
+int owl_amd_upsampling2d_bwd(int dt, int64_t B, int64_t H, int64_t W, int64_t C, int64_t Ho,
+                             int64_t Wo, int64_t sh, int64_t sw, const void *dy, void *dx, void *stream);
+
 /* Adjoint of y = act(z + bias) for z laid out [npix][C] (C fastest):
  * dz = dy * (signum(pre) + 1) / 2 for relu (owl_algodiff_ops.ml:774: 1 above
  * zero, 1/2 at zero, 0 below, NaN propagates), dz = dy for no activation;
@@ -262,6 +275,13 @@ int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max,
       int64_t output_cols, int64_t output_rows, int64_t output_dpts,                \
       int64_t col_stride, int64_t row_stride, int64_t dpt_stride, int64_t padding
 
+/* Upsampling backward: owl_ndarray_upsampling_impl.h:9-14 (input = dx, written;
+ * output = dy, read). */
+#define OWL_AMD_UPSAMPLE_ARGS(T)                                                    \
+  T *input, T *output, int64_t batches, int64_t input_cols, int64_t input_rows,     \
+      int64_t in_channel, int64_t output_cols, int64_t output_rows,                 \
+      int64_t col_scale, int64_t row_scale
+
 #define OWL_AMD_DECLARE_DROPINS(N, T)                                                   \
   /* owl_ndarray_conv_impl.h:246 spatial, :903 spatial_im2col, :1570 mec, :2440 naive */ \
   int owl_amd_##N##_ndarray_conv_spatial(OWL_AMD_CONV2D_FWD_ARGS(T));                    \
@@ -308,7 +328,12 @@ int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max,
   int owl_amd_##N##_ndarray_maxpool_cuboid(OWL_AMD_POOL3D_FWD_ARGS(T));                  \
   int owl_amd_##N##_ndarray_avgpool_cuboid(OWL_AMD_POOL3D_FWD_ARGS(T));                  \
   int owl_amd_##N##_ndarray_maxpool_cuboid_backward(OWL_AMD_POOL3D_BWD_ARGS(T));         \
-  int owl_amd_##N##_ndarray_avgpool_cuboid_backward(OWL_AMD_POOL3D_BWD_ARGS(T));
+  int owl_amd_##N##_ndarray_avgpool_cuboid_backward(OWL_AMD_POOL3D_BWD_ARGS(T));        \
+  /* owl_ndarray_upsampling_impl.h:9 spatial_backward (dx from dy); spatial is   */    \
+  /* the forward with the same arguments (output written), which Owl computes    */    \
+  /* with repeat (owl_dense_ndarray_generic.ml:8223) and has no stub for         */    \
+  int owl_amd_##N##_ndarray_upsampling_spatial_backward(OWL_AMD_UPSAMPLE_ARGS(T));       \
+  int owl_amd_##N##_ndarray_upsampling_spatial(OWL_AMD_UPSAMPLE_ARGS(T));
 
 OWL_AMD_DECLARE_DROPINS(float32, float)
 OWL_AMD_DECLARE_DROPINS(float64, double)

@@ -52,6 +52,8 @@ def lib():
             getattr(L, "oracle_pool_fwd_" + s).argtypes = [ctypes.POINTER(PoolGeom), ctypes.c_int, vp, vp]
             getattr(L, "oracle_pool_argmax_" + s).argtypes = [ctypes.POINTER(PoolGeom), vp, vp, vp]
             getattr(L, "oracle_pool_bwd_" + s).argtypes = [ctypes.POINTER(PoolGeom), ctypes.c_int, vp, vp, vp]
+            for d in ("fwd", "bwd"):
+                getattr(L, f"oracle_upsampling_{d}_" + s).argtypes = [vp, vp] + [ctypes.c_int64] * 8
         _LIB = L
     return _LIB
 
@@ -192,4 +194,26 @@ def pool_bwd(g, x, dy, is_max):
     dy = np.ascontiguousarray(dy, x.dtype)
     dx = np.zeros(g.B * g.H * g.W * g.D * g.C, x.dtype)
     getattr(lib(), "oracle_pool_bwd_" + _suf(x.dtype))(ctypes.byref(g), int(is_max), _p(x), _p(dy), _p(dx))
+    return dx
+
+
+def upsampling_fwd(x, size, out_cols=None, out_rows=None):
+    """y = repeat x [|1; size0; size1; 1|] (Owl's upsampling2d), optionally to an
+    explicit output extent (clamped to the last input row / column)."""
+    x = np.ascontiguousarray(x)
+    B, H, W, C = x.shape
+    Ho = H * size[0] if out_cols is None else out_cols
+    Wo = W * size[1] if out_rows is None else out_rows
+    y = np.empty([B, Ho, Wo, C], x.dtype)
+    getattr(lib(), "oracle_upsampling_fwd_" + _suf(x.dtype))(_p(x), _p(y), B, H, W, C, Ho, Wo, size[0], size[1])
+    return y
+
+
+def upsampling_bwd(dy, in_shape, size):
+    """Owl's upsampling2d_backward stub restated (scatter in Owl's loop order)."""
+    dy = np.ascontiguousarray(dy)
+    B, H, W, C = in_shape
+    dx = np.empty([B, H, W, C], dy.dtype)
+    getattr(lib(), "oracle_upsampling_bwd_" + _suf(dy.dtype))(_p(dx), _p(dy), B, H, W, C, dy.shape[1], dy.shape[2],
+                                                            size[0], size[1])
     return dx

@@ -363,3 +363,49 @@ int oracle_conv_bwd_kernel_owl_f32(const oracle_conv_geom *g, const float *x,
 
 DEFINE_POOL(f32, float)
 DEFINE_POOL(f64, double)
+
+/* ---------------------------------------------------------------- upsampling
+ * Backward: owl_ndarray_upsampling_impl.h:9-60 (spatial_backward), restated
+ * loop for loop -- dx zeroed, then for b, output col c, output row r, channel
+ * i: dx[b, min(c / col_scale, in_cols - 1), min(r / row_scale, in_rows - 1), i]
+ * += dy[b, c, r, i].  Forward: repeat x [|1; col_scale; row_scale; 1|]
+ * (owl_dense_ndarray_generic.ml:8223-8234) -- each input cell copied to its
+ * col_scale x row_scale block; cells past the scaled extent take the last
+ * input row / column, mirroring the backward's clamp. */
+#define DEFINE_UPSAMPLE(S, T)                                                          \
+  void oracle_upsampling_bwd_##S(T *dx, const T *dy, int64_t B, int64_t in_cols,       \
+                                 int64_t in_rows, int64_t C, int64_t out_cols,         \
+                                 int64_t out_rows, int64_t col_scale, int64_t row_scale) { \
+    memset(dx, 0, (size_t)(B * in_cols * in_rows * C) * sizeof(T));                    \
+    for (int64_t b = 0; b < B; ++b)                                                    \
+      for (int64_t c = 0; c < out_cols; ++c) {                                         \
+        int64_t ic = c / col_scale;                                                    \
+        if (ic > in_cols - 1) ic = in_cols - 1;                                        \
+        for (int64_t r = 0; r < out_rows; ++r) {                                       \
+          int64_t ir = r / row_scale;                                                  \
+          if (ir > in_rows - 1) ir = in_rows - 1;                                      \
+          for (int64_t i = 0; i < C; ++i)                                              \
+            dx[((b * in_cols + ic) * in_rows + ir) * C + i] +=                         \
+                dy[((b * out_cols + c) * out_rows + r) * C + i];                       \
+        }                                                                              \
+      }                                                                                \
+  }                                                                                    \
+  void oracle_upsampling_fwd_##S(const T *x, T *y, int64_t B, int64_t in_cols,         \
+                                 int64_t in_rows, int64_t C, int64_t out_cols,         \
+                                 int64_t out_rows, int64_t col_scale, int64_t row_scale) { \
+    for (int64_t b = 0; b < B; ++b)                                                    \
+      for (int64_t c = 0; c < out_cols; ++c) {                                         \
+        int64_t ic = c / col_scale;                                                    \
+        if (ic > in_cols - 1) ic = in_cols - 1;                                        \
+        for (int64_t r = 0; r < out_rows; ++r) {                                       \
+          int64_t ir = r / row_scale;                                                  \
+          if (ir > in_rows - 1) ir = in_rows - 1;                                      \
+          for (int64_t i = 0; i < C; ++i)                                              \
+            y[((b * out_cols + c) * out_rows + r) * C + i] =                           \
+                x[((b * in_cols + ic) * in_rows + ir) * C + i];                        \
+        }                                                                              \
+      }                                                                                \
+  }
+
+DEFINE_UPSAMPLE(f32, float)
+DEFINE_UPSAMPLE(f64, double)

@@ -47,6 +47,7 @@ POOL2D_BWD = "ppp" + "i" * 12
 POOL2D_ARG = "ppp" + "i" * 12
 POOL3D_FWD = "pp" + "i" * 15
 POOL3D_BWD = "ppp" + "i" * 15
+UPSAMPLE = "pp" + "i" * 8
 
 DROPINS = {}
 for _n in ("float32", "float64"):
@@ -66,6 +67,8 @@ for _n in ("float32", "float64"):
                    ("dilated_cuboid_backward_input_im2col", DCONV3D_BWD),
                    ("dilated_cuboid_backward_kernel_im2col", DCONV3D_BWD)):
         DROPINS[f"owl_amd_{_n}_ndarray_conv_{_v}"] = _k
+    DROPINS[f"owl_amd_{_n}_ndarray_upsampling_spatial_backward"] = UPSAMPLE
+    DROPINS[f"owl_amd_{_n}_ndarray_upsampling_spatial"] = UPSAMPLE
     for _v, _k in (("maxpool_spatial", POOL2D_FWD), ("avgpool_spatial", POOL2D_FWD),
                    ("maxpool_spatial_backward", POOL2D_BWD), ("avgpool_spatial_backward", POOL2D_BWD),
                    ("maxpool_spatial_arg", POOL2D_ARG), ("maxpool_cuboid", POOL3D_FWD),
@@ -96,6 +99,8 @@ def lib():
             "owl_amd_conv_fwd": ([cd, i32, i32, vp, vp, vp, vp], i32),
             "owl_amd_conv_fwd_bias_act": ([cd, i32, i32, vp, vp, vp, i32, vp, vp, vp], i32),
             "owl_amd_bias_act_bwd": ([i32, i32, i64, i64, vp, vp, vp, vp, vp], i32),
+            "owl_amd_upsampling2d_fwd": ([i32] + [i64] * 8 + [vp, vp, vp], i32),
+            "owl_amd_upsampling2d_bwd": ([i32] + [i64] * 8 + [vp, vp, vp], i32),
             "owl_amd_conv_bwd_input": ([cd, i32, i32, vp, vp, vp, vp], i32),
             "owl_amd_conv_bwd_kernel_workspace": ([cd, i32], ctypes.c_size_t),
             "owl_amd_conv_bwd_kernel": ([cd, i32, i32, vp, vp, vp, vp, ctypes.c_size_t, vp], i32),

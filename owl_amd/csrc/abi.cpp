@@ -648,6 +648,48 @@ extern "C" int owl_amd_bias_act_bwd(int dt, int act, int64_t npix, int64_t C, co
   return OWL_AMD_OK;
 }
 
+// ------------------------------------------------------------------ upsampling
+static int upsample(bool backward, int dt, int64_t B, int64_t H, int64_t W, int64_t C, int64_t Ho, int64_t Wo,
+                    int64_t sh, int64_t sw, const void *src, void *dst, void *stream) {
+  if (B < 0 || H < 0 || W < 0 || C < 0 || Ho < 0 || Wo < 0 || sh < 1 || sw < 1)
+    ARG_FAIL("bad upsampling sizes");
+  if (H > INT32_MAX || W > INT32_MAX || C > INT32_MAX || Ho > INT32_MAX || Wo > INT32_MAX)
+    ARG_FAIL("upsampling dimension above 2^31");
+  if (dt != OWL_AMD_F32 && dt != OWL_AMD_F64 && dt != OWL_AMD_BF16 && dt != OWL_AMD_F16) {
+    owl_amd_set_error("unsupported dtype %d", dt);
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n_dst = backward ? B * H * W * C : B * Ho * Wo * C;
+  if (n_dst == 0) return OWL_AMD_OK;
+  if (!dst || (!src && (backward ? B * Ho * Wo * C : B * H * W * C) > 0)) ARG_FAIL("null pointer");
+  if (H == 0 || W == 0) ARG_FAIL("empty input image with a non-empty output");
+  if (backward && (Ho == 0 || Wo == 0)) {
+    OWL_CHECK_HIP(hipMemsetAsync(dst, 0, (size_t)n_dst * dtype_size(dt), st));
+    return OWL_AMD_OK;
+  }
+  KTimer kt(OWL_AMD_KT_AUX, backward ? "upsample_bwd_kernel" : "upsample_fwd_kernel", st);
+  const int h = (int)H, w = (int)W, c = (int)C, ho = (int)Ho, wo = (int)Wo, s0 = (int)sh, s1 = (int)sw;
+  if (dt == OWL_AMD_F32) run_upsample<float>(backward, (const float *)src, (float *)dst, B, h, w, c, ho, wo, s0, s1, st);
+  else if (dt == OWL_AMD_F64)
+    run_upsample<double>(backward, (const double *)src, (double *)dst, B, h, w, c, ho, wo, s0, s1, st);
+  else if (dt == OWL_AMD_BF16)
+    run_upsample<bf16_t>(backward, (const bf16_t *)src, (bf16_t *)dst, B, h, w, c, ho, wo, s0, s1, st);
+  else run_upsample<f16_t>(backward, (const f16_t *)src, (f16_t *)dst, B, h, w, c, ho, wo, s0, s1, st);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+extern "C" int owl_amd_upsampling2d_fwd(int dt, int64_t B, int64_t H, int64_t W, int64_t C, int64_t Ho, int64_t Wo,
+                                        int64_t sh, int64_t sw, const void *x, void *y, void *stream) {
+  return upsample(false, dt, B, H, W, C, Ho, Wo, sh, sw, x, y, stream);
+}
+
+extern "C" int owl_amd_upsampling2d_bwd(int dt, int64_t B, int64_t H, int64_t W, int64_t C, int64_t Ho, int64_t Wo,
+                                        int64_t sh, int64_t sw, const void *dy, void *dx, void *stream) {
+  return upsample(true, dt, B, H, W, C, Ho, Wo, sh, sw, dy, dx, stream);
+}
+
 // ------------------------------------------------------------------ device API: FP8
 extern "C" int owl_amd_fp8_quantize(int src_t, const void *src, int64_t n, void *dst_q, float *scale,
                                     void *stream) {
@@ -1169,6 +1211,38 @@ int pool3d_bwd_dropin(bool is_max, T *input, T *output_back, T *input_back, int6
 
 DEFINE_DROPINS(float32, float)
 DEFINE_DROPINS(float64, double)
+
+// Upsampling drop-ins, Owl's stub argument order (owl_ndarray_upsampling_impl.h:9-14):
+// cols = H (scaled by col_scale), rows = W (scaled by row_scale).
+namespace {
+template <typename T>
+int upsample_dropin(bool backward, T *input, T *output, int64_t B, int64_t H, int64_t W, int64_t C, int64_t Ho,
+                    int64_t Wo, int64_t sh, int64_t sw) {
+  if (B < 0 || H < 0 || W < 0 || C < 0 || Ho < 0 || Wo < 0 || sh < 1 || sw < 1) ARG_FAIL("bad upsampling sizes");
+  const size_t nx = (size_t)(B * H * W * C) * sizeof(T), ny = (size_t)(B * Ho * Wo * C) * sizeof(T);
+  if ((backward ? nx : ny) == 0) return OWL_AMD_OK;
+  if (!input || !output) ARG_FAIL("null pointer");
+  if (backward)
+    return run_dropin({{output, ny}, {nullptr, nx}}, 1, input, nx, [&](void **dv, hipStream_t st) {
+      return owl_amd_upsampling2d_bwd(dt<T>(), B, H, W, C, Ho, Wo, sh, sw, dv[0], dv[1], st);
+    });
+  return run_dropin({{input, nx}, {nullptr, ny}}, 1, output, ny, [&](void **dv, hipStream_t st) {
+    return owl_amd_upsampling2d_fwd(dt<T>(), B, H, W, C, Ho, Wo, sh, sw, dv[0], dv[1], st);
+  });
+}
+}  // namespace
+
+#define DEFINE_UPSAMPLE_DROPINS(N, T)                                                             \
+  extern "C" int owl_amd_##N##_ndarray_upsampling_spatial_backward(OWL_AMD_UPSAMPLE_ARGS(T)) {    \
+    return upsample_dropin<T>(true, input, output, batches, input_cols, input_rows, in_channel,   \
+                              output_cols, output_rows, col_scale, row_scale);                    \
+  }                                                                                               \
+  extern "C" int owl_amd_##N##_ndarray_upsampling_spatial(OWL_AMD_UPSAMPLE_ARGS(T)) {             \
+    return upsample_dropin<T>(false, input, output, batches, input_cols, input_rows, in_channel,  \
+                              output_cols, output_rows, col_scale, row_scale);                    \
+  }
+DEFINE_UPSAMPLE_DROPINS(float32, float)
+DEFINE_UPSAMPLE_DROPINS(float64, double)
 
 // ------------------------------------------------------------------ ragged groups
 // A plan for running many forward convolutions (different shapes) as one

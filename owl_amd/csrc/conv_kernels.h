@@ -75,6 +75,12 @@ template <typename T, typename A>
 void run_bias_act_bwd(int act, int64_t npix, int C, const T *pre, const T *dy, T *dz, A *partial, A *db,
                       hipStream_t st);
 
+// 2-D nearest-neighbour upsampling (resample.hip): backward = false maps
+// src = x [B,H,W,C] -> dst = y [B,Ho,Wo,C]; true maps dy -> dx.
+template <typename T>
+void run_upsample(bool backward, const T *src, T *dst, int64_t B, int H, int W, int C, int Ho, int Wo, int sh,
+                  int sw, hipStream_t st);
+
 // Kernel timing (owl_amd_kernel_timing): a KTimer brackets one launch with
 // HIP events on its stream while timing is enabled, and does nothing else.
 void ktime_begin(int kclass, const char *name, hipStream_t st, void **token);

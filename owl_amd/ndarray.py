@@ -597,3 +597,169 @@ def bias_act_backward(pre, output_grad, act="relu", bias_grad=True):
         _tdt(dy), code, npix, C, _ptr(pre) if code else None, _ptr(dy), _ptr(dz),
         _ptr(db) if db is not None else None, _stream()))
     return dz, db
+
+
+# ----------------------------------------------------------------------------- upsampling
+def _check_upsampling(name, x, size):
+    _verify(len(x.shape) == 4 and len(size) == 2,
+            f"{name}: input dimension = {len(x.shape)} (should be 4); "
+            f"size dimension = {len(size)} (should be 2).")
+
+
+def upsampling2d(input, size):
+    """owl_dense_ndarray_generic.ml:8223-8234: ``repeat input [|1; s0; s1; 1|]``
+    (nearest neighbour, s0 along H = Owl's cols, s1 along W = rows)."""
+    _check_upsampling("upsampling2d", input, size)
+    x = _contig(input)
+    B, H, W, C = x.shape
+    s0, s1 = int(size[0]), int(size[1])
+    y = _empty_like_kind(x, [B, H * s0, W * s1, C])
+    if _is_torch(x):
+        _lib.check(_lib.lib().owl_amd_upsampling2d_fwd(_tdt(x), B, H, W, C, H * s0, W * s1, s0, s1, _ptr(x),
+                                                        _ptr(y), _stream()))
+        return y
+    _lib.call(f"owl_amd_{_np_prefix(x)}_ndarray_upsampling_spatial", _ptr(x), _ptr(y), B, H, W, C, H * s0,
+              W * s1, s0, s1)
+    return y
+
+
+def upsampling2d_backward(input, size, output_grad):
+    """owl_dense_ndarray_generic.ml:8249-8290 and the stub
+    owl_ndarray_upsampling_impl.h:9-60: the input's gradient, each cell the
+    sum of the output gradient over its s0 x s1 block (Owl's order)."""
+    _check_upsampling("upsampling2d_backward", input, size)
+    _same_kind(input, output_grad)
+    B, H, W, C = input.shape
+    s0, s1 = int(size[0]), int(size[1])
+    dy = _contig(output_grad)
+    _verify(dy.shape[1] == H * s0 and dy.shape[2] == W * s1,
+            f"upsampling2d_backward: output shape is [{','.join(map(str, dy.shape))}]; scaled output cols is "
+            f"{H * s0}, should be equal to the 2nd dimension of output shape; scaled output rows is {W * s1}, "
+            f"should be equal to the 3rd dimension of output shape.")
+    dx = _empty_like_kind(dy, [B, H, W, C])
+    if _is_torch(dy):
+        _lib.check(_lib.lib().owl_amd_upsampling2d_bwd(_tdt(dy), B, H, W, C, H * s0, W * s1, s0, s1, _ptr(dy),
+                                                        _ptr(dx), _stream()))
+        return dx
+    _lib.call(f"owl_amd_{_np_prefix(dy)}_ndarray_upsampling_spatial_backward", _ptr(dx), _ptr(dy), B, H, W, C,
+              H * s0, W * s1, s0, s1)
+    return dx
+
+
+# ----------------------------------------------------------------------------- 1-D convolution and pooling
+# Owl runs every 1-D op as the 2-D one on [B; 1; W; C] with kernel [1; kw; Ci; Co]
+# and stride [1; s] (owl_dense_ndarray_generic.ml:3402-3787 conv, 4967-5363
+# dilated, 6483-6890 transpose, 7027-7230 / 7840-8224 pooling); so does this.
+def _rs(a, shape):
+    return a.reshape(shape)
+
+
+def _check_1d(name, x, k=None, stride=None, dy=None, kernel_len=None):
+    _verify(len(x.shape) == 3, f"{name}: input dimension = {len(x.shape)} (should be 3)")
+    if k is not None:
+        _verify(len(k.shape) == 3, f"{name}: kernel dimension = {len(k.shape)} (should be 3)")
+        _verify(x.shape[2] == k.shape[1],
+                f"{name}: the 3rd dimension of input shape should be equal to the 2nd dimension of kernel shape")
+    if kernel_len is not None:
+        _verify(kernel_len == 1, f"{name}: kernel dimension = {kernel_len} (should be 1)")
+    if stride is not None:
+        _verify(len(stride) == 1, f"{name}: stride dimension = {len(stride)} (should be 1)")
+    if dy is not None:
+        _verify(len(dy.shape) == 3, f"{name}: output' dimension = {len(dy.shape)} (should be 3)")
+
+
+def _x2(x):
+    return _rs(x, [x.shape[0], 1, x.shape[1], x.shape[2]])
+
+
+def _k2(k):
+    return _rs(k, [1, *k.shape])
+
+
+def _y1(y):
+    return _rs(y, [y.shape[0], y.shape[2], y.shape[3]])
+
+
+def conv1d(input, kernel, stride, padding=SAME):
+    """owl_dense_ndarray_generic.ml:3402-3445."""
+    _check_1d("conv1d", input, kernel, stride)
+    return _y1(conv2d(_x2(input), _k2(kernel), [1, stride[0]], padding))
+
+
+def conv1d_backward_input(input, kernel, stride, output_grad):
+    """owl_dense_ndarray_generic.ml:3489-3561."""
+    _check_1d("conv1d_backward_input", input, kernel, stride, output_grad)
+    return _rs(conv2d_backward_input(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad)), input.shape)
+
+
+def conv1d_backward_kernel(input, kernel, stride, output_grad):
+    """owl_dense_ndarray_generic.ml:3639-3713."""
+    _check_1d("conv1d_backward_kernel", input, kernel, stride, output_grad)
+    return _rs(conv2d_backward_kernel(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad)), kernel.shape)
+
+
+def dilated_conv1d(input, kernel, stride, rate, padding=SAME):
+    """owl_dense_ndarray_generic.ml:4967-5010.  Owl hands ``rate`` to the 2-D
+    function unchanged, so it must have two entries (rows, cols) as there."""
+    _check_1d("dilated_conv1d", input, kernel, stride)
+    return _y1(dilated_conv2d(_x2(input), _k2(kernel), [1, stride[0]], rate, padding))
+
+
+def dilated_conv1d_backward_input(input, kernel, stride, rate, output_grad):
+    """owl_dense_ndarray_generic.ml:5054-5130 (``rate`` as in dilated_conv1d)."""
+    _check_1d("dilated_conv1d_backward_input", input, kernel, stride, output_grad)
+    return _rs(dilated_conv2d_backward_input(_x2(input), _k2(kernel), [1, stride[0]], rate, _x2(output_grad)),
+               input.shape)
+
+
+def dilated_conv1d_backward_kernel(input, kernel, stride, rate, output_grad):
+    """owl_dense_ndarray_generic.ml:5210-5286 (``rate`` as in dilated_conv1d)."""
+    _check_1d("dilated_conv1d_backward_kernel", input, kernel, stride, output_grad)
+    return _rs(dilated_conv2d_backward_kernel(_x2(input), _k2(kernel), [1, stride[0]], rate, _x2(output_grad)),
+               kernel.shape)
+
+
+def transpose_conv1d(input, kernel, stride, padding=SAME):
+    """owl_dense_ndarray_generic.ml:6483-6526."""
+    _check_1d("transpose_conv1d", input, kernel, stride)
+    return _y1(transpose_conv2d(_x2(input), _k2(kernel), [1, stride[0]], padding))
+
+
+def transpose_conv1d_backward_input(input, kernel, stride, output_grad):
+    """owl_dense_ndarray_generic.ml:6570-6646."""
+    _check_1d("transpose_conv1d_backward_input", input, kernel, stride, output_grad)
+    return _rs(transpose_conv2d_backward_input(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad)),
+               input.shape)
+
+
+def transpose_conv1d_backward_kernel(input, kernel, stride, output_grad):
+    """owl_dense_ndarray_generic.ml:6732-6814."""
+    _check_1d("transpose_conv1d_backward_kernel", input, kernel, stride, output_grad)
+    return _rs(transpose_conv2d_backward_kernel(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad)),
+               kernel.shape)
+
+
+def max_pool1d(input, kernel, stride, padding=SAME):
+    """owl_dense_ndarray_generic.ml:7027-7053."""
+    _check_1d("max_pool1d", input, stride=stride, kernel_len=len(kernel))
+    return _y1(max_pool2d(_x2(input), [1, kernel[0]], [1, stride[0]], padding))
+
+
+def avg_pool1d(input, kernel, stride, padding=SAME):
+    """owl_dense_ndarray_generic.ml:7197-7223."""
+    _check_1d("avg_pool1d", input, stride=stride, kernel_len=len(kernel))
+    return _y1(avg_pool2d(_x2(input), [1, kernel[0]], [1, stride[0]], padding))
+
+
+def max_pool1d_backward(padding, input, kernel, stride, output_grad):
+    """owl_dense_ndarray_generic.ml:7840-7871."""
+    _check_1d("max_pool1d_backward", input, stride=stride, kernel_len=len(kernel))
+    return _rs(max_pool2d_backward(padding, _x2(input), [1, kernel[0]], [1, stride[0]], _x2(output_grad)),
+               input.shape)
+
+
+def avg_pool1d_backward(padding, input, kernel, stride, output_grad):
+    """owl_dense_ndarray_generic.ml:8158-8189."""
+    _check_1d("avg_pool1d_backward", input, stride=stride, kernel_len=len(kernel))
+    return _rs(avg_pool2d_backward(padding, _x2(input), [1, kernel[0]], [1, stride[0]], _x2(output_grad)),
+               input.shape)

@@ -10,7 +10,8 @@ values the test asserts; how the helper builds its inputs (``sequential``,
 Sources (paths relative to the reference root):
   test/unit_conv2d_generic.ml, unit_conv3d_generic.ml, unit_pool2d_generic.ml,
   unit_pool3d_generic.ml, unit_dilated_conv2d.ml, unit_dilated_conv3d.ml,
-  unit_transpose_conv2d.ml, unit_transpose_conv3d.ml
+  unit_transpose_conv2d.ml, unit_transpose_conv3d.ml; and
+  unit_upsampling_generic.ml into owl_upsampling_vectors.json
 
 Usage:  python tests/golden/extract_owl_vectors.py [/root/reference]
 """
@@ -112,16 +113,53 @@ def extract_file(path, fname):
     return out
 
 
+def extract_upsampling(path, fname):
+    """test/unit_upsampling_generic.ml: forward tests (``test_upsampling2d_forward
+    shape size``, compared with ``repeat``) and backward golden vectors
+    (``verify_value shape size expected``, the backward of an all-ones input
+    with a ``sequential`` output gradient)."""
+    src = strip_comments(open(path).read())
+    registered = set(re.findall(r"\((To_test_\w+\.fun\d+)\s*\(\)\)", src))
+    out = []
+    for m in re.finditer(r"let\s+(fun\d+)\s*\(\)\s*=(.*?)(?=let\s+fun\d+\s*\(\)|\bend\b)", src, flags=re.S):
+        name, body = m.group(1), m.group(2)
+        rec = {"file": "test/" + fname, "line": line_of(src, m.start()), "module": "To_test_upsampling2d",
+               "fun": name, "registered": f"To_test_upsampling2d.{name}" in registered}
+        f = re.search(r"test_upsampling2d_forward\s+" + ARR + r"\s*" + ARR, body, flags=re.S)
+        if f:
+            rec["helper"] = "test_upsampling2d_forward"
+            rec["input_shape"] = [int(v) for v in parse_array(f.group(1))]
+            rec["size"] = [int(v) for v in parse_array(f.group(2))]
+        else:
+            e = re.search(r"let\s+expected\s*=\s*" + ARR, body, flags=re.S)
+            v = re.search(r"verify_value\s+" + ARR + r"\s*" + ARR + r"\s*expected", body, flags=re.S)
+            if not (e and v):
+                raise RuntimeError(f"{fname}:{name}: unrecognised test body")
+            rec["helper"] = "upsampling2d_backward_verify_value"
+            rec["expected"] = parse_array(e.group(1))
+            rec["input_shape"] = [int(x) for x in parse_array(v.group(1))]
+            rec["size"] = [int(x) for x in parse_array(v.group(2))]
+        out.append(rec)
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     recs = []
     for f in FILES:
         recs.extend(extract_file(os.path.join(ref, "test", f), f))
-    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "owl_vectors.json")
+    here = os.path.dirname(os.path.abspath(__file__))
+    dst = os.path.join(here, "owl_vectors.json")
     with open(dst, "w") as fh:
         json.dump({"source": "owlbarn/owl test/*.ml (golden vectors, transcribed by extract_owl_vectors.py)",
                    "cases": recs}, fh, indent=1)
     print(f"wrote {len(recs)} cases to {dst}")
+    up = extract_upsampling(os.path.join(ref, "test", "unit_upsampling_generic.ml"), "unit_upsampling_generic.ml")
+    dst = os.path.join(here, "owl_upsampling_vectors.json")
+    with open(dst, "w") as fh:
+        json.dump({"source": "owlbarn/owl test/unit_upsampling_generic.ml (transcribed by extract_owl_vectors.py)",
+                   "cases": up}, fh, indent=1)
+    print(f"wrote {len(up)} cases to {dst}")
 
 
 if __name__ == "__main__":
