@@ -75,7 +75,7 @@ void owl_amd_cache_release(void);
 /* Kernel timing (diagnostics; no counterpart in the reference).  While
  * enabled, each convolution call brackets its main compute kernel with a
  * pair of HIP events on the stream it launches on; the auxiliary kernels
- * (weight packing, split-K reduction) are timed as their own class.
+ * (split-K reduction) are timed as their own class.
  * owl_amd_kernel_timing(1) clears the totals and starts recording, (0)
  * stops.  owl_amd_kernel_time waits for the recorded events and returns the
  * total milliseconds and launch count of one class, and the name of the
@@ -84,7 +84,7 @@ enum {
   OWL_AMD_KT_FWD = 0,        /* main kernel of owl_amd_conv_fwd */
   OWL_AMD_KT_BWD_INPUT = 1,  /* main kernel of owl_amd_conv_bwd_input */
   OWL_AMD_KT_BWD_KERNEL = 2, /* main kernel of owl_amd_conv_bwd_kernel */
-  OWL_AMD_KT_AUX = 3,        /* weight packing and split-K reduction */
+  OWL_AMD_KT_AUX = 3,        /* split-K reduction */
   OWL_AMD_KT_COUNT = 4
 };
 int owl_amd_kernel_timing(int enable);

@@ -43,7 +43,10 @@ struct KRec {
   int kclass;
   const char *name;
   hipEvent_t a, b;
+  bool taken;  // handed to a launch
+  KRec *prev;  // enclosing open scope on this thread
 };
+thread_local KRec *t_kt_open = nullptr;
 std::mutex g_kt_mu;
 std::atomic<bool> g_kt_on{false};
 std::vector<KRec *> g_kt_pending;
@@ -81,23 +84,34 @@ void kt_free(KRec *r) {
 }  // namespace
 
 void ktime_begin(int kclass, const char *name, hipStream_t st, void **token) {
+  (void)st;
   *token = nullptr;
   if (!g_kt_on.load(std::memory_order_relaxed)) return;
-  KRec *r = new KRec{kclass, name, kt_event(), kt_event()};
-  if (!r->a || !r->b || hipEventRecord(r->a, st) != hipSuccess) {
-    (void)hipGetLastError();
+  KRec *r = new KRec{kclass, name, kt_event(), kt_event(), false, t_kt_open};
+  if (!r->a || !r->b) {
     std::lock_guard<std::mutex> lk(g_kt_mu);
     kt_free(r);
     return;
   }
+  t_kt_open = r;
   *token = r;
 }
 
+bool ktime_take(hipEvent_t *start, hipEvent_t *stop) {
+  KRec *r = t_kt_open;
+  if (!r || r->taken) return false;
+  r->taken = true;
+  *start = r->a;
+  *stop = r->b;
+  return true;
+}
+
 void ktime_end(void *token, hipStream_t st) {
+  (void)st;
   KRec *r = (KRec *)token;
+  t_kt_open = r->prev;
   std::lock_guard<std::mutex> lk(g_kt_mu);
-  if (hipEventRecord(r->b, st) != hipSuccess) {
-    (void)hipGetLastError();
+  if (!r->taken) {
     kt_free(r);
     return;
   }
@@ -307,22 +321,6 @@ bool fast_path_enabled() {
   return v == 1;
 }
 
-// Scratch for packed forward weights, one per (device, stream) so calls on
-// different streams never share it.  Grows on demand (outside graph capture:
-// a capture must be preceded by an eager call of the same or larger size).
-std::map<std::pair<int, void *>, DevBuf> g_pack;
-int pack_scratch(hipStream_t st, size_t bytes, void **out) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  int dev;
-  int rc = current_device(&dev);
-  if (rc) return rc;
-  DevBuf &b = g_pack[std::make_pair(dev, (void *)st)];
-  rc = ensure_buf(b, bytes, st);
-  if (rc) return rc;
-  *out = b.ptr;
-  return OWL_AMD_OK;
-}
-
 // OWL_AMD_HALO=0 disables the halo kernel (cross-checks, tuning).
 bool halo_enabled() {
   static int v = -1;
@@ -360,18 +358,11 @@ bool try_fast(const ConvParams &p, int in_t, int out_t, bool aligned, hipStream_
     return false;
   // stride-1 2-D problems with 16-bit storage: the halo kernel
   if (halo_enabled() && halo_fits(p, PASS, (int)dtype_size(in_t), (int)dtype_size(out_t))) {
-    // forward: pack the kernel to [tap][co][ci] in a per-(device, stream)
-    // scratch buffer so the halo kernel stages it with plain copies
-    void *wpack = nullptr;
-    if (PASS == 0 && env_int("OWL_AMD_PACK", 1)) {
-      const size_t need = (size_t)p.kh * p.kw * p.kd * p.Ci * p.Co * dtype_size(in_t);
-      if (pack_scratch(st, need, &wpack) != OWL_AMD_OK) wpack = nullptr;
-    }
     bool done = false;
-    if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) done = run_halo<PASS, bf16_t, bf16_t>(p, wpack, st);
-    else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_F32) done = run_halo<PASS, bf16_t, float>(p, wpack, st);
-    else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F16) done = run_halo<PASS, f16_t, f16_t>(p, wpack, st);
-    else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F32) done = run_halo<PASS, f16_t, float>(p, wpack, st);
+    if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) done = run_halo<PASS, bf16_t, bf16_t>(p, st);
+    else if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_F32) done = run_halo<PASS, bf16_t, float>(p, st);
+    else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F16) done = run_halo<PASS, f16_t, f16_t>(p, st);
+    else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F32) done = run_halo<PASS, f16_t, float>(p, st);
     if (done) return true;
   }
   const int shape = fast_shape_for(p.N);

@@ -680,7 +680,7 @@ __global__ __launch_bounds__(NT) void conv_group_fwd_kernel(const GroupEntry *__
 
 template <typename TI, typename TO>
 void run_group_fwd(const GroupEntry *dev_tab, int nprob, int64_t total, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((conv_group_fwd_kernel<TI, TO>), dim3((unsigned)grid), dim3(NT), 0, st, dev_tab, nprob,
+  launch_k((conv_group_fwd_kernel<TI, TO>), dim3((unsigned)grid), dim3(NT), st, dev_tab, nprob,
                      total);
 }
 
@@ -704,9 +704,9 @@ static void launch_igemm(const ConvParams &p, bool aligned, hipStream_t st) {
   const int64_t blocks = tiles * (PASS == 2 ? p.split : 1);
   dim3 grid((unsigned)blocks);
   if (aligned)
-    hipLaunchKernelGGL((conv_igemm_kernel<PASS, TI, TO, true>), grid, dim3(NT), 0, st, p, (int)tiles);
+    launch_k((conv_igemm_kernel<PASS, TI, TO, true>), grid, dim3(NT), st, p, (int)tiles);
   else
-    hipLaunchKernelGGL((conv_igemm_kernel<PASS, TI, TO, false>), grid, dim3(NT), 0, st, p, (int)tiles);
+    launch_k((conv_igemm_kernel<PASS, TI, TO, false>), grid, dim3(NT), st, p, (int)tiles);
 }
 
 template <typename TI> int tile_bm() { return Tile<TI>::BM; }
@@ -771,12 +771,12 @@ template <typename TA, typename TO>
 void run_splitk_reduce(const TA *ws, TO *out, int64_t n, int splits, hipStream_t st) {
   if (n % 4 == 0) {
     const int64_t blocks = (n + 63) / 64;
-    hipLaunchKernelGGL((splitk_reduce_wide_kernel<TA, TO>), dim3((unsigned)blocks), dim3(256), 0, st, ws, out, n,
+    launch_k((splitk_reduce_wide_kernel<TA, TO>), dim3((unsigned)blocks), dim3(256), st, ws, out, n,
                        splits);
     return;
   }
   int64_t blocks = (n + 1023) / 1024;
-  hipLaunchKernelGGL((splitk_reduce_kernel<TA, TO>), dim3((unsigned)blocks), dim3(256), 0, st,
+  launch_k((splitk_reduce_kernel<TA, TO>), dim3((unsigned)blocks), dim3(256), st,
                      ws, out, n, splits);
 }
 

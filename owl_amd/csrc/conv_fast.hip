@@ -648,7 +648,7 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
 template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st) {
   dim3 grid((unsigned)(((p.M + 127) / 128) * ((p.N + 127) / 128) * p.split));
-  hipLaunchKernelGGL((conv_wgrad_kernel<TI, TO>), grid, dim3(256), 0, st, p);
+  launch_k((conv_wgrad_kernel<TI, TO>), grid, dim3(256), st, p);
 }
 template void run_wgrad<bf16_t, bf16_t>(const ConvParams &, hipStream_t);
 template void run_wgrad<bf16_t, float>(const ConvParams &, hipStream_t);
@@ -687,8 +687,12 @@ struct HaloGeom {
   int act;
 };
 
-// KROWS: the kernel operand is laid out [tap][n][k] (k contiguous): the
-// backward-input view of w, or forward weights packed by pack_weights_kernel.
+// KROWS: the kernel operand is laid out [tap][n][k] (k contiguous), the
+// backward-input view of w; its tile is copied to LDS as [n][k] rows and
+// read with 16-byte loads.  Otherwise (forward weights in place,
+// [tap][k][n]) the tile is copied in that natural layout and the fragments
+// are read with the transposing LDS load (ds_read_b64_tr_b16, as in the
+// weight-gradient kernel), so no pass rearranges the weights first.
 template <int PASS, typename TI, typename TO, int FM, int FN, bool KROWS>
 __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI *__restrict__ in,
                                                            const TI *__restrict__ wt, TO *__restrict__ out) {
@@ -703,10 +707,14 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   // one LDS block: the two halo buffers, then the two kernel tiles; after
   // the K loop it holds the output tile for the 16-byte store epilogue
   constexpr int HALO_BYTES = 2 * HBUF * PR * (int)sizeof(TI);
-  constexpr int LDS_BYTES = HALO_BYTES + 2 * BN * LDR * (int)sizeof(TI);
+  // natural-layout kernel tile: BK rows of BN values, padded by 64 bytes so
+  // the four rows one transposing read phase touches cover all 64 banks
+  constexpr int RSB = BN * (int)sizeof(TI) + 64;
+  constexpr int B_BYTES = KROWS ? BN * LDR * (int)sizeof(TI) : BK * RSB;
+  constexpr int LDS_BYTES = HALO_BYTES + 2 * B_BYTES;
   __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES];
   TI *halo = (TI *)smem;
-  TI(*Bs)[BN * LDR] = (TI(*)[BN * LDR])(smem + HALO_BYTES);
+  char *const Bs0 = smem + HALO_BYTES;  // kernel tile buffer i at Bs0 + i * B_BYTES
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int frow = lane & 31, fkg = lane >> 5;
@@ -794,37 +802,32 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   };
 
 
-  // B staging for (tap, c0): PASS 0 reads w[tap][c0 + k][n] (n-contiguous,
-  // transposed through registers); PASS 1 reads w[flipped tap][n][c0 + k].
+  // B staging for (tap, c0): natural layout reads rows w[tap][c0 + k][n0 ..]
+  // (BN / 8 chunks of 16 bytes per row); KROWS reads rows w[tap][n][c0 ..].
+  // Both use unconditional buffer loads (rows or columns past N read as
+  // zeros), which keeps the number of loads in flight static, so the waits
+  // stay precise across the loop.
   constexpr int BELEMS = BK * BN / 256;
-  constexpr int BKV = BELEMS / 4;
-  constexpr int BNV = BELEMS / 8;
-  const int f_ng = tid % (BN / 4), f_kg = tid / (BN / 4);
+  constexpr int BNV = BELEMS / 8;              // 16-byte chunks per thread
+  constexpr int CPRB = BN / 8, RPP = 256 / CPRB;  // natural layout: chunks per row, rows per pass
+  const int nb_ch = tid % CPRB, nb_row = tid / CPRB;
   const int b_kv = (tid % 4) * 8, b_row = tid / 4;
   TI rb0[BELEMS], rb1[BELEMS];  // register ring: B tiles two steps ahead
-  auto load_b = [&](TI(&rb)[BELEMS], int tap, int c0) {
+  auto load_b = [&](TI(&rb)[BELEMS], int tap, int c0) __attribute__((always_inline)) {
     if (!KROWS) {
-      const int n = n0 + f_ng * 4;
-      const TI *src = wt + ((int64_t)tap * g.wCi + c0 + f_kg * BKV) * g.wCo + n;
+      const int n = n0 + nb_ch * 8;
 #pragma unroll
-      for (int kv = 0; kv < BKV; ++kv) {
-        TI tmp[4];
-        if (n < g.N) {
-          __builtin_memcpy(tmp, src + kv * g.wCo, 4 * sizeof(TI));
-        } else {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) tmp[v] = (TI)0.0f;
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) rb[v * BKV + kv] = tmp[v];
+      for (int i = 0; i < BNV; ++i) {
+        const int k = c0 + nb_row + i * RPP;
+        const uint32_t wo = n < g.N ? (uint32_t)((tap * g.wCi + k) * g.wCo + n) * 2u : OOB;
+        const uint4 v = bload(wr, wo);
+        __builtin_memcpy(&rb[i * 8], &v, 16);
       }
     } else {
       // rows of k: [tap][n][k] with N rows of C values per tap
       const int ft = PASS == 1 ? T - 1 - tap : tap;  // backward input flips the taps
 #pragma unroll
       for (int i = 0; i < BNV; ++i) {
-        // unconditional loads (OOB rows read as zeros) keep the number of
-        // loads in flight static, so waits stay precise across the loop
         const int n = n0 + b_row + i * 64;
         const uint32_t wo = n < g.N ? (uint32_t)((ft * g.N + n) * g.C + c0 + b_kv) * 2u : OOB;
         const uint4 v = bload(wr, wo);
@@ -832,15 +835,31 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       }
     }
   };
-  auto store_b = [&](const TI(&rb)[BELEMS], int buf) {
-    TI *dst = Bs[buf];
-    if (!KROWS) {
+  auto store_b = [&](const TI(&rb)[BELEMS], int buf) __attribute__((always_inline)) {
+    char *dst = Bs0 + buf * B_BYTES;
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        __builtin_memcpy(&dst[(f_ng * 4 + v) * LDR + f_kg * BKV], &rb[v * BKV], BKV * sizeof(TI));
+    for (int i = 0; i < BNV; ++i) {
+      const int off = KROWS ? ((b_row + i * 64) * LDR + b_kv) * (int)sizeof(TI) : (nb_row + i * RPP) * RSB + nb_ch * 16;
+      __builtin_memcpy(dst + off, &rb[i * 8], 16);
+    }
+  };
+  // Transposing fragment reads (natural layout): in each group of 16 lanes,
+  // ds_read_b64_tr_b16 gives lane i halfword (i % 4) of the 8 bytes addressed
+  // by lane 4e + i / 4 as its element e.  Lane L points at k row
+  // 16 (L / 32) + (L % 16) / 4 and columns 16 ((L / 16) % 2) + 4 (L % 4) of
+  // the fragment, so it receives column L % 32 at four consecutive k rows;
+  // reads at +8 s2 + 4 hf rows give the k values fkg * 16 + s2 * 8 + 0..7 the
+  // halo fragments hold.
+  const int j16 = lane & 15;
+  const int tr_off = (fkg * 16 + (j16 >> 2)) * RSB + 2 * (16 * ((lane >> 4) & 1) + 4 * (j16 & 3));
+  auto read_b = [&](const char *bbuf, int j, int s2) __attribute__((always_inline)) -> uint4 {
+    if constexpr (!KROWS) {
+      const char *base = bbuf + tr_off + (8 * s2) * RSB + 2 * 32 * j;
+      const v4s16 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s16 *)(base));
+      const v4s16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s16 *)(base + 4 * RSB));
+      return __builtin_bit_cast(uint4, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
     } else {
-#pragma unroll
-      for (int i = 0; i < BNV; ++i) __builtin_memcpy(&dst[(b_row + i * 64) * LDR + b_kv], &rb[i * 8], 16);
+      return ((const uint4 *)(bbuf + ((j * 32 + frow) * LDR + fkg * 16) * (int)sizeof(TI)))[s2];
     }
   };
 
@@ -857,7 +876,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   // last step it fetches tile (0, 0) again, so every step issues the same
   // loads and the waits on them stay exact.
   int ltap = 0, lcb = 0;
-  auto next_load = [&](TI(&rb)[BELEMS]) {
+  auto next_load = [&](TI(&rb)[BELEMS]) __attribute__((always_inline)) {
     const bool in = lcb < cblocks;
     load_b(rb, in ? ltap : 0, in ? lcb * BK : 0);
     if (++ltap == T) {
@@ -871,10 +890,9 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   store_b(rb0, 0);
   next_load(rb0);  // step 2
   __syncthreads();
-  const int fb_off = (frow)*LDR + fkg * 16;
   int cb = 0, tap = 0, ta = 0, tc = 0;  // current step: channel block, tap = (ta, tc)
   // one K-step; rnext holds B(ks + 1) and is refilled with B(ks + 3)
-  auto step = [&](int ks, TI(&rnext)[BELEMS]) {
+  auto step = [&](int ks, TI(&rnext)[BELEMS]) __attribute__((always_inline)) {
     const int bcur = ks & 1;
     // next step's (tap, channel block)
     int ntap = tap + 1, ncb = cb, nta = ta, ntc = tc + 1;
@@ -892,7 +910,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     // A from the halo at this tap's offset, B from LDS
     const TI *hcur = halo + (dbl ? (cb & 1) * HBUF * PR : 0);
     const int toff = (ta * g.rh * HC + tc * g.rw) * PR;
-    const TI *bs = Bs[bcur] + fb_off;
+    const char *bs = Bs0 + bcur * B_BYTES;
     if constexpr (FM == 4) {
       // 128 accumulators leave no room for both K halves' fragments: read
       // and consume one half at a time
@@ -902,7 +920,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
         for (int f = 0; f < FM; ++f) fa[f] = ((const uint4 *)&hcur[hoff[f] + toff])[s2];
 #pragma unroll
-        for (int j = 0; j < FN; ++j) fb[j] = ((const uint4 *)&bs[j * 32 * LDR])[s2];
+        for (int j = 0; j < FN; ++j) fb[j] = read_b(bs, j, s2);
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -919,9 +937,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
-        const uint4 *src = (const uint4 *)&bs[j * 32 * LDR];
-        fb[j][0] = src[0];
-        fb[j][1] = src[1];
+        fb[j][0] = read_b(bs, j, 0);
+        fb[j][1] = read_b(bs, j, 1);
       }
 #pragma unroll
       for (int s2 = 0; s2 < 2; ++s2)
@@ -930,7 +947,9 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
           for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i][s2], fb[j][s2], acc[i][j]);
     }
-    if (more) store_b(rnext, bcur ^ 1);
+    // unconditional, so every step issues and waits for the same loads;
+    // after the last step this fills a buffer nobody reads again
+    store_b(rnext, bcur ^ 1);
     next_load(rnext);
     if (more && ncb != cb) {
       if (dbl) {
@@ -951,10 +970,15 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     tc = ntc;
     cb = ncb;
   };
-  for (int ks = 0; ks < nk; ks += 2) {
+  // pairs of steps, both unconditional (the register ring alternates), then
+  // an odd last step: no path through the loop skips a step's loads, which
+  // keeps the compiler's load waits at the two steps of lookahead
+  int ks = 0;
+  for (; ks + 1 < nk; ks += 2) {
     step(ks, rb1);
-    if (ks + 1 < nk) step(ks + 1, rb0);
+    step(ks + 1, rb0);
   }
+  if (ks < nk) step(ks, rb1);
 
   // ---- epilogue: output pixel (b, i0 + p / Wo, p % Wo), channel n ----
   const uint32_t o_bytes = (uint32_t)((int64_t)g.B * g.Ho * g.Wo * g.N * sizeof(TO));
@@ -1072,31 +1096,11 @@ bool halo_fits(const ConvParams &p, int pass, int in_bytes, int out_bytes) {
   return pass == 0 ? halo_geom<0>(p, in_bytes, out_bytes, g) : halo_geom<1>(p, in_bytes, out_bytes, g);
 }
 
-// Forward weights [tap][ci][co] -> [tap][co][ci] (k-contiguous rows), so the
-// forward halo kernel stages them with plain 16-byte copies.
-template <typename TI>
-__global__ __launch_bounds__(256) void pack_weights_kernel(const TI *__restrict__ w, TI *__restrict__ wp, int T,
-                                                           int Ci, int Co) {
-  __shared__ TI tile[32][33];
-  const int t = blockIdx.z;
-  const int c0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  for (int r = ty; r < 32; r += 8) {
-    const int c = c0 + r, n = n0 + tx;
-    if (c < Ci && n < Co) tile[r][tx] = w[((int64_t)t * Ci + c) * Co + n];
-  }
-  __syncthreads();
-  for (int r = ty; r < 32; r += 8) {
-    const int n = n0 + r, c = c0 + tx;
-    if (c < Ci && n < Co) wp[((int64_t)t * Co + n) * Ci + c] = tile[tx][r];
-  }
-}
-
 // Runs the halo kernel if the problem fits it; returns false otherwise.
-// wpack: scratch for the packed forward weights (T*Ci*Co elements) or null
-// to stage the forward weights through the register transpose instead.
+// The forward reads the weights in place ([tap][ci][co], transposing LDS
+// reads); the backward input reads them as [tap][n = ci][k = co] rows.
 template <int PASS, typename TI, typename TO>
-bool run_halo(const ConvParams &p, void *wpack, hipStream_t st) {
+bool run_halo(const ConvParams &p, hipStream_t st) {
   HaloGeom g;
   if (!halo_geom<PASS>(p, (int)sizeof(TI), (int)sizeof(TO), g)) return false;
   const int FN = g.N <= 64 ? 2 : 4;
@@ -1104,27 +1108,14 @@ bool run_halo(const ConvParams &p, void *wpack, hipStream_t st) {
   const int64_t blocks = (int64_t)g.B * ((g.Ho + g.TR - 1) / g.TR) * ((g.N + BN - 1) / BN);
   const TI *in = (const TI *)p.a;
   const TI *w = (const TI *)p.b;
-  bool krows = PASS == 1;
-  if (PASS == 0 && wpack) {
-    const int T = p.kh * p.kw;
-    dim3 pg((unsigned)((p.Co + 31) / 32), (unsigned)((p.Ci + 31) / 32), (unsigned)T);
-    KTimer kp(OWL_AMD_KT_AUX, "pack_weights_kernel", st);
-    hipLaunchKernelGGL((pack_weights_kernel<TI>), pg, dim3(256), 0, st, w, (TI *)wpack, T, p.Ci, p.Co);
-    w = (const TI *)wpack;
-    krows = true;
-  }
+  constexpr bool KROWS = PASS == 1;
   dim3 grid((unsigned)blocks), blk(256);
   KTimer kt(PASS, "conv_halo_kernel", st);
-  if (FN == 2) {
-    if (krows) hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 4, 2, true>), grid, blk, 0, st, g, in, w, (TO *)p.c);
-    else hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 4, 2, false>), grid, blk, 0, st, g, in, w, (TO *)p.c);
-  } else {
-    if (krows) hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2, 4, true>), grid, blk, 0, st, g, in, w, (TO *)p.c);
-    else hipLaunchKernelGGL((conv_halo_kernel<PASS, TI, TO, 2, 4, false>), grid, blk, 0, st, g, in, w, (TO *)p.c);
-  }
+  if (FN == 2) launch_k((conv_halo_kernel<PASS, TI, TO, 4, 2, KROWS>), grid, blk, st, g, in, w, (TO *)p.c);
+  else launch_k((conv_halo_kernel<PASS, TI, TO, 2, 4, KROWS>), grid, blk, st, g, in, w, (TO *)p.c);
   return true;
 }
-#define OWL_HALO(PASS, TI, TO) template bool run_halo<PASS, TI, TO>(const ConvParams &, void *, hipStream_t);
+#define OWL_HALO(PASS, TI, TO) template bool run_halo<PASS, TI, TO>(const ConvParams &, hipStream_t);
 OWL_HALO(0, bf16_t, bf16_t)
 OWL_HALO(0, bf16_t, float)
 OWL_HALO(1, bf16_t, bf16_t)
@@ -1140,9 +1131,9 @@ static void launch_fast(const ConvParams &p, int pd, hipStream_t st) {
   constexpr int BM = WGM * FM * 32, BN = WGN * FN * 32;
   int64_t tiles = ((p.M + BM - 1) / BM) * (int64_t)((p.N + BN - 1) / BN);
   dim3 g((unsigned)tiles), b(256);
-  if (pd <= 1) hipLaunchKernelGGL((conv_fast_kernel<PASS, TI, TO, FM, FN, WGM, WGN, 1>), g, b, 0, st, p);
-  else if (pd == 2) hipLaunchKernelGGL((conv_fast_kernel<PASS, TI, TO, FM, FN, WGM, WGN, 2>), g, b, 0, st, p);
-  else hipLaunchKernelGGL((conv_fast_kernel<PASS, TI, TO, FM, FN, WGM, WGN, 3>), g, b, 0, st, p);
+  if (pd <= 1) launch_k((conv_fast_kernel<PASS, TI, TO, FM, FN, WGM, WGN, 1>), g, b, st, p);
+  else if (pd == 2) launch_k((conv_fast_kernel<PASS, TI, TO, FM, FN, WGM, WGN, 2>), g, b, st, p);
+  else launch_k((conv_fast_kernel<PASS, TI, TO, FM, FN, WGM, WGN, 3>), g, b, st, p);
 }
 
 // Tile shapes: 0 -> 256 x 64, 1 -> 256 x 128, 2 -> 128 x 256, 3 -> 128 x 128.

@@ -66,10 +66,10 @@ void run_fp8_quantize(const T *src, int64_t n, unsigned char *dst, float *scale,
   if (blocks > 4096) blocks = 4096;
   if (blocks < 1) blocks = 1;
   unsigned *a = (unsigned *)scale;
-  hipLaunchKernelGGL(amax_reset_kernel, dim3(1), dim3(1), 0, st, a);
-  hipLaunchKernelGGL((amax_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, src, n, a);
-  hipLaunchKernelGGL((quant_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, src, n, a, dst);
-  hipLaunchKernelGGL(scale_finalize_kernel, dim3(1), dim3(1), 0, st, a);
+  launch_k(amax_reset_kernel, dim3(1), dim3(1), st, a);
+  launch_k((amax_kernel<T>), dim3((unsigned)blocks), dim3(256), st, src, n, a);
+  launch_k((quant_kernel<T>), dim3((unsigned)blocks), dim3(256), st, src, n, a, dst);
+  launch_k(scale_finalize_kernel, dim3(1), dim3(1), st, a);
 }
 
 // Device-side e4m3 encoder exposed for the conversion test.
@@ -79,7 +79,7 @@ __global__ void e4m3_encode_kernel(const float *src, int64_t n, unsigned char *d
 }
 
 void run_e4m3_encode(const float *src, int64_t n, unsigned char *dst, hipStream_t st) {
-  hipLaunchKernelGGL(e4m3_encode_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, n, dst);
+  launch_k(e4m3_encode_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), st, src, n, dst);
 }
 
 template void run_fp8_quantize<float>(const float *, int64_t, unsigned char *, float *, hipStream_t);

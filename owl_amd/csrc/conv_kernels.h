@@ -1,5 +1,7 @@
 // Kernel-side parameter block and launcher declarations for conv.hip.
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "owl_amd_common.h"
 
 namespace owl_amd {
@@ -57,9 +59,8 @@ void run_fast(const ConvParams &p, int shape, int pd, hipStream_t st);
 int fast_bk(int elem_bytes);
 // conv_fast.hip: stride-1 2-D forward / backward input with an LDS-staged
 // input halo (16-bit storage); returns false when the problem does not fit.
-// wpack: scratch of T*Ci*Co elements for the packed forward weights (or null).
 template <int PASS, typename TI, typename TO>
-bool run_halo(const ConvParams &p, void *wpack, hipStream_t st);
+bool run_halo(const ConvParams &p, hipStream_t st);
 bool halo_fits(const ConvParams &p, int pass, int in_bytes, int out_bytes);
 // conv_fast.hip: backward kernel for 16-bit storage via transposing LDS reads
 // (tile 128 x 128 x 32 pixels; p.split / p.k_per_split as for run_igemm<2>).
@@ -81,10 +82,15 @@ template <typename T>
 void run_upsample(bool backward, const T *src, T *dst, int64_t B, int H, int W, int C, int Ho, int Wo, int sh,
                   int sw, hipStream_t st);
 
-// Kernel timing (owl_amd_kernel_timing): a KTimer brackets one launch with
-// HIP events on its stream while timing is enabled, and does nothing else.
+// Kernel timing (owl_amd_kernel_timing).  While timing is enabled, a KTimer
+// scope holds a start / stop event pair, and the first launch_k() inside the
+// scope (on this thread) hands them to hipExtLaunchKernel, which stamps them
+// with that kernel's own start and end -- the interval a kernel trace
+// reports, without the dispatch latency an event recorded around the launch
+// would add.  Scopes without a launch_k() record nothing.
 void ktime_begin(int kclass, const char *name, hipStream_t st, void **token);
 void ktime_end(void *token, hipStream_t st);
+bool ktime_take(hipEvent_t *start, hipEvent_t *stop);
 struct KTimer {
   void *tok = nullptr;
   hipStream_t st;
@@ -95,5 +101,14 @@ struct KTimer {
   KTimer(const KTimer &) = delete;
   KTimer &operator=(const KTimer &) = delete;
 };
+
+// Launch with no dynamic LDS on stream st; the arguments are converted to
+// the kernel's parameter types.
+template <typename F, typename... Args>
+inline void launch_k(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... args) {
+  hipEvent_t a = nullptr, b = nullptr;
+  ktime_take(&a, &b);
+  hipExtLaunchKernelGGL(kernel, grid, block, 0, st, a, b, 0, args...);
+}
 
 }  // namespace owl_amd

@@ -97,13 +97,13 @@ void run_bias_act_bwd(int act, int64_t npix, int C, const T *pre, const T *dy, T
   while (256 % CW != 0) --CW;  // lanes tile the block exactly
   dim3 grid((unsigned)nchunks, (unsigned)((nv + CW - 1) / CW));
   if (vec_ok)
-    hipLaunchKernelGGL((bias_act_bwd_kernel<T, A, VEC>), grid, dim3(256), 0, st, act, npix, C, CW, PCH, pre, dy, dz,
+    launch_k((bias_act_bwd_kernel<T, A, VEC>), grid, dim3(256), st, act, npix, C, CW, PCH, pre, dy, dz,
                        db ? partial : (A *)nullptr);
   else
-    hipLaunchKernelGGL((bias_act_bwd_kernel<T, A, 1>), grid, dim3(256), 0, st, act, npix, C, CW, PCH, pre, dy, dz,
+    launch_k((bias_act_bwd_kernel<T, A, 1>), grid, dim3(256), st, act, npix, C, CW, PCH, pre, dy, dz,
                        db ? partial : (A *)nullptr);
   if (db)
-    hipLaunchKernelGGL((bias_grad_sum_kernel<A>), dim3((unsigned)((C + 255) / 256)), dim3(256), 0, st, partial,
+    launch_k((bias_grad_sum_kernel<A>), dim3((unsigned)((C + 255) / 256)), dim3(256), st, partial,
                        nchunks, C, db);
 }
 

@@ -89,11 +89,11 @@ void run_upsample(bool backward, const T *src, T *dst, int64_t B, int H, int W, 
   if (blocks < 1) blocks = 1;
   dim3 grid((unsigned)blocks), blk(256);
   if (backward) {
-    if (vec) hipLaunchKernelGGL((upsample_bwd_kernel<T, VEC>), grid, blk, 0, st, src, dst, B, H, W, C, Ho, Wo, sh, sw);
-    else hipLaunchKernelGGL((upsample_bwd_kernel<T, 1>), grid, blk, 0, st, src, dst, B, H, W, C, Ho, Wo, sh, sw);
+    if (vec) launch_k((upsample_bwd_kernel<T, VEC>), grid, blk, st, src, dst, B, H, W, C, Ho, Wo, sh, sw);
+    else launch_k((upsample_bwd_kernel<T, 1>), grid, blk, st, src, dst, B, H, W, C, Ho, Wo, sh, sw);
   } else {
-    if (vec) hipLaunchKernelGGL((upsample_fwd_kernel<T, VEC>), grid, blk, 0, st, src, dst, B, H, W, C, Ho, Wo, sh, sw);
-    else hipLaunchKernelGGL((upsample_fwd_kernel<T, 1>), grid, blk, 0, st, src, dst, B, H, W, C, Ho, Wo, sh, sw);
+    if (vec) launch_k((upsample_fwd_kernel<T, VEC>), grid, blk, st, src, dst, B, H, W, C, Ho, Wo, sh, sw);
+    else launch_k((upsample_fwd_kernel<T, 1>), grid, blk, st, src, dst, B, H, W, C, Ho, Wo, sh, sw);
   }
 }
 
