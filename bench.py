@@ -31,9 +31,9 @@ HBM_PEAK = 8.0e12
 # for a plain copy of known size (x2.0 and x1.0); the runs and their command
 # are in profiles/conv2d_vgg_layer_bf16_hbm_traffic.md
 TRAFFIC = {
-    ("conv_halo_kernel", "fwd"): 13634.1 * 2048 + 50199.2 * 1024,
-    ("conv_halo_kernel", "bwd_input"): 26166.2 * 2048 + 25088.4 * 1024,
-    ("conv_wgrad_kernel", "bwd_kernel"): 187679.5 * 2048 + 29664.0 * 1024,
+    ("conv_halo_kernel", "fwd"): 13813.4 * 2048 + 50176.0 * 1024,
+    ("conv_halo_kernel", "bwd_input"): 28465.2 * 2048 + 32256.0 * 1024,
+    ("conv_wgrad_kernel", "bwd_kernel"): 188176.7 * 2048 + 44064.0 * 1024,
 }
 
 
@@ -75,10 +75,9 @@ def main():
 
     def timed(x, w, dy, out_w, steps, warmup, kernel_times=False):
         """Wall-clock K steps (the measured value), then -- outside that
-        region -- K more steps with per-pass HIP events, and (BF16) K more
-        with the library's HIP events around every kernel it launches.
-        Event records add a release on the stream between kernels, so they
-        are kept out of the region the throughput is taken from."""
+        region -- each pass K times back to back between two HIP events,
+        and (BF16) K more steps with the library's per-kernel timing (the
+        events stamped with each kernel's own start and end)."""
         for _ in range(warmup):
             step(x, w, dy, out_w)
         torch.cuda.synchronize()
@@ -87,19 +86,19 @@ def main():
             step(x, w, dy, out_w)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        # per-pass events on the stream the library launches on
-        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
-        for i in range(steps):
-            e = ev[i]
-            e[0].record()
-            nd.conv2d(x, w, [1, 1], SAME)
-            e[1].record()
-            nd.conv2d_backward_input(x, w, [1, 1], dy)
-            e[2].record()
-            nd.conv2d_backward_kernel(x, w, [1, 1], dy, out_dtype=out_w)
-            e[3].record()
-        torch.cuda.synchronize()
-        per = [statistics.mean(ev[i][j].elapsed_time(ev[i][j + 1]) for i in range(steps)) for j in range(3)]
+        # per pass: K back-to-back calls between events on the launch stream
+        passes = (lambda: nd.conv2d(x, w, [1, 1], SAME),
+                  lambda: nd.conv2d_backward_input(x, w, [1, 1], dy),
+                  lambda: nd.conv2d_backward_kernel(x, w, [1, 1], dy, out_dtype=out_w))
+        per = []
+        for f in passes:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(steps):
+                f()
+            e1.record()
+            torch.cuda.synchronize()
+            per.append(e0.elapsed_time(e1) / steps)
         kt = None
         if kernel_times:
             _lib.kernel_timing(True)

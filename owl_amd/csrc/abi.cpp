@@ -394,12 +394,18 @@ int tile_m(int t) {
 }
 
 // Split-K plan for the backward-kernel GEMM (M = T*Ci, N = Co, K = pixels):
-// about two workgroups per CU (256 CUs), each split at least 16 K-steps long,
-// so the FP32 partials stay a small fraction of the inputs' bytes.
+// one round of workgroups over the 256 CUs -- three per CU for the 16-bit
+// transposing-LDS kernel (its registers and LDS allow three; 768 ran 3%
+// faster than 512 on the VGG layer, counting the larger reduction), two for
+// the implicit-GEMM kernels -- each split at least 16 K-steps long, so the
+// FP32 partials stay a small fraction of the inputs' bytes.  The plan
+// depends only on the shape and the input type, so the workspace query
+// agrees with the call.
 void plan_splitk(const ConvParams &p, int in_t, int *split, int64_t *kps) {
   const int bm = tile_m(in_t), bk = tile_k(in_t);
   int64_t tiles = ((p.M + bm - 1) / bm) * (int64_t)((p.N + bm - 1) / bm);
-  static const int target = env_int("OWL_AMD_SPLIT_TARGET", 512);  // workgroups to aim for (tuning)
+  static const int env_target = env_int("OWL_AMD_SPLIT_TARGET", 0);  // workgroups to aim for (tuning)
+  const int target = env_target > 0 ? env_target : (in_t == OWL_AMD_BF16 || in_t == OWL_AMD_F16 ? 768 : 512);
   int64_t want = (target + tiles - 1) / tiles;
   int64_t max_split = (p.K + 16 * bk - 1) / (16 * bk);
   static const int cap = env_int("OWL_AMD_MAX_SPLIT", 1 << 30);  // diagnostics / tuning
