@@ -395,17 +395,18 @@ int tile_m(int t) {
 
 // Split-K plan for the backward-kernel GEMM (M = T*Ci, N = Co, K = pixels):
 // one round of workgroups over the 256 CUs -- three per CU for the 16-bit
-// transposing-LDS kernel (its registers and LDS allow three; 768 ran 3%
-// faster than 512 on the VGG layer, counting the larger reduction), two for
+// kernel's 128-row tile, two for its 192-row tile (208-214 VGPRs) and for
 // the implicit-GEMM kernels -- each split at least 16 K-steps long, so the
 // FP32 partials stay a small fraction of the inputs' bytes.  The plan
 // depends only on the shape and the input type, so the workspace query
 // agrees with the call.
 void plan_splitk(const ConvParams &p, int in_t, int *split, int64_t *kps) {
-  const int bm = tile_m(in_t), bk = tile_k(in_t);
-  int64_t tiles = ((p.M + bm - 1) / bm) * (int64_t)((p.N + bm - 1) / bm);
+  const bool w16 = in_t == OWL_AMD_BF16 || in_t == OWL_AMD_F16;
+  const int bm = w16 ? wgrad_tile_m(p.M) : tile_m(in_t), bn = w16 ? 128 : tile_m(in_t), bk = tile_k(in_t);
+  int64_t tiles = ((p.M + bm - 1) / bm) * (int64_t)((p.N + bn - 1) / bn);
   static const int env_target = env_int("OWL_AMD_SPLIT_TARGET", 0);  // workgroups to aim for (tuning)
-  const int target = env_target > 0 ? env_target : (in_t == OWL_AMD_BF16 || in_t == OWL_AMD_F16 ? 768 : 512);
+  // three 128-row wgrad workgroups fit a CU, two of the 192-row ones
+  const int target = env_target > 0 ? env_target : (w16 && bm == 128 ? 768 : 512);
   int64_t want = (target + tiles - 1) / tiles;
   int64_t max_split = (p.K + 16 * bk - 1) / (16 * bk);
   static const int cap = env_int("OWL_AMD_MAX_SPLIT", 1 << 30);  // diagnostics / tuning

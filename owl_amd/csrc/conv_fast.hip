@@ -18,6 +18,8 @@
 //     current step's MFMAs run.
 //   * Workgroups are renumbered so that consecutive M tiles (which share
 //     input halo rows) run on the same XCD and hit its L2.
+#include <cstdlib>
+
 #include "conv_kernels.h"
 #include "owl_amd_common.h"
 
@@ -405,12 +407,17 @@ template <> __device__ __forceinline__ f32x16 mfma16<f16_t>(const v4s16 &a0, con
                                                 0);
 }
 
-template <typename TI, typename TO>
+// FMW: 32-row fragments per wave along M; the tile is BM = 64 FMW rows
+// (2 x 2 waves of 32 FMW x 64).
+template <typename TI, typename TO, int FMW>
 __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   // BK 64 (80 KB of LDS) ran slower than BK 32: one workgroup per CU
-  constexpr int BM = 128, BN = 128, BK = 32, KP = BK / 32;  // KP pixels per loader thread
-  constexpr int RS = BM * 2 + 64;  // LDS row stride in bytes (128 16-bit values + 64 bytes pad)
-  __shared__ __attribute__((aligned(16))) char lds[2][2][BK * RS];  // [stage][A|B], 40 KB
+  constexpr int BM = 64 * FMW, BN = 128, BK = 32, KP = BK / 32;  // KP pixels per loader thread
+  constexpr int QA = BM / 64;      // A chunks per loader thread and pixel
+  constexpr int RSA = BM * 2 + 64;  // LDS row strides in bytes (16-bit values + 64 bytes pad)
+  constexpr int RSB = BN * 2 + 64;
+  constexpr int STAGE = BK * (RSA + RSB);  // A rows, then B rows
+  __shared__ __attribute__((aligned(16))) char lds[2][STAGE];  // 40 KB (FMW 2), 48 KB (FMW 3)
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -434,18 +441,18 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   const __amdgpu_buffer_rsrc_t xr = make_rsrc(p.a, x_bytes);
   const __amdgpu_buffer_rsrc_t gr = make_rsrc(p.b, g_bytes);
 
-  // ---- loader assignment: per K-step the A tile (BK pixels x 128 rows m)
-  // and the B tile (BK pixels x 128 columns n) are 16 16-byte chunks per
-  // pixel.  Thread t loads pixels kp = t / 8 and kp + 32 of the step, and of
-  // each the chunks c8 = t % 8 and c8 + 8 of both tiles.
+  // ---- loader assignment: per K-step the A tile (BK pixels x BM rows m)
+  // is 8 QA 16-byte chunks per pixel and the B tile (BK pixels x 128
+  // columns n) 16.  Thread t loads pixel kp = t / 8 of the step, and of it
+  // the A chunks c8 + 8 q (q < QA) and the B chunks c8, c8 + 8.
   const int c8 = tid & 7, kp = tid >> 3;
   // A chunk q covers rows m0 + 8 (c8 + 8 q) .. + 7, all in one tap (Ci % 8 == 0):
   // tap window offset (tdh, tdw, tdd) and the chunk's element offset from
   // the pixel's window origin
-  int tdh[2], tdw[2], tdd[2], toff[2];
-  bool aok[2];
+  int tdh[QA], tdw[QA], tdd[QA], toff[QA];
+  bool aok[QA];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < QA; ++q) {
     const int am = m0 + 8 * (c8 + 8 * q);
     aok[q] = am < M;
     const int tap = aok[q] ? am / p.Ci : 0;
@@ -495,7 +502,7 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   const int gstep = BK * p.Co;
 
   // register ring: loads run two K-steps ahead of the LDS stage they fill
-  typedef uint4 Stage[KP][4];  // per pixel: A chunks 0, 1, B chunks 0, 1
+  typedef uint4 Stage[KP][QA + 2];  // per pixel: A chunks 0 .. QA-1, then B chunks 0, 1
   Stage rs0, rs1;
   auto load = [&](Stage &rg) __attribute__((always_inline)) {
 #pragma unroll
@@ -503,7 +510,7 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
       // branch-free (bitwise predicates, one select per load)
       const bool kok = pk[h] < kend32;
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
+      for (int q = 0; q < QA; ++q) {
         const int hh = pih[h] + tdh[q], ww = pjw[h] + tdw[q], dd = pld[h] + tdd[q];
         const bool ok = kok & aok[q] & ((unsigned)hh < (unsigned)p.H) & ((unsigned)ww < (unsigned)p.W) &
                         ((unsigned)dd < (unsigned)p.D);
@@ -511,8 +518,8 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
         rg[h][q] = bload(xr, ok ? off : OOB);
       }
       const uint32_t goff = (uint32_t)gbase[h] * 2u;
-      rg[h][2] = bload(gr, (kok & bok0) ? goff : OOB);
-      rg[h][3] = bload(gr, (kok & bok1) ? goff + 128u : OOB);
+      rg[h][QA] = bload(gr, (kok & bok0) ? goff : OOB);
+      rg[h][QA + 1] = bload(gr, (kok & bok1) ? goff + 128u : OOB);
       // advance to the pixel BK further on
       pk[h] += BK;
       gbase[h] += gstep;
@@ -539,18 +546,18 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   };
   auto store = [&](const Stage &rg, int st) __attribute__((always_inline)) {
 #pragma unroll
-    for (int h = 0; h < KP; ++h)
+    for (int h = 0; h < KP; ++h) {
+      const int row = kp + 32 * h;
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int row = kp + 32 * h;
-        *(uint4 *)(&lds[st][0][row * RS + (c8 + 8 * q) * 16]) = rg[h][q];
-        *(uint4 *)(&lds[st][1][row * RS + (c8 + 8 * q) * 16]) = rg[h][2 + q];
-      }
+      for (int q = 0; q < QA; ++q) *(uint4 *)(&lds[st][row * RSA + (c8 + 8 * q) * 16]) = rg[h][q];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) *(uint4 *)(&lds[st][BK * RSA + row * RSB + (c8 + 8 * q) * 16]) = rg[h][QA + q];
+    }
   };
 
-  f32x16 acc[2][2];
+  f32x16 acc[FMW][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FMW; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -560,32 +567,36 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   // pixel row (lane%16)/4 (+8 for the upper k-group, +4 for the second half),
   // column base (lane/16 %2)*16 + 4*(lane%4) within the fragment's 32 columns
   const int j16 = lane & 15;
-  const int tr_off = ((j16 >> 2) + 8 * (lane >> 5)) * RS + 2 * (16 * ((lane >> 4) & 1) + 4 * (j16 & 3));
+  const int tr_row = (j16 >> 2) + 8 * (lane >> 5), tr_col = 2 * (16 * ((lane >> 4) & 1) + 4 * (j16 & 3));
+  const int tr_a = tr_row * RSA + tr_col, tr_b = BK * RSA + tr_row * RSB + tr_col;
 
   // one K-step: MFMAs from LDS stage ks & 1; rn holds step ks + 1 (stored
   // to the other stage) and is refilled with step ks + 3.  Loads past the
   // split's end read zeros, so every step issues the same loads.
   auto kstep = [&](int ks, Stage &rn) __attribute__((always_inline)) {
     const int cur = ks & 1;
-    const char *As = lds[cur][0], *Bs = lds[cur][1];
+    const char *Ls = lds[cur];
     // every 16-pixel quarter's fragments in their own registers, so all the
     // transposed reads can be in flight before the first MFMA
     constexpr int NQ = BK / 16;
-    v4s16 a[NQ][2][2], b[NQ][2][2];  // [quarter][fragment][half]
+    v4s16 a[NQ][FMW][2], b[NQ][2][2];  // [quarter][fragment][half]
 #pragma unroll
     for (int s = 0; s < NQ; ++s)
 #pragma unroll
-      for (int f = 0; f < 2; ++f)
+      for (int hf = 0; hf < 2; ++hf) {
 #pragma unroll
-        for (int hf = 0; hf < 2; ++hf) {
-          const int off = tr_off + (16 * s + 4 * hf) * RS;
-          a[s][f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s16 *)(As + off + 2 * (wm * 64 + f * 32)));
-          b[s][f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s16 *)(Bs + off + 2 * (wn * 64 + f * 32)));
-        }
+        for (int f = 0; f < FMW; ++f)
+          a[s][f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+              (lds_v4s16 *)(Ls + tr_a + (16 * s + 4 * hf) * RSA + 2 * (wm * 32 * FMW + f * 32)));
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+          b[s][f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+              (lds_v4s16 *)(Ls + tr_b + (16 * s + 4 * hf) * RSB + 2 * (wn * 64 + f * 32)));
+      }
 #pragma unroll
     for (int s = 0; s < NQ; ++s)
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < FMW; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
           acc[i][j] = mfma16<TI>(a[s][i][0], a[s][i][1], b[s][j][0], b[s][j][1], acc[i][j]);
@@ -594,8 +605,8 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
     load(rn);
     // scheduling: the LDS reads as one group ahead of the MFMAs (an
     // MFMA-by-MFMA interleave with the loads and writes ran slower)
-    __builtin_amdgcn_sched_group_barrier(0x100, 8 * NQ, 0);
-    __builtin_amdgcn_sched_group_barrier(0x008, 4 * NQ, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 2 * NQ * (FMW + 2), 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 2 * NQ * FMW, 0);
     __syncthreads();
   };
 
@@ -622,11 +633,11 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   char *cbase = (char *)p.c + (partial ? (int64_t)split_idx * M * N * 4 : 0);
   const __amdgpu_buffer_rsrc_t cr = make_rsrc(cbase, c_bytes);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FMW; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int n = n0 + wn * 64 + j * 32 + frow;
-      const int mrow = m0 + wm * 64 + i * 32 + 4 * fkg;
+      const int mrow = m0 + wm * 32 * FMW + i * 32 + 4 * fkg;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int m = mrow + 8 * (r >> 2) + (r & 3);
@@ -649,10 +660,24 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
     }
 }
 
+// Rows (tap, ci) per weight-gradient tile: 192 when that pads M no more than
+// 128 does (fewer LDS reads per MFMA), else 128.  OWL_AMD_WGRAD_BM = 128 or
+// 192 forces one (tuning).
+int wgrad_tile_m(int64_t M) {
+  static const int forced = [] {
+    const char *e = getenv("OWL_AMD_WGRAD_BM");
+    return e ? atoi(e) : 0;
+  }();
+  if (forced == 128 || forced == 192) return forced;
+  return (M + 191) / 192 * 192 <= (M + 127) / 128 * 128 ? 192 : 128;
+}
+
 template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st) {
-  dim3 grid((unsigned)(((p.M + 127) / 128) * ((p.N + 127) / 128) * p.split));
-  launch_k((conv_wgrad_kernel<TI, TO>), grid, dim3(256), st, p);
+  const int bm = wgrad_tile_m(p.M);
+  dim3 grid((unsigned)(((p.M + bm - 1) / bm) * ((p.N + 127) / 128) * p.split));
+  if (bm == 192) launch_k((conv_wgrad_kernel<TI, TO, 3>), grid, dim3(256), st, p);
+  else launch_k((conv_wgrad_kernel<TI, TO, 2>), grid, dim3(256), st, p);
 }
 template void run_wgrad<bf16_t, bf16_t>(const ConvParams &, hipStream_t);
 template void run_wgrad<bf16_t, float>(const ConvParams &, hipStream_t);

@@ -64,6 +64,7 @@ bool run_halo(const ConvParams &p, hipStream_t st);
 bool halo_fits(const ConvParams &p, int pass, int in_bytes, int out_bytes);
 // conv_fast.hip: backward kernel for 16-bit storage via transposing LDS reads
 // (tile 128 x 128 x 32 pixels; p.split / p.k_per_split as for run_igemm<2>).
+int wgrad_tile_m(int64_t M);
 template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st);
 template <typename TI> int tile_bm();
