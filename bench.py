@@ -26,6 +26,7 @@ B, H, W, CI, CO, K = 64, 56, 56, 64, 128, 3
 FLOP_PASS = 2.0 * B * H * W * CO * K * K * CI  # 29.595 GFLOP
 PEAK = {"bf16": 2.5e15, "f16": 2.5e15, "f32": 157.3e12}
 HBM_PEAK = 8.0e12
+SPINUP_S = 0.25  # untimed work before each timed region, after the W warmup steps
 # HBM bytes per launch of the BF16 step's kernels, from two rocprofv3 --pmc
 # runs (FETCH_SIZE, then WRITE_SIZE) scaled by what the same counters read
 # for a plain copy of known size (x2.0 and x1.0); the runs and their command
@@ -78,9 +79,16 @@ def main():
         region -- each pass K times back to back between two HIP events,
         and (BF16) K more steps with the library's per-kernel timing (the
         events stamped with each kernel's own start and end)."""
+        # W warmup steps, then untimed steps until SPINUP_S has passed: the
+        # step runs ~9% slower until the GPU has been busy for ~10 ms
+        w0 = time.perf_counter()
         for _ in range(warmup):
             step(x, w, dy, out_w)
         torch.cuda.synchronize()
+        while time.perf_counter() - w0 < SPINUP_S:
+            for _ in range(8):
+                step(x, w, dy, out_w)
+            torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             step(x, w, dy, out_w)
@@ -228,6 +236,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "spinup_s": SPINUP_S,
         "ms_per_step": round(wall_b / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
