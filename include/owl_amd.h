@@ -183,7 +183,9 @@ int owl_amd_conv_group_info(const owl_amd_conv_group *g, int *tile_edges,
                             int64_t *total_tiles);
 void owl_amd_conv_group_destroy(owl_amd_conv_group *g);
 
-/* Pooling (owl_ndarray_pool_impl.h): results bit-identical to Owl's C. */
+/* Pooling (owl_ndarray_pool_impl.h): F32 / F64 results bit-identical to
+ * Owl's C.  BF16 / F16 (no counterpart in Owl, whose pooling is F32 / F64)
+ * compare and add in FP32 and round once. */
 int owl_amd_pool_fwd(const owl_amd_pool_desc *d, int t, int is_max,
                      const void *x, void *y, void *stream);
 int owl_amd_pool_argmax(const owl_amd_pool_desc *d, int t, const void *x,

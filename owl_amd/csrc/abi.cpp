@@ -748,8 +748,8 @@ static int check_pool(const owl_amd_pool_desc *d, int t) {
   if (d->sh < 1 || d->sw < 1 || d->sd < 1) ARG_FAIL("pool stride must be >= 1");
   if (!fits_int(d->B * d->H * d->W * d->D * d->C) || !fits_int(d->B * d->Ho * d->Wo * d->Do * d->C))
     ARG_FAIL("pool tensor too large for 32-bit indexing");
-  if (t != OWL_AMD_F32 && t != OWL_AMD_F64) {
-    owl_amd_set_error("pooling supports F32 and F64");
+  if (t != OWL_AMD_F32 && t != OWL_AMD_F64 && t != OWL_AMD_BF16 && t != OWL_AMD_F16) {
+    owl_amd_set_error("pooling supports F32, F64, BF16 and F16");
     return OWL_AMD_ERR_UNSUPPORTED;
   }
   return OWL_AMD_OK;
@@ -772,8 +772,12 @@ extern "C" int owl_amd_pool_fwd(const owl_amd_pool_desc *d, int t, int is_max, c
   PoolParams p = make_pool(d);
   if ((int64_t)p.B * p.Ho * p.Wo * p.Do * p.C == 0) return OWL_AMD_OK;
   if (!x || !y) ARG_FAIL("null pointer");
-  if (t == OWL_AMD_F32) run_pool_fwd<float>(p, is_max ? 1 : 0, (const float *)x, (float *)y, nullptr, (hipStream_t)stream);
-  else run_pool_fwd<double>(p, is_max ? 1 : 0, (const double *)x, (double *)y, nullptr, (hipStream_t)stream);
+  const int mode = is_max ? 1 : 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (t == OWL_AMD_F32) run_pool_fwd<float>(p, mode, (const float *)x, (float *)y, nullptr, st);
+  else if (t == OWL_AMD_F64) run_pool_fwd<double>(p, mode, (const double *)x, (double *)y, nullptr, st);
+  else if (t == OWL_AMD_BF16) run_pool_fwd<bf16_t>(p, mode, (const bf16_t *)x, (bf16_t *)y, nullptr, st);
+  else run_pool_fwd<f16_t>(p, mode, (const f16_t *)x, (f16_t *)y, nullptr, st);
   OWL_CHECK_HIP(hipGetLastError());
   return OWL_AMD_OK;
 }
@@ -785,8 +789,11 @@ extern "C" int owl_amd_pool_argmax(const owl_amd_pool_desc *d, int t, const void
   PoolParams p = make_pool(d);
   if ((int64_t)p.B * p.Ho * p.Wo * p.Do * p.C == 0) return OWL_AMD_OK;
   if (!x || !y || !argmax) ARG_FAIL("null pointer");
-  if (t == OWL_AMD_F32) run_pool_fwd<float>(p, 2, (const float *)x, (float *)y, argmax, (hipStream_t)stream);
-  else run_pool_fwd<double>(p, 2, (const double *)x, (double *)y, argmax, (hipStream_t)stream);
+  hipStream_t st = (hipStream_t)stream;
+  if (t == OWL_AMD_F32) run_pool_fwd<float>(p, 2, (const float *)x, (float *)y, argmax, st);
+  else if (t == OWL_AMD_F64) run_pool_fwd<double>(p, 2, (const double *)x, (double *)y, argmax, st);
+  else if (t == OWL_AMD_BF16) run_pool_fwd<bf16_t>(p, 2, (const bf16_t *)x, (bf16_t *)y, argmax, st);
+  else run_pool_fwd<f16_t>(p, 2, (const f16_t *)x, (f16_t *)y, argmax, st);
   OWL_CHECK_HIP(hipGetLastError());
   return OWL_AMD_OK;
 }
@@ -817,8 +824,11 @@ extern "C" int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max, c
                axis_has_empty(d->W, d->kw, d->sw, d->pw, d->Wo) ||
                axis_has_empty(d->D, d->kd, d->sd, d->pd, d->Do);
   if (is_max && empty && (int64_t)p.Ho * p.Wo * p.Do * p.B * p.C > 0) run_set_flag(flag, st);
-  if (t == OWL_AMD_F32) run_pool_bwd<float>(p, is_max != 0, (const float *)x, (const float *)dy, (float *)dx, flag, st);
-  else run_pool_bwd<double>(p, is_max != 0, (const double *)x, (const double *)dy, (double *)dx, flag, st);
+  const bool mx = is_max != 0;
+  if (t == OWL_AMD_F32) run_pool_bwd<float>(p, mx, (const float *)x, (const float *)dy, (float *)dx, flag, st);
+  else if (t == OWL_AMD_F64) run_pool_bwd<double>(p, mx, (const double *)x, (const double *)dy, (double *)dx, flag, st);
+  else if (t == OWL_AMD_BF16) run_pool_bwd<bf16_t>(p, mx, (const bf16_t *)x, (const bf16_t *)dy, (bf16_t *)dx, flag, st);
+  else run_pool_bwd<f16_t>(p, mx, (const f16_t *)x, (const f16_t *)dy, (f16_t *)dx, flag, st);
   OWL_CHECK_HIP(hipGetLastError());
   return OWL_AMD_OK;
 }

@@ -8,6 +8,8 @@
 //   * the window is scanned H ("cols") outer, W ("rows"), D ("dpts") inner;
 //   * max: acc = -inf; if (acc < t) acc = t   -> first strict max, NaN never wins;
 //   * avg: acc = 0; acc += t; acc / count_in_bounds  (FP32 stays FP32);
+//   * BF16 / FP16 (device entry points only; Owl's pooling is FP32 / FP64)
+//     compare and add in FP32 and round once;
 //   * backward is a gather: an input cell adds the contributions of the
 //     windows that cover it in ascending (i, j, l) order, the order in which
 //     Owl's scatter loop (:139-198, :377-440) adds them, starting from 0.
@@ -129,9 +131,9 @@ __global__ __launch_bounds__(256) void pool_bwd_kernel(PoolParams p, const T *__
   covering(w, p.sw, p.pw, p.kw, p.Wo, jlo, jhi);
   covering(d, p.sd, p.pd, p.kd, p.Do, llo, lhi);
 
-  T acc[VEC];
+  A acc[VEC];  // the element type for FP32 / FP64 (Owl's arithmetic); FP32 for 16-bit
 #pragma unroll
-  for (int v = 0; v < VEC; ++v) acc[v] = (T)0;
+  for (int v = 0; v < VEC; ++v) acc[v] = (A)0;
 
   for (int i = ilo; i <= ihi; ++i) {
     Win wh = window(i, p.sh, p.ph, p.kh, p.H);
@@ -165,18 +167,21 @@ __global__ __launch_bounds__(256) void pool_bwd_kernel(PoolParams p, const T *__
               }
 #pragma unroll
           for (int v = 0; v < VEC; ++v) {
-            if (bi[v] == my_id + v) acc[v] += g[v];
+            if (bi[v] == my_id + v) acc[v] += (A)g[v];
             if (bi[v] < 0) *orphan_flag = 1;  // Owl sends this gradient to index 0
           }
         } else {
           int cnt = (wh.hi - wh.lo) * (ww.hi - ww.lo) * (wd.hi - wd.lo);
 #pragma unroll
-          for (int v = 0; v < VEC; ++v) acc[v] += g[v] / (T)cnt;
+          for (int v = 0; v < VEC; ++v) acc[v] += (A)g[v] / (A)cnt;
         }
       }
     }
   }
-  __builtin_memcpy(dx + my_id, acc, sizeof(acc));
+  T o[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) o[v] = (T)acc[v];
+  __builtin_memcpy(dx + my_id, o, sizeof(o));
 }
 
 // Max backward when the windows tile the input exactly (window == stride,
@@ -248,13 +253,13 @@ __global__ __launch_bounds__(256) void pool_bwd_fixup_kernel(PoolParams p, const
   typedef typename PoolAcc<T>::type A;
   if (*orphan_flag == 0) return;
   __shared__ int hit[256];
-  __shared__ T val[256];
+  __shared__ A val[256];
   int64_t total = (int64_t)p.B * p.Ho * p.Wo * p.Do * p.C;
-  T acc = (T)0;
+  A acc = (A)0;
   for (int64_t base = 0; base < total; base += 256) {
     int64_t gid = base + threadIdx.x;
     int flag = 0;
-    T g = (T)0;
+    A g = (A)0;
     if (gid < total) {
       int c = (int)(gid % p.C);
       int64_t t = gid / p.C;
@@ -278,7 +283,7 @@ __global__ __launch_bounds__(256) void pool_bwd_fixup_kernel(PoolParams p, const
             }
           }
       flag = bi == 0;
-      g = dy[gid];
+      g = (A)dy[gid];
     }
     hit[threadIdx.x] = flag;
     val[threadIdx.x] = g;
@@ -289,7 +294,7 @@ __global__ __launch_bounds__(256) void pool_bwd_fixup_kernel(PoolParams p, const
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    dx[0] = acc;
+    dx[0] = (T)acc;
     *orphan_flag = 0;
   }
 }
@@ -358,5 +363,9 @@ template void run_pool_fwd<float>(const PoolParams &, int, const float *, float 
 template void run_pool_fwd<double>(const PoolParams &, int, const double *, double *, int64_t *, hipStream_t);
 template void run_pool_bwd<float>(const PoolParams &, bool, const float *, const float *, float *, int *, hipStream_t);
 template void run_pool_bwd<double>(const PoolParams &, bool, const double *, const double *, double *, int *, hipStream_t);
+template void run_pool_fwd<bf16_t>(const PoolParams &, int, const bf16_t *, bf16_t *, int64_t *, hipStream_t);
+template void run_pool_fwd<f16_t>(const PoolParams &, int, const f16_t *, f16_t *, int64_t *, hipStream_t);
+template void run_pool_bwd<bf16_t>(const PoolParams &, bool, const bf16_t *, const bf16_t *, bf16_t *, int *, hipStream_t);
+template void run_pool_bwd<f16_t>(const PoolParams &, bool, const f16_t *, const f16_t *, f16_t *, int *, hipStream_t);
 
 }  // namespace owl_amd

@@ -94,6 +94,33 @@ def _verify(cond, msg):
         raise InvalidArgument(msg)
 
 
+def _overlaps(a, b):
+    if _is_torch(a):
+        a0, b0 = a.data_ptr(), b.data_ptr()
+        return a0 < b0 + b.numel() * b.element_size() and b0 < a0 + a.numel() * a.element_size()
+    return np.may_share_memory(a, b)
+
+
+def _out_buf(out, ref, shape, dtype=None, name="", inputs=()):
+    """The result array: a new one, or the caller's ``out`` (Owl's ``*_ ~out``
+    variants write into a given array) after checking that it is the same kind
+    of array as the input, has the result's shape and dtype, is contiguous and
+    writeable, and does not overlap an input."""
+    if out is None:
+        return _empty_like_kind(ref, shape, dtype=dtype)
+    want = dtype or ref.dtype
+    _verify(_is_torch(out) == _is_torch(ref), f"{name}: out is not the same kind of array as the input")
+    _verify(list(out.shape) == list(shape), f"{name}: out shape {list(out.shape)} should be {list(shape)}")
+    _verify(out.dtype == want, f"{name}: out dtype {out.dtype} should be {want}")
+    if _is_torch(out):
+        _verify(out.is_contiguous() and out.device == ref.device, f"{name}: out must be contiguous, on the input's device")
+    else:
+        _verify(out.flags.c_contiguous and out.flags.writeable, f"{name}: out must be a writeable C-contiguous array")
+    for a in inputs:
+        _verify(not _overlaps(out, a), f"{name}: out overlaps an input")
+    return out
+
+
 def _same_kind(*arrs):
     t = [_is_torch(a) for a in arrs]
     _verify(all(t) or not any(t), "mix of numpy arrays and device tensors")
@@ -136,8 +163,10 @@ def conv_problem(x_shape, k_shape, stride, rate=None, padding=SAME):
     return _Conv(B, sp_in, Ci, ksp, Co, sp_out, stride, rate, pads)
 
 
-def _dev_conv(pass_, g, a, b, out, out_t=None):
-    """Run one pass on device tensors.  pass_: fwd(a=x, b=w), bi(a=w, b=dy), bk(a=x, b=dy)."""
+def _dev_conv(pass_, g, a, b, out, out_t=None, workspace=None):
+    """Run one pass on device tensors.  pass_: fwd(a=x, b=w), bi(a=w, b=dy), bk(a=x, b=dy).
+    ``workspace``: a uint8 device tensor for the backward-kernel pass's
+    split-K partials (``_ws_bytes``), else the library's own per-device one."""
     st = _stream()
     d = g.desc()
     it = _tdt(a)
@@ -148,8 +177,9 @@ def _dev_conv(pass_, g, a, b, out, out_t=None):
     elif pass_ == "bi":
         _lib.check(L.owl_amd_conv_bwd_input(ctypes.byref(d), it, ot, _ptr(a), _ptr(b), _ptr(out), st))
     else:
+        ws, nb = (None, 0) if workspace is None else (_ptr(workspace), workspace.numel())
         _lib.check(L.owl_amd_conv_bwd_kernel(ctypes.byref(d), it, ot, _ptr(a), _ptr(b), _ptr(out),
-                                            None, 0, st))
+                                            ws, nb, st))
     return out
 
 
@@ -196,7 +226,7 @@ def _check_conv_args(name, nd, x, k, stride, rate=None, dy=None):
                 f"{name}: output' shape does not match input / kernel")
 
 
-def _conv_fwd(name, nd, x, k, stride, rate, padding, variant, out_dtype=None):
+def _conv_fwd(name, nd, x, k, stride, rate, padding, variant, out_dtype=None, out=None):
     _same_kind(x, k)
     _check_conv_args(name, nd, x, k, stride, rate)
     x, k = _contig(x), _contig(k)
@@ -207,7 +237,7 @@ def _conv_fwd(name, nd, x, k, stride, rate, padding, variant, out_dtype=None):
     _verify(all(o > 0 for o in sp_out), f"{name}: empty output shape {sp_out}")
     pads = _pads_from_shapes(sp_in, ksp, sp_out, stride, rate) if padding == SAME else [0] * nd
     g = _Conv(B, sp_in, Ci, ksp, Co, sp_out, stride, rate, pads)
-    y = _empty_like_kind(x, [B, *sp_out, Co], dtype=out_dtype)
+    y = _out_buf(out, x, [B, *sp_out, Co], out_dtype, name, (x, k))
     if _is_torch(x):
         return _dev_conv("fwd", g, x, k, y)
     pc = 0 if padding == SAME else 1
@@ -218,96 +248,113 @@ def _conv_fwd(name, nd, x, k, stride, rate, padding, variant, out_dtype=None):
     return y
 
 
-def _conv_bwd(name, nd, which, x, k, stride, rate, dy, variant, out_dtype=None):
+def _ws_bytes(g, t):
+    """Bytes of split-K workspace the backward-kernel pass of geometry g needs for input type t."""
+    return int(_lib.lib().owl_amd_conv_bwd_kernel_workspace(ctypes.byref(g.desc()), _tdt(t)))
+
+
+def _bwd_geom(nd, xs, ks, stride, rate, dys):
+    """Geometry of a (dilated) conv's backward passes from the shapes of x, w, dy."""
+    rate = list(rate) if rate is not None else [1] * nd
+    sp_in, ksp, sp_out = list(xs[1:-1]), list(ks[:-2]), list(dys[1:-1])
+    pads = _pads_from_shapes(sp_in, ksp, sp_out, stride, rate)
+    return _Conv(xs[0], sp_in, xs[-1], ksp, ks[-1], sp_out, stride, rate, pads)
+
+
+def _tbwd_geom(nd, xs, ks, stride, dys):
+    """Geometry of a transposed conv's backward passes: the conv whose input is
+    dy ([.., Co]) and whose output is x ([.., Ci])."""
+    sp_in, ksp, sp_out = list(xs[1:-1]), list(ks[:-2]), list(dys[1:-1])
+    pads = _pads_from_shapes(sp_out, ksp, sp_in, stride, [1] * nd)
+    return _Conv(xs[0], sp_out, ks[-1], ksp, xs[-1], sp_in, stride, [1] * nd, pads)
+
+
+def _conv_bwd(name, nd, which, x, k, stride, rate, dy, variant, out_dtype=None, out=None, workspace=None):
     _same_kind(x, k, dy)
     _check_conv_args(name, nd, x, k, stride, rate, dy)
     x, k, dy = _contig(x), _contig(k), _contig(dy)
-    B, sp_in, Ci = x.shape[0], list(x.shape[1:-1]), x.shape[-1]
-    ksp, Co = list(k.shape[:-2]), k.shape[-1]
-    sp_out = list(dy.shape[1:-1])
     rate = list(rate) if rate is not None else [1] * nd
-    pads = _pads_from_shapes(sp_in, ksp, sp_out, stride, rate)
-    g = _Conv(B, sp_in, Ci, ksp, Co, sp_out, stride, rate, pads)
+    g = _bwd_geom(nd, list(x.shape), list(k.shape), stride, rate, list(dy.shape))
     if which == "input":
-        out = _empty_like_kind(x, list(x.shape), dtype=out_dtype)
+        res = _out_buf(out, x, list(x.shape), out_dtype, name, (x, k, dy))
         if _is_torch(x):
-            return _dev_conv("bi", g, k, dy, out)
-        args = (out, k, dy)
+            return _dev_conv("bi", g, k, dy, res)
+        args = (res, k, dy)
     else:
-        out = _empty_like_kind(k, list(k.shape), dtype=out_dtype)
+        res = _out_buf(out, k, list(k.shape), out_dtype, name, (x, k, dy))
         if _is_torch(x):
-            return _dev_conv("bk", g, x, dy, out)
-        args = (x, out, dy)
+            return _dev_conv("bk", g, x, dy, res, workspace=workspace)
+        args = (x, res, dy)
     if nd == 2:
         _host_conv2d(variant, *args, g, None, rate)
     else:
         _host_conv3d(variant, *args, g, None, rate if variant.startswith("dilated") else None)
-    return out
+    return res
 
 
 # ----------------------------------------------------------------------------- conv2d / 3d
-def conv2d(input, kernel, stride, padding=SAME, out_dtype=None):
+def conv2d(input, kernel, stride, padding=SAME, out_dtype=None, out=None):
     """owl_dense_ndarray_generic.ml:2324-2400."""
-    return _conv_fwd("conv2d", 2, input, kernel, stride, None, padding, "spatial", out_dtype)
+    return _conv_fwd("conv2d", 2, input, kernel, stride, None, padding, "spatial", out_dtype, out=out)
 
 
-def conv2d_backward_input(input, kernel, stride, output_grad, out_dtype=None):
+def conv2d_backward_input(input, kernel, stride, output_grad, out_dtype=None, out=None):
     """owl_dense_ndarray_generic.ml:2479-2560."""
     return _conv_bwd("conv2d_backward_input", 2, "input", input, kernel, stride, None, output_grad,
-                     "spatial_backward_input", out_dtype)
+                     "spatial_backward_input", out_dtype, out=out)
 
 
-def conv2d_backward_kernel(input, kernel, stride, output_grad, out_dtype=None):
+def conv2d_backward_kernel(input, kernel, stride, output_grad, out_dtype=None, out=None):
     """owl_dense_ndarray_generic.ml:2660-2740."""
     return _conv_bwd("conv2d_backward_kernel", 2, "kernel", input, kernel, stride, None, output_grad,
-                     "spatial_backward_kernel", out_dtype)
+                     "spatial_backward_kernel", out_dtype, out=out)
 
 
-def conv3d(input, kernel, stride, padding=SAME, out_dtype=None):
+def conv3d(input, kernel, stride, padding=SAME, out_dtype=None, out=None):
     """owl_dense_ndarray_generic.ml:2848 (dispatches to cuboid_im2col, owl_ndarray_conv.ml:826)."""
-    return _conv_fwd("conv3d", 3, input, kernel, stride, None, padding, "cuboid_im2col", out_dtype)
+    return _conv_fwd("conv3d", 3, input, kernel, stride, None, padding, "cuboid_im2col", out_dtype, out=out)
 
 
-def conv3d_backward_input(input, kernel, stride, output_grad, out_dtype=None):
+def conv3d_backward_input(input, kernel, stride, output_grad, out_dtype=None, out=None):
     """owl_dense_ndarray_generic.ml:3017."""
     return _conv_bwd("conv3d_backward_input", 3, "input", input, kernel, stride, None, output_grad,
-                     "cuboid_backward_input_im2col", out_dtype)
+                     "cuboid_backward_input_im2col", out_dtype, out=out)
 
 
-def conv3d_backward_kernel(input, kernel, stride, output_grad, out_dtype=None):
+def conv3d_backward_kernel(input, kernel, stride, output_grad, out_dtype=None, out=None):
     return _conv_bwd("conv3d_backward_kernel", 3, "kernel", input, kernel, stride, None, output_grad,
-                     "cuboid_backward_kernel_im2col", out_dtype)
+                     "cuboid_backward_kernel_im2col", out_dtype, out=out)
 
 
-def dilated_conv2d(input, kernel, stride, rate, padding=SAME, out_dtype=None):
+def dilated_conv2d(input, kernel, stride, rate, padding=SAME, out_dtype=None, out=None):
     """owl_dense_ndarray_generic.ml:3797-3880 (output shape uses the dilated extent)."""
     return _conv_fwd("dilated_conv2d", 2, input, kernel, stride, rate, padding, "dilated_spatial_im2col",
-                     out_dtype)
+                     out_dtype, out=out)
 
 
-def dilated_conv2d_backward_input(input, kernel, stride, rate, output_grad, out_dtype=None):
+def dilated_conv2d_backward_input(input, kernel, stride, rate, output_grad, out_dtype=None, out=None):
     return _conv_bwd("dilated_conv2d_backward_input", 2, "input", input, kernel, stride, rate, output_grad,
-                     "dilated_spatial_backward_input_im2col", out_dtype)
+                     "dilated_spatial_backward_input_im2col", out_dtype, out=out)
 
 
-def dilated_conv2d_backward_kernel(input, kernel, stride, rate, output_grad, out_dtype=None):
+def dilated_conv2d_backward_kernel(input, kernel, stride, rate, output_grad, out_dtype=None, out=None):
     return _conv_bwd("dilated_conv2d_backward_kernel", 2, "kernel", input, kernel, stride, rate,
-                     output_grad, "dilated_spatial_backward_kernel_im2col", out_dtype)
+                     output_grad, "dilated_spatial_backward_kernel_im2col", out_dtype, out=out)
 
 
-def dilated_conv3d(input, kernel, stride, rate, padding=SAME, out_dtype=None):
+def dilated_conv3d(input, kernel, stride, rate, padding=SAME, out_dtype=None, out=None):
     return _conv_fwd("dilated_conv3d", 3, input, kernel, stride, rate, padding, "dilated_cuboid_im2col",
-                     out_dtype)
+                     out_dtype, out=out)
 
 
-def dilated_conv3d_backward_input(input, kernel, stride, rate, output_grad, out_dtype=None):
+def dilated_conv3d_backward_input(input, kernel, stride, rate, output_grad, out_dtype=None, out=None):
     return _conv_bwd("dilated_conv3d_backward_input", 3, "input", input, kernel, stride, rate, output_grad,
-                     "dilated_cuboid_backward_input_im2col", out_dtype)
+                     "dilated_cuboid_backward_input_im2col", out_dtype, out=out)
 
 
-def dilated_conv3d_backward_kernel(input, kernel, stride, rate, output_grad, out_dtype=None):
+def dilated_conv3d_backward_kernel(input, kernel, stride, rate, output_grad, out_dtype=None, out=None):
     return _conv_bwd("dilated_conv3d_backward_kernel", 3, "kernel", input, kernel, stride, rate,
-                     output_grad, "dilated_cuboid_backward_kernel_im2col", out_dtype)
+                     output_grad, "dilated_cuboid_backward_kernel_im2col", out_dtype, out=out)
 
 
 # ----------------------------------------------------------------------------- transpose conv
@@ -321,21 +368,21 @@ def _tconv_geom(x, k, sp_out):
     return B, sp_in, Ci, ksp, Co
 
 
-def transpose_conv2d(input, kernel, stride, padding=SAME):
-    return _transpose_fwd("transpose_conv2d", 2, input, kernel, stride, padding)
+def transpose_conv2d(input, kernel, stride, padding=SAME, out=None):
+    return _transpose_fwd("transpose_conv2d", 2, input, kernel, stride, padding, out)
 
 
-def transpose_conv3d(input, kernel, stride, padding=SAME):
-    return _transpose_fwd("transpose_conv3d", 3, input, kernel, stride, padding)
+def transpose_conv3d(input, kernel, stride, padding=SAME, out=None):
+    return _transpose_fwd("transpose_conv3d", 3, input, kernel, stride, padding, out)
 
 
-def _transpose_fwd(name, nd, x, k, stride, padding):
+def _transpose_fwd(name, nd, x, k, stride, padding, out=None):
     _same_kind(x, k)
     _check_conv_args(name, nd, x, k, stride)
     x, k = _contig(x), _contig(k)
     B, sp_in, Ci, ksp, Co = _tconv_geom(x, k, None)
     sp_out = [transpose_out_dim(padding, n, kk, s) for n, kk, s in zip(sp_in, ksp, stride)]
-    y = _empty_like_kind(x, [B, *sp_out, Co])
+    y = _out_buf(out, x, [B, *sp_out, Co], None, name, (x, k))
     # backward_input of a conv whose input is y ([.., Co]) and output is x ([.., Ci])
     pads = _pads_from_shapes(sp_out, ksp, sp_in, stride, [1] * nd)
     g = _Conv(B, sp_out, Co, ksp, Ci, sp_in, stride, [1] * nd, pads)
@@ -348,49 +395,49 @@ def _transpose_fwd(name, nd, x, k, stride, padding):
     return y
 
 
-def transpose_conv2d_backward_input(input, kernel, stride, output_grad):
-    return _transpose_bwd("transpose_conv2d_backward_input", 2, "input", input, kernel, stride, output_grad)
+def transpose_conv2d_backward_input(input, kernel, stride, output_grad, out=None):
+    return _transpose_bwd("transpose_conv2d_backward_input", 2, "input", input, kernel, stride, output_grad,
+                          out)
 
 
-def transpose_conv2d_backward_kernel(input, kernel, stride, output_grad):
-    return _transpose_bwd("transpose_conv2d_backward_kernel", 2, "kernel", input, kernel, stride, output_grad)
+def transpose_conv2d_backward_kernel(input, kernel, stride, output_grad, out=None):
+    return _transpose_bwd("transpose_conv2d_backward_kernel", 2, "kernel", input, kernel, stride, output_grad,
+                          out)
 
 
-def transpose_conv3d_backward_input(input, kernel, stride, output_grad):
-    return _transpose_bwd("transpose_conv3d_backward_input", 3, "input", input, kernel, stride, output_grad)
+def transpose_conv3d_backward_input(input, kernel, stride, output_grad, out=None):
+    return _transpose_bwd("transpose_conv3d_backward_input", 3, "input", input, kernel, stride, output_grad,
+                          out)
 
 
-def transpose_conv3d_backward_kernel(input, kernel, stride, output_grad):
-    return _transpose_bwd("transpose_conv3d_backward_kernel", 3, "kernel", input, kernel, stride, output_grad)
+def transpose_conv3d_backward_kernel(input, kernel, stride, output_grad, out=None):
+    return _transpose_bwd("transpose_conv3d_backward_kernel", 3, "kernel", input, kernel, stride, output_grad,
+                          out)
 
 
-def _transpose_bwd(name, nd, which, x, k, stride, dy):
+def _transpose_bwd(name, nd, which, x, k, stride, dy, out=None, workspace=None):
     _same_kind(x, k, dy)
     _check_conv_args(name, nd, x, k, stride, None, dy)
     x, k, dy = _contig(x), _contig(k), _contig(dy)
-    B, sp_in, Ci, ksp, Co = _tconv_geom(x, k, None)
-    sp_out = list(dy.shape[1:-1])
-    # conv with input = dy ([.., Co]) and output = x ([.., Ci])
-    pads = _pads_from_shapes(sp_out, ksp, sp_in, stride, [1] * nd)
-    g = _Conv(B, sp_out, Co, ksp, Ci, sp_in, stride, [1] * nd, pads)
+    g = _tbwd_geom(nd, list(x.shape), list(k.shape), stride, list(dy.shape))
     if which == "input":
-        out = _empty_like_kind(x, list(x.shape))
+        res = _out_buf(out, x, list(x.shape), None, name, (x, k, dy))
         if _is_torch(x):
-            return _dev_conv("fwd", g, dy, k, out)
+            return _dev_conv("fwd", g, dy, k, res)
         if nd == 2:
             # Owl passes a dummy SAME code here (owl_dense_ndarray_generic.ml:5783)
-            _host_conv2d("spatial", dy, k, out, g, 0)
+            _host_conv2d("spatial", dy, k, res, g, 0)
         else:
-            _host_conv3d("cuboid_im2col", dy, k, out, g, 0)
+            _host_conv3d("cuboid_im2col", dy, k, res, g, 0)
     else:
-        out = _empty_like_kind(k, list(k.shape))
+        res = _out_buf(out, k, list(k.shape), None, name, (x, k, dy))
         if _is_torch(x):
-            return _dev_conv("bk", g, dy, x, out)
+            return _dev_conv("bk", g, dy, x, res, workspace=workspace)
         if nd == 2:
-            _host_conv2d("spatial_backward_kernel", dy, out, x, g, None)
+            _host_conv2d("spatial_backward_kernel", dy, res, x, g, None)
         else:
-            _host_conv3d("cuboid_backward_kernel_im2col", dy, out, x, g, None)
-    return out
+            _host_conv3d("cuboid_backward_kernel_im2col", dy, res, x, g, None)
+    return res
 
 
 # ----------------------------------------------------------------------------- pooling
@@ -415,11 +462,11 @@ def _pool_desc(B, sp_in, C, kernel, sp_out, stride, pads):
     return _lib.PoolDesc(B, *f(sp_in, 1), C, *f(kernel, 1), *f(sp_out, 1), *f(stride, 1), *f(pads, 0))
 
 
-def _pool_fwd(name, nd, is_max, x, kernel, stride, padding):
+def _pool_fwd(name, nd, is_max, x, kernel, stride, padding, out=None):
     _pool_check(name, nd, x, kernel, stride)
     x = _contig(x)
     B, sp_in, C, sp_out, pads = _pool_geom(x, kernel, stride, padding)
-    y = _empty_like_kind(x, [B, *sp_out, C])
+    y = _out_buf(out, x, [B, *sp_out, C], None, name, (x,))
     if _is_torch(x):
         d = _pool_desc(B, sp_in, C, kernel, sp_out, stride, pads)
         _lib.check(_lib.lib().owl_amd_pool_fwd(ctypes.byref(d), _tdt(x), int(is_max), _ptr(x), _ptr(y),
@@ -438,21 +485,21 @@ def _pool_fwd(name, nd, is_max, x, kernel, stride, padding):
     return y
 
 
-def max_pool2d(input, kernel, stride, padding=SAME):
+def max_pool2d(input, kernel, stride, padding=SAME, out=None):
     """owl_dense_ndarray_generic.ml:6905-6961."""
-    return _pool_fwd("max_pool2d", 2, True, input, kernel, stride, padding)
+    return _pool_fwd("max_pool2d", 2, True, input, kernel, stride, padding, out)
 
 
-def avg_pool2d(input, kernel, stride, padding=SAME):
-    return _pool_fwd("avg_pool2d", 2, False, input, kernel, stride, padding)
+def avg_pool2d(input, kernel, stride, padding=SAME, out=None):
+    return _pool_fwd("avg_pool2d", 2, False, input, kernel, stride, padding, out)
 
 
-def max_pool3d(input, kernel, stride, padding=SAME):
-    return _pool_fwd("max_pool3d", 3, True, input, kernel, stride, padding)
+def max_pool3d(input, kernel, stride, padding=SAME, out=None):
+    return _pool_fwd("max_pool3d", 3, True, input, kernel, stride, padding, out)
 
 
-def avg_pool3d(input, kernel, stride, padding=SAME):
-    return _pool_fwd("avg_pool3d", 3, False, input, kernel, stride, padding)
+def avg_pool3d(input, kernel, stride, padding=SAME, out=None):
+    return _pool_fwd("avg_pool3d", 3, False, input, kernel, stride, padding, out)
 
 
 def max_pool2d_argmax(input, kernel, stride, padding=SAME):
@@ -475,13 +522,13 @@ def max_pool2d_argmax(input, kernel, stride, padding=SAME):
     return y, arg
 
 
-def _pool_bwd(name, nd, is_max, padding, x, kernel, stride, dy):
+def _pool_bwd(name, nd, is_max, padding, x, kernel, stride, dy, out=None):
     _pool_check(name, nd, x, kernel, stride)
     _same_kind(x, dy)
     x, dy = _contig(x), _contig(dy)
     B, sp_in, C, sp_out, _ = _pool_geom(x, kernel, stride, padding)
     _verify(list(dy.shape) == [B, *sp_out, C], f"{name}: output' shape {list(dy.shape)} does not match")
-    dx = _empty_like_kind(x, list(x.shape))
+    dx = _out_buf(out, x, list(x.shape), None, name, (x, dy))
     if nd == 2:
         top, left, _, _ = calc_conv2d_padding(sp_in[0], sp_in[1], kernel[0], kernel[1], sp_out[0],
                                               sp_out[1], stride[1], stride[0])
@@ -511,21 +558,21 @@ def _pool_bwd(name, nd, is_max, padding, x, kernel, stride, dy):
     return dx
 
 
-def max_pool2d_backward(padding, input, kernel, stride, output_grad):
+def max_pool2d_backward(padding, input, kernel, stride, output_grad, out=None):
     """owl_dense_ndarray_generic.ml:7715-7775."""
-    return _pool_bwd("max_pool2d_backward", 2, True, padding, input, kernel, stride, output_grad)
+    return _pool_bwd("max_pool2d_backward", 2, True, padding, input, kernel, stride, output_grad, out)
 
 
-def avg_pool2d_backward(padding, input, kernel, stride, output_grad):
-    return _pool_bwd("avg_pool2d_backward", 2, False, padding, input, kernel, stride, output_grad)
+def avg_pool2d_backward(padding, input, kernel, stride, output_grad, out=None):
+    return _pool_bwd("avg_pool2d_backward", 2, False, padding, input, kernel, stride, output_grad, out)
 
 
-def max_pool3d_backward(padding, input, kernel, stride, output_grad):
-    return _pool_bwd("max_pool3d_backward", 3, True, padding, input, kernel, stride, output_grad)
+def max_pool3d_backward(padding, input, kernel, stride, output_grad, out=None):
+    return _pool_bwd("max_pool3d_backward", 3, True, padding, input, kernel, stride, output_grad, out)
 
 
-def avg_pool3d_backward(padding, input, kernel, stride, output_grad):
-    return _pool_bwd("avg_pool3d_backward", 3, False, padding, input, kernel, stride, output_grad)
+def avg_pool3d_backward(padding, input, kernel, stride, output_grad, out=None):
+    return _pool_bwd("avg_pool3d_backward", 3, False, padding, input, kernel, stride, output_grad, out)
 
 
 # ----------------------------------------------------------------------------- fused conv layer epilogue
@@ -537,13 +584,13 @@ def _act_code(act):
     return _ACT[act]
 
 
-def _conv_bias_act(name, nd, x, k, bias, stride, rate, padding, act, keep_pre, out_dtype):
+def _conv_bias_act(name, nd, x, k, bias, stride, rate, padding, act, keep_pre, out_dtype, out=None, pre_out=None):
     _verify(_is_torch(x) and _is_torch(k), f"{name}: the fused layer ops take device tensors")
     _check_conv_args(name, nd, x, k, stride, rate)
     x, k = _contig(x), _contig(k)
     g = conv_problem(list(x.shape), list(k.shape), stride, rate, padding)
-    y = _empty_like_kind(x, [g.B, *g.sp_out, g.Co], dtype=out_dtype)
-    pre = torch.empty_like(y) if keep_pre else None
+    y = _out_buf(out, x, [g.B, *g.sp_out, g.Co], out_dtype, name, (x, k))
+    pre = _out_buf(pre_out, x, list(y.shape), y.dtype, name, (x, k, y)) if keep_pre else None
     acc_dt = torch.float64 if y.dtype == torch.float64 else torch.float32
     if bias is not None:
         _verify(_is_torch(bias) and tuple(bias.shape) == (g.Co,),
@@ -557,21 +604,24 @@ def _conv_bias_act(name, nd, x, k, bias, stride, rate, padding, act, keep_pre, o
     return (y, pre) if keep_pre else y
 
 
-def conv2d_bias_act(input, kernel, bias, stride, padding=SAME, act="relu", keep_pre=False, out_dtype=None):
+def conv2d_bias_act(input, kernel, bias, stride, padding=SAME, act="relu", keep_pre=False, out_dtype=None,
+                    out=None, pre_out=None):
     """Owl's 2-D conv layer forward, ``act (conv2d ~padding x w stride + b)``
     (owl_neural_neuron.ml:1253 Conv2D.run, then the activation), with the bias
     add and the activation in the convolution's epilogue.  relu is
     ``fmax(v, 0)`` as owl_ndarray_maths_stub.c:3559-3569.  Device tensors
     only; bias is [Co] in float32 (float64 for float64 output) or None.
-    Returns y, or (y, pre) with pre = conv + bias when keep_pre."""
+    Returns y, or (y, pre) with pre = conv + bias when keep_pre; ``out`` and
+    ``pre_out`` are optional arrays to write them into."""
     return _conv_bias_act("conv2d_bias_act", 2, input, kernel, bias, stride, None, padding, act, keep_pre,
-                          out_dtype)
+                          out_dtype, out, pre_out)
 
 
-def conv3d_bias_act(input, kernel, bias, stride, padding=SAME, act="relu", keep_pre=False, out_dtype=None):
+def conv3d_bias_act(input, kernel, bias, stride, padding=SAME, act="relu", keep_pre=False, out_dtype=None,
+                    out=None, pre_out=None):
     """The 3-D form of ``conv2d_bias_act`` (Owl's Conv3D layer)."""
     return _conv_bias_act("conv3d_bias_act", 3, input, kernel, bias, stride, None, padding, act, keep_pre,
-                          out_dtype)
+                          out_dtype, out, pre_out)
 
 
 def bias_act_backward(pre, output_grad, act="relu", bias_grad=True):
@@ -606,14 +656,14 @@ def _check_upsampling(name, x, size):
             f"size dimension = {len(size)} (should be 2).")
 
 
-def upsampling2d(input, size):
+def upsampling2d(input, size, out=None):
     """owl_dense_ndarray_generic.ml:8223-8234: ``repeat input [|1; s0; s1; 1|]``
     (nearest neighbour, s0 along H = Owl's cols, s1 along W = rows)."""
     _check_upsampling("upsampling2d", input, size)
     x = _contig(input)
     B, H, W, C = x.shape
     s0, s1 = int(size[0]), int(size[1])
-    y = _empty_like_kind(x, [B, H * s0, W * s1, C])
+    y = _out_buf(out, x, [B, H * s0, W * s1, C], None, "upsampling2d", (x,))
     if _is_torch(x):
         _lib.check(_lib.lib().owl_amd_upsampling2d_fwd(_tdt(x), B, H, W, C, H * s0, W * s1, s0, s1, _ptr(x),
                                                         _ptr(y), _stream()))
@@ -623,7 +673,7 @@ def upsampling2d(input, size):
     return y
 
 
-def upsampling2d_backward(input, size, output_grad):
+def upsampling2d_backward(input, size, output_grad, out=None):
     """owl_dense_ndarray_generic.ml:8249-8290 and the stub
     owl_ndarray_upsampling_impl.h:9-60: the input's gradient, each cell the
     sum of the output gradient over its s0 x s1 block (Owl's order)."""
@@ -636,7 +686,7 @@ def upsampling2d_backward(input, size, output_grad):
             f"upsampling2d_backward: output shape is [{','.join(map(str, dy.shape))}]; scaled output cols is "
             f"{H * s0}, should be equal to the 2nd dimension of output shape; scaled output rows is {W * s1}, "
             f"should be equal to the 3rd dimension of output shape.")
-    dx = _empty_like_kind(dy, [B, H, W, C])
+    dx = _out_buf(out, dy, [B, H, W, C], None, "upsampling2d_backward", (dy,))
     if _is_torch(dy):
         _lib.check(_lib.lib().owl_amd_upsampling2d_bwd(_tdt(dy), B, H, W, C, H * s0, W * s1, s0, s1, _ptr(dy),
                                                         _ptr(dx), _stream()))
@@ -680,86 +730,161 @@ def _y1(y):
     return _rs(y, [y.shape[0], y.shape[2], y.shape[3]])
 
 
-def conv1d(input, kernel, stride, padding=SAME):
+def _o2(out, name, kernel=False):
+    """A 1-D op's ``out`` as the 2-D view its 2-D op writes (a reshape that
+    must not copy, so the array has to be contiguous)."""
+    if out is None:
+        return None
+    _verify(len(out.shape) == 3, f"{name}: out dimension = {len(out.shape)} (should be 3)")
+    _verify(out.is_contiguous() if _is_torch(out) else out.flags.c_contiguous, f"{name}: out must be contiguous")
+    return _k2(out) if kernel else _x2(out)
+
+
+def conv1d(input, kernel, stride, padding=SAME, out=None):
     """owl_dense_ndarray_generic.ml:3402-3445."""
     _check_1d("conv1d", input, kernel, stride)
-    return _y1(conv2d(_x2(input), _k2(kernel), [1, stride[0]], padding))
+    return _y1(conv2d(_x2(input), _k2(kernel), [1, stride[0]], padding, out=_o2(out, "conv1d")))
 
 
-def conv1d_backward_input(input, kernel, stride, output_grad):
+def conv1d_backward_input(input, kernel, stride, output_grad, out=None):
     """owl_dense_ndarray_generic.ml:3489-3561."""
     _check_1d("conv1d_backward_input", input, kernel, stride, output_grad)
-    return _rs(conv2d_backward_input(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad)), input.shape)
+    return _rs(conv2d_backward_input(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad),
+                                     out=_o2(out, "conv1d_backward_input")), input.shape)
 
 
-def conv1d_backward_kernel(input, kernel, stride, output_grad):
+def conv1d_backward_kernel(input, kernel, stride, output_grad, out=None):
     """owl_dense_ndarray_generic.ml:3639-3713."""
     _check_1d("conv1d_backward_kernel", input, kernel, stride, output_grad)
-    return _rs(conv2d_backward_kernel(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad)), kernel.shape)
+    return _rs(conv2d_backward_kernel(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad),
+                                      out=_o2(out, "conv1d_backward_kernel", kernel=True)), kernel.shape)
 
 
-def dilated_conv1d(input, kernel, stride, rate, padding=SAME):
+def dilated_conv1d(input, kernel, stride, rate, padding=SAME, out=None):
     """owl_dense_ndarray_generic.ml:4967-5010.  Owl hands ``rate`` to the 2-D
     function unchanged, so it must have two entries (rows, cols) as there."""
     _check_1d("dilated_conv1d", input, kernel, stride)
-    return _y1(dilated_conv2d(_x2(input), _k2(kernel), [1, stride[0]], rate, padding))
+    return _y1(dilated_conv2d(_x2(input), _k2(kernel), [1, stride[0]], rate, padding,
+                              out=_o2(out, "dilated_conv1d")))
 
 
-def dilated_conv1d_backward_input(input, kernel, stride, rate, output_grad):
+def dilated_conv1d_backward_input(input, kernel, stride, rate, output_grad, out=None):
     """owl_dense_ndarray_generic.ml:5054-5130 (``rate`` as in dilated_conv1d)."""
     _check_1d("dilated_conv1d_backward_input", input, kernel, stride, output_grad)
-    return _rs(dilated_conv2d_backward_input(_x2(input), _k2(kernel), [1, stride[0]], rate, _x2(output_grad)),
-               input.shape)
+    return _rs(dilated_conv2d_backward_input(_x2(input), _k2(kernel), [1, stride[0]], rate, _x2(output_grad),
+                                             out=_o2(out, "dilated_conv1d_backward_input")), input.shape)
 
 
-def dilated_conv1d_backward_kernel(input, kernel, stride, rate, output_grad):
+def dilated_conv1d_backward_kernel(input, kernel, stride, rate, output_grad, out=None):
     """owl_dense_ndarray_generic.ml:5210-5286 (``rate`` as in dilated_conv1d)."""
     _check_1d("dilated_conv1d_backward_kernel", input, kernel, stride, output_grad)
-    return _rs(dilated_conv2d_backward_kernel(_x2(input), _k2(kernel), [1, stride[0]], rate, _x2(output_grad)),
+    return _rs(dilated_conv2d_backward_kernel(_x2(input), _k2(kernel), [1, stride[0]], rate, _x2(output_grad),
+                                              out=_o2(out, "dilated_conv1d_backward_kernel", kernel=True)),
                kernel.shape)
 
 
-def transpose_conv1d(input, kernel, stride, padding=SAME):
+def transpose_conv1d(input, kernel, stride, padding=SAME, out=None):
     """owl_dense_ndarray_generic.ml:6483-6526."""
     _check_1d("transpose_conv1d", input, kernel, stride)
-    return _y1(transpose_conv2d(_x2(input), _k2(kernel), [1, stride[0]], padding))
+    return _y1(transpose_conv2d(_x2(input), _k2(kernel), [1, stride[0]], padding, out=_o2(out, "transpose_conv1d")))
 
 
-def transpose_conv1d_backward_input(input, kernel, stride, output_grad):
+def transpose_conv1d_backward_input(input, kernel, stride, output_grad, out=None):
     """owl_dense_ndarray_generic.ml:6570-6646."""
     _check_1d("transpose_conv1d_backward_input", input, kernel, stride, output_grad)
-    return _rs(transpose_conv2d_backward_input(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad)),
-               input.shape)
+    return _rs(transpose_conv2d_backward_input(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad),
+                                               out=_o2(out, "transpose_conv1d_backward_input")), input.shape)
 
 
-def transpose_conv1d_backward_kernel(input, kernel, stride, output_grad):
+def transpose_conv1d_backward_kernel(input, kernel, stride, output_grad, out=None):
     """owl_dense_ndarray_generic.ml:6732-6814."""
     _check_1d("transpose_conv1d_backward_kernel", input, kernel, stride, output_grad)
-    return _rs(transpose_conv2d_backward_kernel(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad)),
+    return _rs(transpose_conv2d_backward_kernel(_x2(input), _k2(kernel), [1, stride[0]], _x2(output_grad),
+                                                out=_o2(out, "transpose_conv1d_backward_kernel", kernel=True)),
                kernel.shape)
 
 
-def max_pool1d(input, kernel, stride, padding=SAME):
+def max_pool1d(input, kernel, stride, padding=SAME, out=None):
     """owl_dense_ndarray_generic.ml:7027-7053."""
     _check_1d("max_pool1d", input, stride=stride, kernel_len=len(kernel))
-    return _y1(max_pool2d(_x2(input), [1, kernel[0]], [1, stride[0]], padding))
+    return _y1(max_pool2d(_x2(input), [1, kernel[0]], [1, stride[0]], padding, out=_o2(out, "max_pool1d")))
 
 
-def avg_pool1d(input, kernel, stride, padding=SAME):
+def avg_pool1d(input, kernel, stride, padding=SAME, out=None):
     """owl_dense_ndarray_generic.ml:7197-7223."""
     _check_1d("avg_pool1d", input, stride=stride, kernel_len=len(kernel))
-    return _y1(avg_pool2d(_x2(input), [1, kernel[0]], [1, stride[0]], padding))
+    return _y1(avg_pool2d(_x2(input), [1, kernel[0]], [1, stride[0]], padding, out=_o2(out, "avg_pool1d")))
 
 
-def max_pool1d_backward(padding, input, kernel, stride, output_grad):
+def max_pool1d_backward(padding, input, kernel, stride, output_grad, out=None):
     """owl_dense_ndarray_generic.ml:7840-7871."""
     _check_1d("max_pool1d_backward", input, stride=stride, kernel_len=len(kernel))
-    return _rs(max_pool2d_backward(padding, _x2(input), [1, kernel[0]], [1, stride[0]], _x2(output_grad)),
-               input.shape)
+    return _rs(max_pool2d_backward(padding, _x2(input), [1, kernel[0]], [1, stride[0]], _x2(output_grad),
+                                   out=_o2(out, "max_pool1d_backward")), input.shape)
 
 
-def avg_pool1d_backward(padding, input, kernel, stride, output_grad):
+def avg_pool1d_backward(padding, input, kernel, stride, output_grad, out=None):
     """owl_dense_ndarray_generic.ml:8158-8189."""
     _check_1d("avg_pool1d_backward", input, stride=stride, kernel_len=len(kernel))
-    return _rs(avg_pool2d_backward(padding, _x2(input), [1, kernel[0]], [1, stride[0]], _x2(output_grad)),
-               input.shape)
+    return _rs(avg_pool2d_backward(padding, _x2(input), [1, kernel[0]], [1, stride[0]], _x2(output_grad),
+                                   out=_o2(out, "avg_pool1d_backward")), input.shape)
+
+
+# ----------------------------------------------------------------------------- in-place variants
+# Owl's ``*_ ~out`` wrappers compute into a given array (the computation
+# graph's evaluator calls these, owl_computation_cpu_eval.ml:180-281).  Here
+# ``out`` is the first argument, then the function's own arguments; the array
+# is checked (kind, shape, dtype, contiguity, no overlap with an input) and
+# the call returns None.
+def _inplace(f, owl_line):
+    def g(out, *args, **kw):
+        f(*args, out=out, **kw)
+    g.__name__ = g.__qualname__ = f.__name__ + "_"
+    g.__doc__ = (f"``{f.__name__}`` written into ``out`` (``{f.__name__}_ ~out``, "
+                 f"owl_dense_ndarray_generic.ml:{owl_line}).")
+    return g
+
+
+conv1d_ = _inplace(conv1d, 3447)
+conv1d_backward_input_ = _inplace(conv1d_backward_input, 3563)
+conv1d_backward_kernel_ = _inplace(conv1d_backward_kernel, 3715)
+conv2d_ = _inplace(conv2d, 2402)
+conv2d_backward_input_ = _inplace(conv2d_backward_input, 2569)
+conv2d_backward_kernel_ = _inplace(conv2d_backward_kernel, 2752)
+conv3d_ = _inplace(conv3d, 2934)
+conv3d_backward_input_ = _inplace(conv3d_backward_input, 3111)
+conv3d_backward_kernel_ = _inplace(conv3d_backward_kernel, 3302)
+dilated_conv1d_ = _inplace(dilated_conv1d, 5012)
+dilated_conv1d_backward_input_ = _inplace(dilated_conv1d_backward_input, 5132)
+dilated_conv1d_backward_kernel_ = _inplace(dilated_conv1d_backward_kernel, 5288)
+dilated_conv2d_ = _inplace(dilated_conv2d, 3879)
+dilated_conv2d_backward_input_ = _inplace(dilated_conv2d_backward_input, 4056)
+dilated_conv2d_backward_kernel_ = _inplace(dilated_conv2d_backward_kernel, 4247)
+dilated_conv3d_ = _inplace(dilated_conv3d, 4445)
+dilated_conv3d_backward_input_ = _inplace(dilated_conv3d_backward_input, 4645)
+dilated_conv3d_backward_kernel_ = _inplace(dilated_conv3d_backward_kernel, 4856)
+transpose_conv1d_ = _inplace(transpose_conv1d, 6528)
+transpose_conv1d_backward_input_ = _inplace(transpose_conv1d_backward_input, 6648)
+transpose_conv1d_backward_kernel_ = _inplace(transpose_conv1d_backward_kernel, 6816)
+transpose_conv2d_ = _inplace(transpose_conv2d, 5443)
+transpose_conv2d_backward_input_ = _inplace(transpose_conv2d_backward_input, 5809)
+transpose_conv2d_backward_kernel_ = _inplace(transpose_conv2d_backward_kernel, 5614)
+transpose_conv3d_ = _inplace(transpose_conv3d, 5995)
+transpose_conv3d_backward_input_ = _inplace(transpose_conv3d_backward_input, 6172)
+transpose_conv3d_backward_kernel_ = _inplace(transpose_conv3d_backward_kernel, 6381)
+max_pool1d_ = _inplace(max_pool1d, 7055)
+max_pool2d_ = _inplace(max_pool2d, 6964)
+max_pool3d_ = _inplace(max_pool3d, 7322)
+avg_pool1d_ = _inplace(avg_pool1d, 7225)
+avg_pool2d_ = _inplace(avg_pool2d, 7139)
+avg_pool3d_ = _inplace(avg_pool3d, 7453)
+max_pool1d_backward_ = _inplace(max_pool1d_backward, 7873)
+max_pool2d_backward_ = _inplace(max_pool2d_backward, 7778)
+max_pool3d_backward_ = _inplace(max_pool3d_backward, 7650)
+avg_pool1d_backward_ = _inplace(avg_pool1d_backward, 8191)
+avg_pool2d_backward_ = _inplace(avg_pool2d_backward, 8097)
+avg_pool3d_backward_ = _inplace(avg_pool3d_backward, 7971)
+upsampling2d_ = _inplace(upsampling2d, 8236)
+upsampling2d_backward_ = _inplace(upsampling2d_backward, 8305)
+
+__all__ += [n for n in list(globals()) if n.endswith("_") and not n.startswith("_") and callable(globals()[n])]
