@@ -151,6 +151,40 @@ def main():
         "layer": "BF16 relu(conv2d(x, w) + b), x[64,56,56,64] w[3,3,64,128]",
     }
 
+    # a VGG block as a computation graph (BF16): conv + bias + relu twice,
+    # then a 2x2 max-pool; eager unfused ndarray calls, the optimised graph
+    # (both layers fused) evaluated eagerly, and its HIP-graph replay
+    from owl_amd import cgraph as G
+    w2b = (torch.randn(K, K, CO, CO, device=dev, generator=g) * (2.0 / (K * K * CO)) ** 0.5).bfloat16()
+    b2 = (torch.randn(CO, device=dev, generator=g) * 0.1).bfloat16()
+    b1 = bias.bfloat16()
+    gx, gw1, gb1 = G.var_arr("x", [B, H, W, CI], torch.bfloat16), G.var_arr("w1", [K, K, CI, CO], torch.bfloat16), \
+        G.var_arr("b1", [CO], torch.bfloat16)
+    gw2, gb2 = G.var_arr("w2", [K, K, CO, CO], torch.bfloat16), G.var_arr("b2", [CO], torch.bfloat16)
+    h1 = G.relu(G.add(G.conv2d(gx, gw1, [1, 1]), gb1))
+    gy = G.max_pool2d(G.relu(G.add(G.conv2d(h1, gw2, [1, 1]), gb2)), [2, 2], [2, 2], VALID)
+    graph = G.optimise(G.make_graph([gx, gw1, gb1, gw2, gb2], [gy], "vgg_block"))
+    for n_, v_ in zip([gx, gw1, gb1, gw2, gb2], [xb, wb, b1, w2b, b2]):
+        G.assign_arr(n_, v_)
+
+    def block_eager():
+        h = torch.fmax(nd.conv2d(xb, wb, [1, 1], SAME) + b1, zero)
+        h = torch.fmax(nd.conv2d(h, w2b, [1, 1], SAME) + b2, zero)
+        return nd.max_pool2d(h, [2, 2], [2, 2], VALID)
+    t_eager = t_loop(block_eager)
+    t_graph = t_loop(lambda: G.eval_graph(graph))
+    G.capture_graph(graph)
+    t_replay = t_loop(lambda: G.replay_graph(graph))
+    gflop_block = (FLOP_PASS + 2.0 * B * H * W * CO * K * K * CO) / 1e9
+    cgraph_block = {
+        "eager_unfused_us": round(t_eager, 1), "graph_eval_us": round(t_graph, 1),
+        "graph_replay_us": round(t_replay, 1),
+        "graph_replay_tflops": round(gflop_block * 1e9 / (t_replay * 1e-6) / 1e12, 1),
+        "graph_blocks_mb": round(graph.planned_bytes() / 1e6, 1),
+        "block": "BF16 max_pool2x2(relu(conv(relu(conv(x, w1) + b1), w2) + b2)), x[64,56,56,64] "
+                 "w1[3,3,64,128] w2[3,3,128,128]; replay includes copying the 5 inputs into the graph",
+    }
+
     # pooling on the FP32 layer output
     y32 = nd.conv2d(x32, w32, [1, 1], SAME)
     gp = torch.randn(B, H // 2, W // 2, CO, device=dev, generator=g)
@@ -255,6 +289,7 @@ def main():
         "errors": errs,
         "pool_max2x2s2": pool,
         "conv_bias_relu_bf16": fused,
+        "cgraph_vgg_block_bf16": cgraph_block,
         "kernels_bf16": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
