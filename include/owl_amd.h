@@ -45,8 +45,19 @@ typedef enum {
   OWL_AMD_F64 = 1,
   OWL_AMD_BF16 = 2,
   OWL_AMD_F16 = 3,
-  OWL_AMD_FP8E4M3 = 4
+  OWL_AMD_FP8E4M3 = 4,
+  OWL_AMD_C32 = 5, /* complex, FP32 parts (Owl's complex32) */
+  OWL_AMD_C64 = 6  /* complex, FP64 parts (Owl's complex64) */
 } owl_amd_dtype;
+
+/* Complex elements: real part then imaginary part, the layout of C99
+ * `_Complex float` / `_Complex double` that Owl's complex stubs use. */
+typedef struct {
+  float re, im;
+} owl_amd_complex32;
+typedef struct {
+  double re, im;
+} owl_amd_complex64;
 
 typedef struct {
   int64_t B, H, W, D, Ci; /* input  [B][H][W][D][Ci] */
@@ -340,9 +351,23 @@ int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max,
 OWL_AMD_DECLARE_DROPINS(float32, float)
 OWL_AMD_DECLARE_DROPINS(float64, double)
 
-/* Complex dtypes (owl_ndarray_conv_stub.c:91-122) are not provided on the GPU:
- * owl_amd_complex_supported() returns 0 and the OCaml shim keeps Owl's CPU
- * stubs for complex32 / complex64. */
+/* Complex dtypes (owl_ndarray_conv_stub.c:91-122, owl_ndarray_pool_stub.c:92-165,
+ * owl_ndarray_upsampling_stub.c:31-45): the same drop-ins for complex32 and
+ * complex64, and OWL_AMD_C32 / OWL_AMD_C64 on the device entry points.
+ *  - Convolutions run as the real convolution of the interleaved channels
+ *    (2 Ci -> 2 Co) with the kernel expanded on the device into the real
+ *    matrix of the complex-linear map; Owl's complex GEMMs use plain (not
+ *    conjugate) transposes, so the backward passes are the plain transposes
+ *    too.  Results agree with Owl's complex arithmetic to FP rounding.
+ *  - Average pooling and upsampling are the real kernels on 2 C channels
+ *    (complex sums and division by the count are componentwise): bit-exact.
+ *  - Max pooling (and its backward and argmax) returns OWL_AMD_ERR_UNSUPPORTED:
+ *    Owl's complex max starts from -INFINITY, i.e. -inf + 0i whose modulus is
+ *    +inf, so its modulus-then-argument comparison (owl_core.h:125,133) never
+ *    replaces it; the shim keeps Owl's CPU stub, whose result callers get today.
+ * owl_amd_complex_supported() returns 1. */
+OWL_AMD_DECLARE_DROPINS(complex32, owl_amd_complex32)
+OWL_AMD_DECLARE_DROPINS(complex64, owl_amd_complex64)
 int owl_amd_complex_supported(void);
 
 #ifdef __cplusplus

@@ -11,7 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libowl_amd.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM = 0, 1, 2, 3, 4
-F32, F64, BF16, F16, FP8E4M3 = 0, 1, 2, 3, 4
+F32, F64, BF16, F16, FP8E4M3, C32, C64 = 0, 1, 2, 3, 4, 5, 6
 KT_FWD, KT_BWD_INPUT, KT_BWD_KERNEL, KT_AUX = 0, 1, 2, 3
 
 
@@ -50,7 +50,7 @@ POOL3D_BWD = "ppp" + "i" * 15
 UPSAMPLE = "pp" + "i" * 8
 
 DROPINS = {}
-for _n in ("float32", "float64"):
+for _n in ("float32", "float64", "complex32", "complex64"):
     for _v, _k in (("spatial", CONV2D_FWD), ("spatial_im2col", CONV2D_FWD), ("spatial_mec", CONV2D_FWD),
                    ("spatial_naive", CONV2D_FWD), ("dilated_spatial_im2col", CONV2D_FWD),
                    ("spatial_backward_input", CONV2D_BWD), ("spatial_backward_kernel", CONV2D_BWD),

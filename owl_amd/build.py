@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libowl_amd.so")
 INC = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["conv.hip", "conv_fast.hip", "conv_fp8.hip", "pool.hip", "epilogue.hip", "resample.hip", "abi.cpp"]
+SOURCES = ["conv.hip", "conv_fast.hip", "conv_fp8.hip", "pool.hip", "epilogue.hip", "resample.hip", "complex.hip", "abi.cpp"]
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I" + INC, "-I" + CSRC,

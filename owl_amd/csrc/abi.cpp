@@ -183,6 +183,7 @@ struct DeviceState {
   int *flag = nullptr;           // pooling orphan flag (one int)
   DevBuf ws;                     // split-K workspace
   DevBuf red;                    // bias-gradient partial sums
+  DevBuf cplx;                   // complex convolutions: expanded kernel / real weight gradient
   DevBuf slot[4];                // drop-in staging buffers
 };
 
@@ -244,6 +245,8 @@ size_t dtype_size(int t) {
     case OWL_AMD_BF16:
     case OWL_AMD_F16: return 2;
     case OWL_AMD_FP8E4M3: return 1;
+    case OWL_AMD_C32: return 8;
+    case OWL_AMD_C64: return 16;
   }
   return 0;
 }
@@ -429,7 +432,7 @@ extern "C" int owl_amd_init(int device) {
   return ensure_state(&s);
 }
 
-extern "C" int owl_amd_complex_supported(void) { return 0; }
+extern "C" int owl_amd_complex_supported(void) { return 1; }
 
 extern "C" int owl_amd_cache_stats(int64_t *n, int64_t *bytes) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -460,9 +463,75 @@ extern "C" void owl_amd_cache_release(void) {
   }
 }
 
+// ------------------------------------------------------------------ complex kinds
+// A complex convolution is the real one over the interleaved channels
+// (2 Ci -> 2 Co) with the kernel expanded into the real matrix of the
+// complex-linear map (complex.hip).  The expanded kernel, or the real weight
+// gradient before it is folded, lives in a per-device grow-only buffer, so
+// complex calls on one device that overlap on different streams race.
+namespace {
+bool is_complex(int t) { return t == OWL_AMD_C32 || t == OWL_AMD_C64; }
+int real_of(int t) { return t == OWL_AMD_C32 ? OWL_AMD_F32 : OWL_AMD_F64; }
+owl_amd_conv_desc widen(const owl_amd_conv_desc *d) {
+  owl_amd_conv_desc r = *d;
+  r.Ci *= 2;
+  r.Co *= 2;
+  return r;
+}
+int cplx_scratch(size_t bytes, void **out, hipStream_t st) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  DeviceState *s;
+  int rc = ensure_state(&s);
+  if (rc) return rc;
+  rc = ensure_buf(s->cplx, bytes, st);
+  if (rc) return rc;
+  *out = s->cplx.ptr;
+  return OWL_AMD_OK;
+}
+
+// pass 0: a = x, b = w, c = y; pass 1: a = w, b = dy, c = dx; pass 2: a = x, b = dy, c = dw
+int complex_conv(int pass, const owl_amd_conv_desc *d, int in_t, int out_t, const void *a, const void *b, void *c,
+                 void *ws, size_t ws_bytes, void *stream) {
+  int rc = check_conv(d);
+  if (rc) return rc;
+  if (in_t != out_t) ARG_FAIL("complex convolutions read and write the same kind (%d, %d)", in_t, out_t);
+  const int rt = real_of(in_t);
+  const owl_amd_conv_desc r = widen(d);
+  const int64_t T = d->kh * d->kw * d->kd;
+  const int64_t nk = T * d->Ci * d->Co;  // complex kernel elements
+  hipStream_t st = (hipStream_t)stream;
+  if (nk == 0) {  // nothing to expand: the real call handles the empty cases
+    if (pass == 0) return owl_amd_conv_fwd(&r, rt, rt, a, b, c, stream);
+    if (pass == 1) return owl_amd_conv_bwd_input(&r, rt, rt, a, b, c, stream);
+    return owl_amd_conv_bwd_kernel(&r, rt, rt, a, b, c, ws, ws_bytes, stream);
+  }
+  if (!a || !b || !c) ARG_FAIL("null pointer");
+  if (d->Ci > INT32_MAX / 2 || d->Co > INT32_MAX / 2) ARG_FAIL("channel count above 2^30");
+  void *scr;
+  rc = cplx_scratch((size_t)nk * 4 * dtype_size(rt), &scr, st);
+  if (rc) return rc;
+  const int ci = (int)d->Ci, co = (int)d->Co;
+  if (pass == 0 || pass == 1) {
+    const void *w = pass == 0 ? b : a;
+    if (rt == OWL_AMD_F32) run_cplx_expand<float>((const float *)w, (float *)scr, T, ci, co, pass, st);
+    else run_cplx_expand<double>((const double *)w, (double *)scr, T, ci, co, pass, st);
+    OWL_CHECK_HIP(hipGetLastError());
+    return pass == 0 ? owl_amd_conv_fwd(&r, rt, rt, a, scr, c, stream)
+                     : owl_amd_conv_bwd_input(&r, rt, rt, scr, b, c, stream);
+  }
+  rc = owl_amd_conv_bwd_kernel(&r, rt, rt, a, b, scr, ws, ws_bytes, stream);
+  if (rc) return rc;
+  if (rt == OWL_AMD_F32) run_cplx_fold<float>((const float *)scr, (float *)c, T, ci, co, st);
+  else run_cplx_fold<double>((const double *)scr, (double *)c, T, ci, co, st);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+}  // namespace
+
 // ------------------------------------------------------------------ device API: conv
 extern "C" int owl_amd_conv_fwd(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x,
                                 const void *w, void *y, void *stream) {
+  if (is_complex(in_t) || is_complex(out_t)) return complex_conv(0, d, in_t, out_t, x, w, y, nullptr, 0, stream);
   return owl_amd_conv_fwd_bias_act(d, in_t, out_t, x, w, nullptr, OWL_AMD_ACT_NONE, y, nullptr, stream);
 }
 
@@ -471,6 +540,10 @@ extern "C" int owl_amd_conv_fwd_bias_act(const owl_amd_conv_desc *d, int in_t, i
                                          void *stream) {
   int rc = check_conv(d);
   if (rc) return rc;
+  if (is_complex(in_t) || is_complex(out_t)) {
+    owl_amd_set_error("the fused bias / activation epilogue takes real kinds");
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
   if (act != OWL_AMD_ACT_NONE && act != OWL_AMD_ACT_RELU) ARG_FAIL("unknown activation %d", act);
   if (pre && pre == y) ARG_FAIL("pre must not alias y");
   ConvParams p = make_params(d);
@@ -493,6 +566,7 @@ extern "C" int owl_amd_conv_fwd_bias_act(const owl_amd_conv_desc *d, int in_t, i
 
 extern "C" int owl_amd_conv_bwd_input(const owl_amd_conv_desc *d, int in_t, int out_t,
                                       const void *w, const void *dy, void *dx, void *stream) {
+  if (is_complex(in_t) || is_complex(out_t)) return complex_conv(1, d, in_t, out_t, w, dy, dx, nullptr, 0, stream);
   int rc = check_conv(d);
   if (rc) return rc;
   ConvParams p = make_params(d);
@@ -521,6 +595,10 @@ static void bwd_kernel_params(const owl_amd_conv_desc *d, ConvParams &p) {
 
 extern "C" size_t owl_amd_conv_bwd_kernel_workspace(const owl_amd_conv_desc *d, int in_t) {
   if (check_conv(d)) return 0;
+  if (is_complex(in_t)) {
+    const owl_amd_conv_desc r = widen(d);
+    return owl_amd_conv_bwd_kernel_workspace(&r, real_of(in_t));
+  }
   ConvParams p;
   bwd_kernel_params(d, p);
   if (p.M == 0 || p.N == 0 || p.K == 0) return 0;
@@ -535,6 +613,8 @@ extern "C" size_t owl_amd_conv_bwd_kernel_workspace(const owl_amd_conv_desc *d, 
 extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int out_t,
                                        const void *x, const void *dy, void *dw, void *workspace,
                                        size_t ws_bytes, void *stream) {
+  if (is_complex(in_t) || is_complex(out_t))
+    return complex_conv(2, d, in_t, out_t, x, dy, dw, workspace, ws_bytes, stream);
   int rc = check_conv(d);
   if (rc) return rc;
   ConvParams p;
@@ -651,6 +731,8 @@ static int upsample(bool backward, int dt, int64_t B, int64_t H, int64_t W, int6
                     int64_t sh, int64_t sw, const void *src, void *dst, void *stream) {
   if (B < 0 || H < 0 || W < 0 || C < 0 || Ho < 0 || Wo < 0 || sh < 1 || sw < 1)
     ARG_FAIL("bad upsampling sizes");
+  // complex: the real kernel on the interleaved parts (sums are componentwise)
+  if (is_complex(dt)) return upsample(backward, real_of(dt), B, H, W, 2 * C, Ho, Wo, sh, sw, src, dst, stream);
   if (H > INT32_MAX || W > INT32_MAX || C > INT32_MAX || Ho > INT32_MAX || Wo > INT32_MAX)
     ARG_FAIL("upsampling dimension above 2^31");
   if (dt != OWL_AMD_F32 && dt != OWL_AMD_F64 && dt != OWL_AMD_BF16 && dt != OWL_AMD_F16) {
@@ -748,8 +830,8 @@ static int check_pool(const owl_amd_pool_desc *d, int t) {
   if (d->sh < 1 || d->sw < 1 || d->sd < 1) ARG_FAIL("pool stride must be >= 1");
   if (!fits_int(d->B * d->H * d->W * d->D * d->C) || !fits_int(d->B * d->Ho * d->Wo * d->Do * d->C))
     ARG_FAIL("pool tensor too large for 32-bit indexing");
-  if (t != OWL_AMD_F32 && t != OWL_AMD_F64 && t != OWL_AMD_BF16 && t != OWL_AMD_F16) {
-    owl_amd_set_error("pooling supports F32, F64, BF16 and F16");
+  if (t != OWL_AMD_F32 && t != OWL_AMD_F64 && t != OWL_AMD_BF16 && t != OWL_AMD_F16 && !is_complex(t)) {
+    owl_amd_set_error("pooling supports F32, F64, BF16, F16 and (average only) C32, C64");
     return OWL_AMD_ERR_UNSUPPORTED;
   }
   return OWL_AMD_OK;
@@ -765,8 +847,29 @@ static PoolParams make_pool(const owl_amd_pool_desc *d) {
   return p;
 }
 
+// Complex pooling: the average is the real one on the interleaved parts (the
+// complex sum and the division by the count are componentwise); the maximum
+// is not provided (Owl's starts from -inf + 0i, whose modulus is +inf, so
+// its modulus-then-argument comparison never replaces it).
+static int complex_pool(const owl_amd_pool_desc *d, int t, int is_max, owl_amd_pool_desc *r) {
+  if (is_max) {
+    owl_amd_set_error("complex max pooling is not provided: Owl's starts from -inf + 0i (modulus +inf), "
+                      "which its modulus-then-argument comparison never replaces");
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  *r = *d;
+  r->C *= 2;
+  (void)t;
+  return OWL_AMD_OK;
+}
+
 extern "C" int owl_amd_pool_fwd(const owl_amd_pool_desc *d, int t, int is_max, const void *x,
                                 void *y, void *stream) {
+  if (d && is_complex(t)) {
+    owl_amd_pool_desc r;
+    int rc = complex_pool(d, t, is_max, &r);
+    return rc ? rc : owl_amd_pool_fwd(&r, real_of(t), 0, x, y, stream);
+  }
   int rc = check_pool(d, t);
   if (rc) return rc;
   PoolParams p = make_pool(d);
@@ -784,6 +887,10 @@ extern "C" int owl_amd_pool_fwd(const owl_amd_pool_desc *d, int t, int is_max, c
 
 extern "C" int owl_amd_pool_argmax(const owl_amd_pool_desc *d, int t, const void *x, void *y,
                                    int64_t *argmax, void *stream) {
+  if (d && is_complex(t)) {
+    owl_amd_pool_desc r;
+    return complex_pool(d, t, 1, &r);
+  }
   int rc = check_pool(d, t);
   if (rc) return rc;
   PoolParams p = make_pool(d);
@@ -806,6 +913,11 @@ static bool axis_has_empty(int64_t n, int64_t k, int64_t s, int64_t p, int64_t o
 
 extern "C" int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max, const void *x,
                                 const void *dy, void *dx, void *stream) {
+  if (d && is_complex(t)) {
+    owl_amd_pool_desc r;
+    int rc = complex_pool(d, t, is_max, &r);
+    return rc ? rc : owl_amd_pool_bwd(&r, real_of(t), 0, x, dy, dx, stream);
+  }
   int rc = check_pool(d, t);
   if (rc) return rc;
   PoolParams p = make_pool(d);
@@ -884,6 +996,8 @@ int run_dropin(std::initializer_list<HostArg> args, int out_slot, void *out_host
 template <typename T> constexpr int dt();
 template <> constexpr int dt<float>() { return OWL_AMD_F32; }
 template <> constexpr int dt<double>() { return OWL_AMD_F64; }
+template <> constexpr int dt<owl_amd_complex32>() { return OWL_AMD_C32; }
+template <> constexpr int dt<owl_amd_complex64>() { return OWL_AMD_C64; }
 
 enum ConvPass { FWD = 0, BWD_IN = 1, BWD_KER = 2 };
 
@@ -952,6 +1066,8 @@ int pool_dropin_fwd(bool is_max, T *input, T *output, int64_t B, int64_t H, int6
   }
   int rc = check_pool(&d, dt<T>());
   if (rc) return rc;
+  owl_amd_pool_desc r;
+  if (is_complex(dt<T>()) && (rc = complex_pool(&d, dt<T>(), is_max, &r))) return rc;
   if (!input || !output) ARG_FAIL("null pointer");
   const size_t nx = (size_t)(B * H * W * D * C) * sizeof(T);
   const size_t ny = (size_t)(B * Ho * Wo * Do * C) * sizeof(T);
@@ -975,6 +1091,8 @@ int pool_dropin_bwd(bool is_max, T *input, T *output_back, T *input_back, int64_
   d.pd = pd < 0 ? 0 : pd;
   int rc = check_pool(&d, dt<T>());
   if (rc) return rc;
+  owl_amd_pool_desc r;
+  if (is_complex(dt<T>()) && (rc = complex_pool(&d, dt<T>(), is_max, &r))) return rc;
   if (!input || !output_back || !input_back) ARG_FAIL("null pointer");
   const size_t nx = (size_t)(B * H * W * D * C) * sizeof(T);
   const size_t ny = (size_t)(B * Ho * Wo * Do * C) * sizeof(T);
@@ -1000,6 +1118,8 @@ int pool_dropin_arg(T *input, T *output, int64_t *argmax, int64_t B, int64_t H, 
   d.pd = 0;
   int rc = check_pool(&d, dt<T>());
   if (rc) return rc;
+  owl_amd_pool_desc r;
+  if (is_complex(dt<T>()) && (rc = complex_pool(&d, dt<T>(), 1, &r))) return rc;
   if (!input || !output || !argmax) ARG_FAIL("null pointer");
   const size_t nx = (size_t)(B * H * W * C) * sizeof(T);
   const size_t ny = (size_t)(B * Ho * Wo * C) * sizeof(T);
@@ -1219,6 +1339,8 @@ int pool3d_bwd_dropin(bool is_max, T *input, T *output_back, T *input_back, int6
 
 DEFINE_DROPINS(float32, float)
 DEFINE_DROPINS(float64, double)
+DEFINE_DROPINS(complex32, owl_amd_complex32)
+DEFINE_DROPINS(complex64, owl_amd_complex64)
 
 // Upsampling drop-ins, Owl's stub argument order (owl_ndarray_upsampling_impl.h:9-14):
 // cols = H (scaled by col_scale), rows = W (scaled by row_scale).
@@ -1251,6 +1373,8 @@ int upsample_dropin(bool backward, T *input, T *output, int64_t B, int64_t H, in
   }
 DEFINE_UPSAMPLE_DROPINS(float32, float)
 DEFINE_UPSAMPLE_DROPINS(float64, double)
+DEFINE_UPSAMPLE_DROPINS(complex32, owl_amd_complex32)
+DEFINE_UPSAMPLE_DROPINS(complex64, owl_amd_complex64)
 
 // ------------------------------------------------------------------ ragged groups
 // A plan for running many forward convolutions (different shapes) as one

@@ -83,6 +83,14 @@ template <typename T>
 void run_upsample(bool backward, const T *src, T *dst, int64_t B, int H, int W, int C, int Ho, int Wo, int sh,
                   int sw, hipStream_t st);
 
+// complex.hip: complex kernels as real ones (see there).  pattern 0 expands
+// the forward's kernel, 1 the backward input's; fold turns the real weight
+// gradient [T][2Ci][2Co] back into complex [T][Ci][Co].
+template <typename R>
+void run_cplx_expand(const R *w, R *wr, int64_t T, int Ci, int Co, int pattern, hipStream_t st);
+template <typename R>
+void run_cplx_fold(const R *dwr, R *dw, int64_t T, int Ci, int Co, hipStream_t st);
+
 // Kernel timing (owl_amd_kernel_timing).  While timing is enabled, a KTimer
 // scope holds a start / stop event pair, and the first launch_k() inside the
 // scope (on this thread) hands them to hipExtLaunchKernel, which stamps them

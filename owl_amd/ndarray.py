@@ -50,13 +50,18 @@ def _np_prefix(a):
         return "float32"
     if a.dtype == np.float64:
         return "float64"
-    raise InvalidArgument(f"unsupported dtype {a.dtype} (Owl S = float32, D = float64)")
+    if a.dtype == np.complex64:
+        return "complex32"   # Owl's C: pairs of float32
+    if a.dtype == np.complex128:
+        return "complex64"   # Owl's Z: pairs of float64
+    raise InvalidArgument(f"unsupported dtype {a.dtype} (Owl S, D, C, Z = float32, float64, complex64, complex128)")
 
 
 _TORCH_DT = {}
 if torch is not None:
     _TORCH_DT = {torch.float32: _lib.F32, torch.float64: _lib.F64,
-                 torch.bfloat16: _lib.BF16, torch.float16: _lib.F16}
+                 torch.bfloat16: _lib.BF16, torch.float16: _lib.F16,
+                 torch.complex64: _lib.C32, torch.complex128: _lib.C64}
 
 
 def _tdt(t):

@@ -43,7 +43,7 @@ def test_library_loads_and_binds():
     L = _lib.lib()
     for name in _lib.DROPINS:
         assert getattr(L, name).restype == ctypes.c_int
-    assert L.owl_amd_complex_supported() == 0
+    assert L.owl_amd_complex_supported() == 1
 
 
 def _desc(**kw):
