@@ -408,8 +408,10 @@ template <> __device__ __forceinline__ f32x16 mfma16<f16_t>(const v4s16 &a0, con
 }
 
 // FMW: 32-row fragments per wave along M; the tile is BM = 64 FMW rows
-// (2 x 2 waves of 32 FMW x 64).
-template <typename TI, typename TO, int FMW>
+// (2 x 2 waves of 32 FMW x 64).  D1: a 2-D problem (D = Do = kd = 1, no
+// depth pad), whose depth digit is always 0: the loader skips its bounds
+// check and carry.
+template <typename TI, typename TO, int FMW, bool D1>
 __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   // BK 64 (80 KB of LDS) ran slower than BK 32: one workgroup per CU
   constexpr int BM = 64 * FMW, BN = 128, BK = 32, KP = BK / 32;  // KP pixels per loader thread
@@ -513,7 +515,7 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
       for (int q = 0; q < QA; ++q) {
         const int hh = pih[h] + tdh[q], ww = pjw[h] + tdw[q], dd = pld[h] + tdd[q];
         const bool ok = kok & aok[q] & ((unsigned)hh < (unsigned)p.H) & ((unsigned)ww < (unsigned)p.W) &
-                        ((unsigned)dd < (unsigned)p.D);
+                        (D1 || ((unsigned)dd < (unsigned)p.D));
         const uint32_t off = (uint32_t)(pbase[h] + toff[q]) * 2u;
         rg[h][q] = bload(xr, ok ? off : OOB);
       }
@@ -524,11 +526,14 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
       pk[h] += BK;
       gbase[h] += gstep;
       pbase[h] += base_step;
-      pld[h] += inc_l;
-      int c = pld[h] >= lim_l;
-      if (c) {
-        pld[h] -= lim_l;
-        pbase[h] += corr_l;
+      int c = 0;
+      if constexpr (!D1) {
+        pld[h] += inc_l;
+        c = pld[h] >= lim_l;
+        if (c) {
+          pld[h] -= lim_l;
+          pbase[h] += corr_l;
+        }
       }
       pjw[h] += inc_j + (c ? p.sw : 0);
       c = pjw[h] >= lim_j;
@@ -676,8 +681,14 @@ template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st) {
   const int bm = wgrad_tile_m(p.M);
   dim3 grid((unsigned)(((p.M + bm - 1) / bm) * ((p.N + 127) / 128) * p.split));
-  if (bm == 192) launch_k((conv_wgrad_kernel<TI, TO, 3>), grid, dim3(256), st, p);
-  else launch_k((conv_wgrad_kernel<TI, TO, 2>), grid, dim3(256), st, p);
+  const bool d1 = p.D == 1 && p.Do == 1 && p.kd == 1 && p.pd == 0;
+  if (bm == 192) {
+    if (d1) launch_k((conv_wgrad_kernel<TI, TO, 3, true>), grid, dim3(256), st, p);
+    else launch_k((conv_wgrad_kernel<TI, TO, 3, false>), grid, dim3(256), st, p);
+  } else {
+    if (d1) launch_k((conv_wgrad_kernel<TI, TO, 2, true>), grid, dim3(256), st, p);
+    else launch_k((conv_wgrad_kernel<TI, TO, 2, false>), grid, dim3(256), st, p);
+  }
 }
 template void run_wgrad<bf16_t, bf16_t>(const ConvParams &, hipStream_t);
 template void run_wgrad<bf16_t, float>(const ConvParams &, hipStream_t);
