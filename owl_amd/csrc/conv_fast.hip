@@ -720,6 +720,7 @@ struct HaloGeom {
   int kh, kw, rh, rw;  // taps and dilation
   int ph, pw;          // leading pads (of this forward view)
   int B, TR;           // batch, output rows per tile
+  int ms;              // MFMA fragment edge: 32 or 16
   int wCi, wCo;        // the kernel buffer's [kh][kw][wCi][wCo] sizes
   // forward only: fused bias / activation and the pre-activation output
   const float *bias;
@@ -727,23 +728,65 @@ struct HaloGeom {
   int act;
 };
 
+// MFMA of the halo kernel by fragment edge MS: 32 -> v_mfma_f32_32x32x16
+// (two per 32-channel K-step, 16 accumulators per lane), 16 ->
+// v_mfma_f32_16x16x32 (one per K-step, 4 accumulators per lane).  Both run
+// at the same rate; 16-pixel fragments tile rows whose pixel count is not a
+// multiple of 32 with less padding.
+template <typename TI, int MS> struct HaloMfma;
+template <> struct HaloMfma<bf16_t, 32> {
+  typedef f32x16 acc_t;
+  __device__ static __forceinline__ acc_t mma(const uint4 &a, const uint4 &b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+  }
+};
+template <> struct HaloMfma<f16_t, 32> {
+  typedef f32x16 acc_t;
+  __device__ static __forceinline__ acc_t mma(const uint4 &a, const uint4 &b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0,
+                                                  0);
+  }
+};
+template <> struct HaloMfma<bf16_t, 16> {
+  typedef f32x4 acc_t;
+  __device__ static __forceinline__ acc_t mma(const uint4 &a, const uint4 &b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+  }
+};
+template <> struct HaloMfma<f16_t, 16> {
+  typedef f32x4 acc_t;
+  __device__ static __forceinline__ acc_t mma(const uint4 &a, const uint4 &b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0,
+                                                  0);
+  }
+};
+
+// MS: fragment edge (32 or 16).  FM x FN fragments per wave; the four waves
+// form (4 / WGN) x WGN along (pixels, channels), so a tile holds
+// CAP = (4 / WGN) FM MS pixels and BN = WGN FN MS channels.
 // KROWS: the kernel operand is laid out [tap][n][k] (k contiguous), the
 // backward-input view of w; its tile is copied to LDS as [n][k] rows and
 // read with 16-byte loads.  Otherwise (forward weights in place,
 // [tap][k][n]) the tile is copied in that natural layout and the fragments
 // are read with the transposing LDS load (ds_read_b64_tr_b16, as in the
 // weight-gradient kernel), so no pass rearranges the weights first.
-template <int PASS, typename TI, typename TO, int FM, int FN, bool KROWS>
+template <int PASS, typename TI, typename TO, int MS, int FM, int FN, int WGN, bool KROWS>
 __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI *__restrict__ in,
                                                            const TI *__restrict__ wt, TO *__restrict__ out) {
-  typedef FastMfma<TI> MF;
-  constexpr int BK = 32, BN = FN * 32, PR = BK + 8;  // halo pixel row: 32 channels + 16 B pad
+  typedef HaloMfma<TI, MS> MF;
+  typedef typename MF::acc_t acc_t;
+  constexpr int NR = MS == 32 ? 16 : 4;  // accumulators per lane and fragment
+  constexpr int WGM = 4 / WGN;
+  constexpr int CAP = WGM * FM * MS;     // pixels per tile
+  constexpr int BK = 32, BN = WGN * FN * MS, PR = BK + 8;  // halo pixel row: 32 channels + 16 B pad
   constexpr int LDR = BK + 8;
   // Two halo buffers of HBUF pixels: while one channel block's taps run,
   // the next block's halo is loaded into registers (HQ 16-byte chunks per
   // thread) and written to the other buffer.  A halo above HBUF pixels uses
   // both buffers as one and is restaged between channel blocks.
-  constexpr int HQ = FM == 4 ? 10 : 6, HBUF = 352;
+  constexpr int HQ = CAP > 256 ? 10 : 6, HBUF = 352;
   // one LDS block: the two halo buffers, then the two kernel tiles; after
   // the K loop it holds the output tile for the 16-byte store epilogue
   constexpr int HALO_BYTES = 2 * HBUF * PR * (int)sizeof(TI);
@@ -757,7 +800,11 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   char *const Bs0 = smem + HALO_BYTES;  // kernel tile buffer i at Bs0 + i * B_BYTES
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int frow = lane & 31, fkg = lane >> 5;
+  const int wm = wave / WGN, wn = wave % WGN;
+  // lane -> fragment row / column (frow) and k group (fkg): 32-edge
+  // fragments give each half-wave 16 of a K-step's 32 channels (8 per MFMA),
+  // 16-edge fragments each quarter-wave 8
+  const int frow = lane & (MS - 1), fkg = lane / MS;
   const int n_tiles = (g.N + BN - 1) / BN;
   const int row_tiles = (g.Ho + g.TR - 1) / g.TR;
   const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -775,10 +822,10 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   int hoff[FM];
 #pragma unroll
   for (int f = 0; f < FM; ++f) {
-    int p = wave * FM * 32 + f * 32 + frow;
+    int p = wm * FM * MS + f * MS + frow;
     if (p >= P) p = 0;
     const int ti = p / g.Wo, tj = p - (p / g.Wo) * g.Wo;
-    hoff[f] = (ti * HC + tj) * PR + fkg * 16;
+    hoff[f] = (ti * HC + tj) * PR + fkg * (MS == 32 ? 16 : 8);
   }
 
   const uint32_t in_bytes = (uint32_t)((int64_t)g.B * g.Hi * g.Wi * g.C * 2);
@@ -841,7 +888,6 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     }
   };
 
-
   // B staging for (tap, c0): natural layout reads rows w[tap][c0 + k][n0 ..]
   // (BN / 8 chunks of 16 bytes per row); KROWS reads rows w[tap][n][c0 ..].
   // Both use unconditional buffer loads (rows or columns past N read as
@@ -875,41 +921,54 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       }
     }
   };
+  // MS 16 reads k rows r and r + 8 of the natural tile in one phase, which
+  // share banks at this row stride: rows with bit 3 set keep their 16-byte
+  // chunks swapped in pairs two apart (columns XOR 16).  RPP is a multiple
+  // of 16, so a thread's rows all have the same bit 3.
+  const int nb_swz = MS == 16 && !KROWS ? (nb_ch ^ (2 * ((nb_row >> 3) & 1))) : nb_ch;
   auto store_b = [&](const TI(&rb)[BELEMS], int buf) __attribute__((always_inline)) {
     char *dst = Bs0 + buf * B_BYTES;
 #pragma unroll
     for (int i = 0; i < BNV; ++i) {
-      const int off = KROWS ? ((b_row + i * 64) * LDR + b_kv) * (int)sizeof(TI) : (nb_row + i * RPP) * RSB + nb_ch * 16;
+      const int off = KROWS ? ((b_row + i * 64) * LDR + b_kv) * (int)sizeof(TI) : (nb_row + i * RPP) * RSB + nb_swz * 16;
       __builtin_memcpy(dst + off, &rb[i * 8], 16);
     }
   };
   // Transposing fragment reads (natural layout): in each group of 16 lanes,
   // ds_read_b64_tr_b16 gives lane i halfword (i % 4) of the 8 bytes addressed
-  // by lane 4e + i / 4 as its element e.  Lane L points at k row
-  // 16 (L / 32) + (L % 16) / 4 and columns 16 ((L / 16) % 2) + 4 (L % 4) of
-  // the fragment, so it receives column L % 32 at four consecutive k rows;
-  // reads at +8 s2 + 4 hf rows give the k values fkg * 16 + s2 * 8 + 0..7 the
-  // halo fragments hold.
-  const int j16 = lane & 15;
-  const int tr_off = (fkg * 16 + (j16 >> 2)) * RSB + 2 * (16 * ((lane >> 4) & 1) + 4 * (j16 & 3));
+  // by lane 4e + i / 4 as its element e.  Lane i of a group pointing at k row
+  // k0 + i / 4 and columns c0 + 4 (i % 4) .. + 3 therefore receives column
+  // c0 + i at the four rows k0 .. k0 + 3; a second read 4 rows further on
+  // completes the 8 k values a lane's fragment holds.  MS 32: the group of
+  // lanes 16 h .. 16 h + 15 covers columns 16 (h % 2) .., rows 16 (h / 2) +
+  // 8 s2 .. for the K-step's MFMA s2.  MS 16: group h covers the fragment's
+  // 16 columns at rows 8 h ..
+  // MS 16 with the swizzle above: the odd groups (rows with bit 3 set) read
+  // fragment j's columns from block j ^ 1.
+  const int j16 = lane & 15, hodd = (lane >> 4) & 1;
+  const int tr_off = MS == 32 ? (fkg * 16 + (j16 >> 2)) * RSB + 2 * (16 * ((lane >> 4) & 1) + 4 * (j16 & 3))
+                              : (8 * (lane >> 4) + (j16 >> 2)) * RSB + 2 * (4 * (j16 & 3));
+  const int wn0 = wn * FN * MS;  // this wave's first column in the tile (a multiple of 32)
   auto read_b = [&](const char *bbuf, int j, int s2) __attribute__((always_inline)) -> uint4 {
     if constexpr (!KROWS) {
-      const char *base = bbuf + tr_off + (8 * s2) * RSB + 2 * 32 * j;
+      const int jb = MS == 16 ? (j ^ hodd) : j;
+      const char *base = bbuf + tr_off + (8 * s2) * RSB + 2 * (wn0 + MS * jb);
       const v4s16 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s16 *)(base));
       const v4s16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s16 *)(base + 4 * RSB));
       return __builtin_bit_cast(uint4, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
     } else {
-      return ((const uint4 *)(bbuf + ((j * 32 + frow) * LDR + fkg * 16) * (int)sizeof(TI)))[s2];
+      const int kofs = MS == 32 ? fkg * 16 + 8 * s2 : fkg * 8;
+      return *(const uint4 *)(bbuf + ((wn0 + j * MS + frow) * LDR + kofs) * (int)sizeof(TI));
     }
   };
 
-  f32x16 acc[FM][FN];
+  acc_t acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+      for (int r = 0; r < NR; ++r) acc[i][j][r] = 0.0f;
 
   const int nk = cblocks * T;
   // load cursor: (tap, channel block) of the next B tile to fetch.  Past the
@@ -951,7 +1010,18 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     const TI *hcur = halo + (dbl ? (cb & 1) * HBUF * PR : 0);
     const int toff = (ta * g.rh * HC + tc * g.rw) * PR;
     const char *bs = Bs0 + bcur * B_BYTES;
-    if constexpr (FM == 4) {
+    if constexpr (MS == 16) {
+      // one MFMA per fragment pair covers the K-step's 32 channels
+      uint4 fa[FM], fb[FN];
+#pragma unroll
+      for (int f = 0; f < FM; ++f) fa[f] = *(const uint4 *)&hcur[hoff[f] + toff];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) fb[j] = read_b(bs, j, 0);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i], fb[j], acc[i][j]);
+    } else if constexpr (FM == 4) {
       // 128 accumulators leave no room for both K halves' fragments: read
       // and consume one half at a time
 #pragma unroll
@@ -1021,14 +1091,19 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   if (ks < nk) step(ks, rb1);
 
   // ---- epilogue: output pixel (b, i0 + p / Wo, p % Wo), channel n ----
+  // Accumulator r of fragment (i, j) holds pixel prow + rofs(r) and channel
+  // ncol: a 32x32 result gives lane l column l % 32 of rows
+  // 8 (r / 4) + 4 (l / 32) + r % 4, a 16x16 result column l % 16 of rows
+  // 4 (l / 16) + r.
   const uint32_t o_bytes = (uint32_t)((int64_t)g.B * g.Ho * g.Wo * g.N * sizeof(TO));
   const int obase = (b * g.Ho + i0) * g.Wo;  // first output pixel of the tile
+  auto rofs = [](int r) __attribute__((always_inline)) -> int { return MS == 32 ? 8 * (r >> 2) + (r & 3) : r; };
   // fused bias and activation (forward only); relu is fmax(v, 0): NaN -> 0
   const bool epi = PASS == 0 && (g.bias || g.act);
   float bj[FN];
 #pragma unroll
   for (int j = 0; j < FN; ++j) {
-    const int n = n0 + j * 32 + frow;
+    const int n = n0 + wn0 + j * MS + frow;
     bj[j] = (epi && g.bias && n < g.N) ? g.bias[n] : 0.0f;
   }
   auto val = [&](int i, int j, int r, bool act) __attribute__((always_inline)) -> float {
@@ -1041,19 +1116,19 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   };
   // staged output row: BN values, plus 16 B of padding when that still fits
   constexpr int ORS_PAD = BN * (int)sizeof(TO) + 16;
-  constexpr int ORS = FM * 128 * ORS_PAD <= LDS_BYTES ? ORS_PAD : BN * (int)sizeof(TO);
+  constexpr int ORS = CAP * ORS_PAD <= LDS_BYTES ? ORS_PAD : BN * (int)sizeof(TO);
   auto emit = [&](TO *dst, bool act) __attribute__((always_inline)) {
     const __amdgpu_buffer_rsrc_t orr = make_rsrc(dst, o_bytes);
-    if constexpr (FM * 128 * ORS <= LDS_BYTES) {
+    if constexpr (CAP * ORS <= LDS_BYTES) {
       // Stage the tile in LDS as [pixel][channel] rows (the LDS is free after
       // the K loop's last barrier), then store whole 16-byte chunks.
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          char *rowp = smem + (wave * FM * 32 + i * 32 + 4 * fkg) * ORS + (j * 32 + frow) * (int)sizeof(TO);
+          char *rowp = smem + (wm * FM * MS + i * MS + 4 * fkg) * ORS + (wn0 + j * MS + frow) * (int)sizeof(TO);
 #pragma unroll
-          for (int r = 0; r < 16; ++r) *(TO *)(rowp + (8 * (r >> 2) + (r & 3)) * ORS) = (TO)val(i, j, r, act);
+          for (int r = 0; r < NR; ++r) *(TO *)(rowp + rofs(r) * ORS) = (TO)val(i, j, r, act);
         }
       __syncthreads();
       constexpr int CPR = BN * (int)sizeof(TO) / 16;  // 16-byte chunks per row
@@ -1070,11 +1145,11 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          const int n = n0 + j * 32 + frow;
-          const int prow = wave * FM * 32 + i * 32 + 4 * fkg;
+          const int n = n0 + wn0 + j * MS + frow;
+          const int prow = wm * FM * MS + i * MS + 4 * fkg;
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const int pp = prow + 8 * (r >> 2) + (r & 3);
+          for (int r = 0; r < NR; ++r) {
+            const int pp = prow + rofs(r);
             const bool ok = pp < P && n < g.N;
             const uint32_t vo = ok ? (uint32_t)((obase + pp) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
             const float v = val(i, j, r, act);
@@ -1095,11 +1170,30 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   emit(out, true);
 }
 
-// Fragments per wave along the pixels: 4 (128 pixels) for 64-wide channel
-// slices, 2 (64 pixels) for 128-wide ones; either way 128 accumulators.
-static int halo_fm(int N) { return N <= 64 ? 4 : 2; }
+// Pixel capacity of a halo tile: 128-wide channel slices (N > 64) run 4
+// waves of 2 x 4 32-edge fragments (256 pixels) or 2 x 2 waves of 7 x 4
+// 16-edge ones (224 pixels); 64-wide slices 4 waves of 4 x 2 32-edge
+// fragments (512 pixels) or of 7 x 4 16-edge ones (448 pixels).  Either way
+// 112 or 128 accumulators per lane.
+static int halo_cap(int N, int ms) { return N <= 64 ? (ms == 32 ? 512 : 448) : (ms == 32 ? 256 : 224); }
+
+// Whole output rows per tile for a capacity (0 when one row does not fit),
+// and whether their input halo fits the LDS (two 352-pixel buffers).
+static int halo_rows(const HaloGeom &g, int cap) {
+  const int tr = cap / g.Wo;
+  return tr > g.Ho ? g.Ho : tr;
+}
+static bool halo_rows_fit(const HaloGeom &g, int tr) {
+  if (tr < 1) return false;
+  const int HR = (tr - 1) + (g.kh - 1) * g.rh + 1, HC = (g.Wo - 1) + (g.kw - 1) * g.rw + 1;
+  return HR * HC <= 2 * 352;
+}
 
 // Geometry of the halo kernel for a problem, or false if it does not fit.
+// The fragment edge is the one whose tiles pad the output least (ties go to
+// 32): a 56-pixel row fills 16-edge tiles exactly (224 = 4 rows, 448 = 8),
+// while 32-edge tiles leave 12.5% of their MFMA work on padding.
+// OWL_AMD_HALO_MS = 16 or 32 forces an edge where it fits (tests, tuning).
 template <int PASS>
 static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom &g) {
   if (in_bytes != 2) return false;
@@ -1119,16 +1213,26 @@ static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom
   g.bias = PASS == 0 ? (const float *)p.bias : nullptr;
   g.pre = PASS == 0 ? p.pre : nullptr;
   g.act = PASS == 0 ? p.act : OWL_AMD_ACT_NONE;
-  if (g.C % 32 != 0 || g.N % 8 != 0 || g.Wo > 256) return false;
-  // 128 pixels per wave (FM = 4) when the channel slice is 64 wide, else 64
-  g.TR = halo_fm(g.N) * 128 / g.Wo;
-  if (g.TR > g.Ho) g.TR = g.Ho;
-  const int HR = (g.TR - 1) + (g.kh - 1) * g.rh + 1, HC = (g.Wo - 1) + (g.kw - 1) * g.rw + 1;
-  if (HR * HC > 2 * 352) return false;  // halo must fit its LDS (two 352-pixel buffers)
+  if (g.C % 32 != 0 || g.N % 8 != 0 || g.Wo > 256 || g.Ho < 1) return false;
   if ((int64_t)g.B * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * out_bytes >= (1LL << 31) ||
       (int64_t)g.kh * g.kw * g.C * g.N * 2 >= (1LL << 31))
     return false;  // 32-bit buffer offsets
-  return true;
+  const char *fe = getenv("OWL_AMD_HALO_MS");
+  const int forced = fe ? atoi(fe) : 0;
+  g.ms = 0;
+  double best = -1.0;
+  for (int ms : {32, 16}) {
+    const int cap = halo_cap(g.N, ms), tr = halo_rows(g, cap);
+    if (!halo_rows_fit(g, tr)) continue;
+    const double use = (double)g.Ho * g.Wo / ((double)((g.Ho + tr - 1) / tr) * cap);
+    const bool take = forced == 16 || forced == 32 ? ms == forced || g.ms == 0 : use > best + 1e-9;
+    if (take) {
+      g.ms = ms;
+      g.TR = tr;
+      best = use;
+    }
+  }
+  return g.ms != 0;
 }
 
 bool halo_fits(const ConvParams &p, int pass, int in_bytes, int out_bytes) {
@@ -1143,16 +1247,21 @@ template <int PASS, typename TI, typename TO>
 bool run_halo(const ConvParams &p, hipStream_t st) {
   HaloGeom g;
   if (!halo_geom<PASS>(p, (int)sizeof(TI), (int)sizeof(TO), g)) return false;
-  const int FN = g.N <= 64 ? 2 : 4;
-  const int BN = FN * 32;  // FM = halo_fm(N): 4 with FN 2, 2 with FN 4
+  const int BN = g.N <= 64 ? 64 : 128;
   const int64_t blocks = (int64_t)g.B * ((g.Ho + g.TR - 1) / g.TR) * ((g.N + BN - 1) / BN);
   const TI *in = (const TI *)p.a;
   const TI *w = (const TI *)p.b;
+  TO *out = (TO *)p.c;
   constexpr bool KROWS = PASS == 1;
   dim3 grid((unsigned)blocks), blk(256);
   KTimer kt(PASS, "conv_halo_kernel", st);
-  if (FN == 2) launch_k((conv_halo_kernel<PASS, TI, TO, 4, 2, KROWS>), grid, blk, st, g, in, w, (TO *)p.c);
-  else launch_k((conv_halo_kernel<PASS, TI, TO, 2, 4, KROWS>), grid, blk, st, g, in, w, (TO *)p.c);
+  if (g.ms == 16) {
+    if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 1, KROWS>), grid, blk, st, g, in, w, out);
+    else launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 2, KROWS>), grid, blk, st, g, in, w, out);
+  } else {
+    if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 32, 4, 2, 1, KROWS>), grid, blk, st, g, in, w, out);
+    else launch_k((conv_halo_kernel<PASS, TI, TO, 32, 2, 4, 1, KROWS>), grid, blk, st, g, in, w, out);
+  }
   return true;
 }
 #define OWL_HALO(PASS, TI, TO) template bool run_halo<PASS, TI, TO>(const ConvParams &, hipStream_t);

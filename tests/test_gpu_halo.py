@@ -9,7 +9,15 @@ blocks.  Also 1x1 kernels (one tap per block), dilation, VALID padding and
 output channels that do not fill a tile (the forward reads the weights in
 place with transposing LDS loads; columns past the last channel read as
 zeros).  FP32 output: rel-L2 < 1e-5 (the MFMA accumulation is the only
-error); BF16 output: rel-L2 < 1e-2 end to end."""
+error); BF16 output: rel-L2 < 1e-2 end to end.
+
+Each case runs with the fragment edge the library picks and with each edge
+forced (OWL_AMD_HALO_MS): 32-edge tiles (256 / 512 pixels, 32x32x16 MFMAs)
+and 16-edge tiles (224 / 448 pixels, 16x16x32 MFMAs, 2 x 2 waves for
+128-wide channel slices), including short last row tiles, a partial second
+channel slice and the backward input with 128 input channels."""
+import os
+
 import numpy as np
 import pytest
 
@@ -31,6 +39,9 @@ CASES = [
     ([1, 7, 7, 256], [1, 1, 256, 72], 1, VALID),     # one tap, 8 blocks
     ([3, 14, 14, 128], [3, 3, 128, 128], 1, SAME),   # whole image per tile
     ([2, 8, 30, 32], [3, 3, 32, 200], 1, SAME),      # 128-wide slices, the last one partial
+    ([3, 13, 56, 64], [3, 3, 64, 128], 1, SAME),     # row tiles 4,4,4,1 (forward), 8,5 (backward input)
+    ([2, 11, 30, 64], [3, 3, 64, 200], 1, SAME),     # two 128-wide slices, the second 72 wide
+    ([4, 14, 14, 128], [3, 3, 128, 256], 1, SAME),   # backward input with 128 input channels
 ]
 
 
@@ -50,11 +61,26 @@ def _run(x, w, g, rate, pad, td, out_dtype):
     return y.double().cpu().numpy(), dx.double().cpu().numpy()
 
 
+@pytest.fixture(params=["auto", "16", "32"])
+def halo_edge(request):
+    old = os.environ.get("OWL_AMD_HALO_MS")
+    if request.param == "auto":
+        os.environ.pop("OWL_AMD_HALO_MS", None)
+    else:
+        os.environ["OWL_AMD_HALO_MS"] = request.param
+    yield request.param
+    if old is None:
+        os.environ.pop("OWL_AMD_HALO_MS", None)
+    else:
+        os.environ["OWL_AMD_HALO_MS"] = old
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", range(len(CASES)))
 @pytest.mark.parametrize("dtype", ["bf16", "f16"])
-def test_halo_vs_oracle(case, dtype):
+def test_halo_vs_oracle(case, dtype, halo_edge):
     xs, ks, rate, pad = CASES[case]
+    print(f"halo case {case} {dtype} edge {halo_edge} x{xs} w{ks}", flush=True)
     td = torch.bfloat16 if dtype == "bf16" else torch.float16
     rng = np.random.default_rng(100 + case)
     x = rng.standard_normal(xs)
