@@ -780,8 +780,21 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   constexpr int NR = MS == 32 ? 16 : 4;  // accumulators per lane and fragment
   constexpr int WGM = 4 / WGN;
   constexpr int CAP = WGM * FM * MS;     // pixels per tile
-  constexpr int BK = 32, BN = WGN * FN * MS, PR = BK + 8;  // halo pixel row: 32 channels + 16 B pad
-  constexpr int LDR = BK + 8;
+  // Halo pixels and [n][k] kernel rows: 32-edge fragments use 80-byte rows
+  // (32 channels + 16 B pad), which their reads (32 consecutive pixels per
+  // half-wave) hit without bank conflicts.  16-edge fragments read 16 pixels
+  // x 4 chunks per instruction, which conflicts on padded rows (two LDS
+  // cycles per read, three across a row end, measured on gfx950).  With
+  // 64-byte rows and chunk c of pixel (or row) p at 16-byte slot
+  // c ^ ((p >> 1) & 3) those reads are conflict-free at any tap shift, and
+  // so are the 4-chunks-per-pixel writes, but each A read then needs its
+  // address rebuilt per tap (five VALU operations).  That paid off only for
+  // the backward input (38.8-39.4 against 39.1-40.0 us on a 64x56x56x64 ->
+  // 128 layer; its B rows need no per-tap work), while the forward ran
+  // slower (44.0 against 42.0 us), so only the KROWS instance swizzles.
+  constexpr bool SWZ = MS == 16 && KROWS;
+  constexpr int BK = 32, BN = WGN * FN * MS, PR = SWZ ? BK : BK + 8;
+  constexpr int LDR = PR;
   // Two halo buffers of HBUF pixels: while one channel block's taps run,
   // the next block's halo is loaded into registers (HQ 16-byte chunks per
   // thread) and written to the other buffer.  A halo above HBUF pixels uses
@@ -794,7 +807,11 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   // the four rows one transposing read phase touches cover all 64 banks
   constexpr int RSB = BN * (int)sizeof(TI) + 64;
   constexpr int B_BYTES = KROWS ? BN * LDR * (int)sizeof(TI) : BK * RSB;
-  constexpr int LDS_BYTES = HALO_BYTES + 2 * B_BYTES;
+  // 16-bit outputs are staged for the 16-byte store epilogue in the same
+  // block, which is sized for whichever use is larger
+  constexpr int K_BYTES = HALO_BYTES + 2 * B_BYTES;
+  constexpr int STG_BYTES = CAP * (BN * (int)sizeof(TO) + 16);
+  constexpr int LDS_BYTES = sizeof(TO) == 2 && STG_BYTES > K_BYTES ? STG_BYTES : K_BYTES;
   __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES];
   TI *halo = (TI *)smem;
   char *const Bs0 = smem + HALO_BYTES;  // kernel tile buffer i at Bs0 + i * B_BYTES
@@ -825,7 +842,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     int p = wm * FM * MS + f * MS + frow;
     if (p >= P) p = 0;
     const int ti = p / g.Wo, tj = p - (p / g.Wo) * g.Wo;
-    hoff[f] = (ti * HC + tj) * PR + fkg * (MS == 32 ? 16 : 8);
+    // 16-edge: the halo pixel index (the swizzle needs the tap's pixel)
+    hoff[f] = SWZ ? ti * HC + tj : (ti * HC + tj) * PR + fkg * (MS == 32 ? 16 : 8);
   }
 
   const uint32_t in_bytes = (uint32_t)((int64_t)g.B * g.Hi * g.Wi * g.C * 2);
@@ -833,6 +851,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   const int T = g.kh * g.kw;
   const __amdgpu_buffer_rsrc_t wr = make_rsrc(wt, (uint32_t)(T * g.N * g.C * 2));
   const int cblocks = g.C / BK;
+  // 16-byte slot XOR of pixel / kernel row p (0 for 80-byte rows)
+  auto swz = [](int p) __attribute__((always_inline)) -> int { return SWZ ? (p >> 1) & 3 : 0; };
 
   // halo staging: chunk q -> pixel q / 4, 16-byte channel chunk q % 4;
   // OOB marks padding, which the buffer loads fill with zeros.
@@ -876,7 +896,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
     for (int u = 0; u < HQ; ++u) {
       const int q = tid + u * 256;
-      if (q < hq_total) *(uint4 *)&hb[(q >> 2) * PR + (q & 3) * 8] = hreg[u];
+      if (q < hq_total) *(uint4 *)&hb[(q >> 2) * PR + ((q & 3) ^ swz(q >> 2)) * 8] = hreg[u];
     }
   };
   auto stage_halo = [&](TI *hb, int c0) __attribute__((always_inline)) {
@@ -884,7 +904,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     halo_commit(hb);
     for (int q = HQ * 256 + tid; q < hq_total; q += 256) {
       const uint32_t so = chunk_src(q);
-      *(uint4 *)&hb[(q >> 2) * PR + (q & 3) * 8] = bload(inr, so == OOB ? OOB : so + (uint32_t)c0 * 2u);
+      *(uint4 *)&hb[(q >> 2) * PR + ((q & 3) ^ swz(q >> 2)) * 8] = bload(inr, so == OOB ? OOB : so + (uint32_t)c0 * 2u);
     }
   };
 
@@ -930,7 +950,9 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     char *dst = Bs0 + buf * B_BYTES;
 #pragma unroll
     for (int i = 0; i < BNV; ++i) {
-      const int off = KROWS ? ((b_row + i * 64) * LDR + b_kv) * (int)sizeof(TI) : (nb_row + i * RPP) * RSB + nb_swz * 16;
+      // KROWS rows b_row + 64 i share bits 1-2, so one slot XOR per thread
+      const int off = KROWS ? ((b_row + i * 64) * LDR + ((b_kv >> 3) ^ swz(b_row)) * 8) * (int)sizeof(TI)
+                            : (nb_row + i * RPP) * RSB + nb_swz * 16;
       __builtin_memcpy(dst + off, &rb[i * 8], 16);
     }
   };
@@ -957,7 +979,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       const v4s16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s16 *)(base + 4 * RSB));
       return __builtin_bit_cast(uint4, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
     } else {
-      const int kofs = MS == 32 ? fkg * 16 + 8 * s2 : fkg * 8;
+      // 16-edge rows wn0 + 16 j + frow: bits 1-2 come from frow
+      const int kofs = MS == 32 ? fkg * 16 + 8 * s2 : 8 * (fkg ^ swz(frow));
       return *(const uint4 *)(bbuf + ((wn0 + j * MS + frow) * LDR + kofs) * (int)sizeof(TI));
     }
   };
@@ -1008,13 +1031,21 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     if (pf && tap == 0 && cb + 1 < cblocks) halo_issue((cb + 1) * BK);
     // A from the halo at this tap's offset, B from LDS
     const TI *hcur = halo + (dbl ? (cb & 1) * HBUF * PR : 0);
-    const int toff = (ta * g.rh * HC + tc * g.rw) * PR;
+    const int tsh = ta * g.rh * HC + tc * g.rw;  // the tap's pixel shift in the halo
+    const int toff = tsh * PR;
     const char *bs = Bs0 + bcur * B_BYTES;
     if constexpr (MS == 16) {
       // one MFMA per fragment pair covers the K-step's 32 channels
       uint4 fa[FM], fb[FN];
 #pragma unroll
-      for (int f = 0; f < FM; ++f) fa[f] = *(const uint4 *)&hcur[hoff[f] + toff];
+      for (int f = 0; f < FM; ++f) {
+        if constexpr (SWZ) {
+          const int pix = hoff[f] + tsh;
+          fa[f] = *(const uint4 *)&hcur[pix * PR + 8 * (fkg ^ swz(pix))];
+        } else {
+          fa[f] = *(const uint4 *)&hcur[hoff[f] + toff];
+        }
+      }
 #pragma unroll
       for (int j = 0; j < FN; ++j) fb[j] = read_b(bs, j, 0);
 #pragma unroll
