@@ -32,9 +32,9 @@ SPINUP_S = 0.25  # untimed work before each timed region, after the W warmup ste
 # for a plain copy of known size (x2.0 and x1.0); the runs and their command
 # are in profiles/conv2d_vgg_layer_bf16_hbm_traffic.md
 TRAFFIC = {
-    ("conv_halo_kernel", "fwd"): 13764.5 * 2048 + 50176.0 * 1024,
-    ("conv_halo_kernel", "bwd_input"): 28525.9 * 2048 + 32256.0 * 1024,
-    ("conv_wgrad_kernel", "bwd_kernel"): 112590.0 * 2048 + 48960.0 * 1024,
+    ("conv_halo_kernel", "fwd"): 13765.6 * 2048 + 50176.0 * 1024,
+    ("conv_halo_kernel", "bwd_input"): 26483.9 * 2048 + 27328.0 * 1024,
+    ("conv_wgrad_kernel", "bwd_kernel"): 112598.2 * 2048 + 48960.0 * 1024,
 }
 
 
@@ -259,6 +259,27 @@ def main():
     kernels = {k: {"kernel": v["kernel"], "avg_ms": round(v["avg_ms"], 4), "launches": v["launches"]}
                for k, v in kt_b.items()}
 
+    # the Owl drop-in path: FP32 host (numpy, pageable) buffers through the
+    # float32 stubs, each call copying its operands to the device, running
+    # the same kernels and copying the result back.  Reported beside the
+    # device-resident value, never as it.
+    xh, wh, dyh = x32.cpu().numpy(), w32.cpu().numpy(), dy32.cpu().numpy()
+    host_step = lambda: (nd.conv2d(xh, wh, [1, 1], SAME), nd.conv2d_backward_input(xh, wh, [1, 1], dyh),
+                         nd.conv2d_backward_kernel(xh, wh, [1, 1], dyh))
+    host_step()
+    th = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        host_step()
+        th.append(time.perf_counter() - t0)
+    th_med = statistics.median(th)
+    pcie_bytes = 4 * (3 * B * H * W * CI + 3 * B * H * W * CO + 3 * K * K * CI * CO)  # per step, both directions
+    dropin = {"step_ms": round(th_med * 1e3, 2), "tflops": round(3 * FLOP_PASS / th_med / 1e12, 2),
+              "host_device_bytes_per_step": pcie_bytes,
+              "what": "Owl float32 drop-ins on pageable host buffers (x, w, dy in; y, dx, dw out): H2D copies + "
+                      "FP32 kernels + D2H copies per call, median of 5 steps"}
+    del xh, wh, dyh
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline()
@@ -290,6 +311,7 @@ def main():
         "pool_max2x2s2": pool,
         "conv_bias_relu_bf16": fused,
         "cgraph_vgg_block_bf16": cgraph_block,
+        "dropin_fp32_host_buffers": dropin,
         "kernels_bf16": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
