@@ -408,16 +408,19 @@ void plan_splitk(const ConvParams &p, int in_t, int *split, int64_t *kps) {
   const int bm = w16 ? wgrad_tile_m(p.M) : tile_m(in_t), bn = w16 ? 128 : tile_m(in_t), bk = tile_k(in_t);
   int64_t tiles = ((p.M + bm - 1) / bm) * (int64_t)((p.N + bn - 1) / bn);
   static const int env_target = env_int("OWL_AMD_SPLIT_TARGET", 0);  // workgroups to aim for (tuning)
-  // three 128-row wgrad workgroups fit a CU, two of the 192-row ones
-  const int target = env_target > 0 ? env_target : (w16 && bm == 128 ? 768 : 512);
-  int64_t want = (target + tiles - 1) / tiles;
-  int64_t max_split = (p.K + 16 * bk - 1) / (16 * bk);
+  // three 128-row wgrad workgroups fit a CU, two of the 192-row ones; with
+  // two wave groups per workgroup (each summing half a split) one 512-thread
+  // workgroup per CU, 256 of them, whole ones only
+  const int kg = w16 ? wgrad_kgroups() : 1;
+  const int target = env_target > 0 ? env_target : (kg == 2 ? 256 : (w16 && bm == 128 ? 768 : 512));
+  int64_t want = kg == 2 ? target / tiles : (target + tiles - 1) / tiles;
+  int64_t max_split = (p.K + 16 * kg * bk - 1) / (16 * kg * bk);
   static const int cap = env_int("OWL_AMD_MAX_SPLIT", 1 << 30);  // diagnostics / tuning
   if (max_split > cap) max_split = cap;
   if (want > max_split) want = max_split;
   if (want < 1) want = 1;
   int64_t k = (p.K + want - 1) / want;
-  k = (k + bk - 1) / bk * bk;
+  k = (k + kg * bk - 1) / (kg * bk) * (kg * bk);  // halves of whole K-steps
   *kps = k;
   *split = (int)((p.K + k - 1) / k);
 }

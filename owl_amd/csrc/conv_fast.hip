@@ -410,18 +410,23 @@ template <> __device__ __forceinline__ f32x16 mfma16<f16_t>(const v4s16 &a0, con
 // FMW: 32-row fragments per wave along M; the tile is BM = 64 FMW rows
 // (2 x 2 waves of 32 FMW x 64).  D1: a 2-D problem (D = Do = kd = 1, no
 // depth pad), whose depth digit is always 0: the loader skips its bounds
-// check and carry.
-template <typename TI, typename TO, int FMW, bool D1>
-__global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
+// check and carry.  KG: groups of four waves per workgroup.  With KG = 2 the
+// two groups sum the two halves of the split's pixels into the same tile,
+// each with its own LDS stages, and add their sums through LDS at the end
+// (first half + second half): half as many FP32 partials for the reduce, at
+// one 512-thread workgroup per CU instead of two of 256.
+template <typename TI, typename TO, int FMW, bool D1, int KG>
+__global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(ConvParams p) {
   // BK 64 (80 KB of LDS) ran slower than BK 32: one workgroup per CU
   constexpr int BM = 64 * FMW, BN = 128, BK = 32, KP = BK / 32;  // KP pixels per loader thread
   constexpr int QA = BM / 64;      // A chunks per loader thread and pixel
   constexpr int RSA = BM * 2 + 64;  // LDS row strides in bytes (16-bit values + 64 bytes pad)
   constexpr int RSB = BN * 2 + 64;
   constexpr int STAGE = BK * (RSA + RSB);  // A rows, then B rows
-  __shared__ __attribute__((aligned(16))) char lds[2][STAGE];  // 40 KB (FMW 2), 48 KB (FMW 3)
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ __attribute__((aligned(16))) char lds_all[KG][2][STAGE];  // 40 KB (FMW 2), 48 KB (FMW 3) per group
+  // tid / wave: within the group
+  const int tid = threadIdx.x & 255, grp = threadIdx.x >> 8, lane = tid & 63, wave = tid >> 6;
+  char(*lds)[STAGE] = lds_all[grp];
   const int wm = wave >> 1, wn = wave & 1;
   const int M = (int)p.M, N = p.N;
   const int n_tiles = (N + BN - 1) / BN;
@@ -433,10 +438,17 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   const int lid = blockIdx.x;
   const int tile = lid % mn_tiles, split_idx = lid / mn_tiles;
   const int m0 = (tile / n_tiles) * BM, n0 = (tile % n_tiles) * BN;
-  const int64_t kbeg = (int64_t)split_idx * p.k_per_split;
-  int64_t kend = kbeg + p.k_per_split;
-  if (kend > p.K) kend = p.K;
-  const int nk = (int)((kend - kbeg + BK - 1) / BK);
+  // this group's pixels: the whole split (KG 1) or one half of it (KG 2,
+  // k_per_split a multiple of 2 BK); both groups run the first half's step
+  // count, the second half's loads past its end read zeros
+  const int64_t sbeg = (int64_t)split_idx * p.k_per_split;
+  const int64_t send = sbeg + p.k_per_split < p.K ? sbeg + p.k_per_split : p.K;
+  const int64_t half = p.k_per_split / KG;
+  const int64_t kbeg = sbeg + grp * half;
+  int64_t kend = kbeg + half < send ? kbeg + half : send;
+  if (kend < kbeg) kend = kbeg;
+  const int64_t len0 = half < send - sbeg ? half : send - sbeg;
+  const int nk = (int)((len0 + BK - 1) / BK);
 
   const uint32_t x_bytes = (uint32_t)((int64_t)p.B * p.H * p.W * p.D * p.Ci * 2);
   const uint32_t g_bytes = (uint32_t)((int64_t)p.K * p.Co * 2);
@@ -631,6 +643,42 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
     }
   }
 
+  // ---- KG 2: add the halves.  Group 0 keeps the j = 0 fragments and group
+  // 1 the j = 1 ones; each hands the other half over through LDS (the
+  // stages are free after the loop's last barrier), and every sum is
+  // (first half) + (second half).
+  if constexpr (KG == 2) {
+    __syncthreads();
+    float *X = (float *)&lds_all[0][0][0];  // two regions of FMW * 16 * 256 floats
+    constexpr int REG = FMW * 16 * 256;
+    static_assert(2 * REG * 4 <= KG * 2 * STAGE, "exchange fits the stages");
+    float *mine = X + grp * REG;
+    const float *theirs = X + (1 - grp) * REG;
+    if (grp == 0) {
+#pragma unroll
+      for (int i = 0; i < FMW; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mine[(i * 16 + r) * 256 + tid] = acc[i][1][r];
+    } else {
+#pragma unroll
+      for (int i = 0; i < FMW; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mine[(i * 16 + r) * 256 + tid] = acc[i][0][r];
+    }
+    __syncthreads();
+    if (grp == 0) {
+#pragma unroll
+      for (int i = 0; i < FMW; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][0][r] = acc[i][0][r] + theirs[(i * 16 + r) * 256 + tid];
+    } else {
+#pragma unroll
+      for (int i = 0; i < FMW; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][1][r] = theirs[(i * 16 + r) * 256 + tid] + acc[i][1][r];
+    }
+  }
+
   // ---- epilogue (same layout as the other MFMA kernels) ----
   const int frow = lane & 31, fkg = lane >> 5;
   const bool partial = p.split > 1;
@@ -641,6 +689,7 @@ __global__ __launch_bounds__(256, 2) void conv_wgrad_kernel(ConvParams p) {
   for (int i = 0; i < FMW; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
+      if (KG == 2 && j != grp) continue;  // the other group stores it
       const int n = n0 + wn * 64 + j * 32 + frow;
       const int mrow = m0 + wm * 32 * FMW + i * 32 + 4 * fkg;
 #pragma unroll
@@ -677,17 +726,35 @@ int wgrad_tile_m(int64_t M) {
   return (M + 191) / 192 * 192 <= (M + 127) / 128 * 128 ? 192 : 128;
 }
 
+// Groups of four waves per weight-gradient workgroup: 2 (default), or 1 with
+// OWL_AMD_WGRAD_KG=1 (two 256-thread workgroups per CU, each summing a whole
+// split).  Read on every call; the split-K plan, and so the workspace size,
+// depends on it.
+int wgrad_kgroups() {
+  const char *e = getenv("OWL_AMD_WGRAD_KG");
+  return e && atoi(e) == 1 ? 1 : 2;
+}
+
 template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st) {
   const int bm = wgrad_tile_m(p.M);
-  dim3 grid((unsigned)(((p.M + bm - 1) / bm) * ((p.N + 127) / 128) * p.split));
+  const int kg = p.split > 1 ? wgrad_kgroups() : 1;
+  dim3 grid((unsigned)(((p.M + bm - 1) / bm) * ((p.N + 127) / 128) * p.split)), blk(256 * kg);
   const bool d1 = p.D == 1 && p.Do == 1 && p.kd == 1 && p.pd == 0;
-  if (bm == 192) {
-    if (d1) launch_k((conv_wgrad_kernel<TI, TO, 3, true>), grid, dim3(256), st, p);
-    else launch_k((conv_wgrad_kernel<TI, TO, 3, false>), grid, dim3(256), st, p);
+  if (kg == 2) {
+    if (bm == 192) {
+      if (d1) launch_k((conv_wgrad_kernel<TI, TO, 3, true, 2>), grid, blk, st, p);
+      else launch_k((conv_wgrad_kernel<TI, TO, 3, false, 2>), grid, blk, st, p);
+    } else {
+      if (d1) launch_k((conv_wgrad_kernel<TI, TO, 2, true, 2>), grid, blk, st, p);
+      else launch_k((conv_wgrad_kernel<TI, TO, 2, false, 2>), grid, blk, st, p);
+    }
+  } else if (bm == 192) {
+    if (d1) launch_k((conv_wgrad_kernel<TI, TO, 3, true, 1>), grid, blk, st, p);
+    else launch_k((conv_wgrad_kernel<TI, TO, 3, false, 1>), grid, blk, st, p);
   } else {
-    if (d1) launch_k((conv_wgrad_kernel<TI, TO, 2, true>), grid, dim3(256), st, p);
-    else launch_k((conv_wgrad_kernel<TI, TO, 2, false>), grid, dim3(256), st, p);
+    if (d1) launch_k((conv_wgrad_kernel<TI, TO, 2, true, 1>), grid, blk, st, p);
+    else launch_k((conv_wgrad_kernel<TI, TO, 2, false, 1>), grid, blk, st, p);
   }
 }
 template void run_wgrad<bf16_t, bf16_t>(const ConvParams &, hipStream_t);

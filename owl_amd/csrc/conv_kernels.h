@@ -65,6 +65,9 @@ bool halo_fits(const ConvParams &p, int pass, int in_bytes, int out_bytes);
 // conv_fast.hip: backward kernel for 16-bit storage via transposing LDS reads
 // (tile 128 x 128 x 32 pixels; p.split / p.k_per_split as for run_igemm<2>).
 int wgrad_tile_m(int64_t M);
+// Groups of four waves per weight-gradient workgroup (2, or 1 with
+// OWL_AMD_WGRAD_KG=1): with 2 each group sums half of a split's pixels.
+int wgrad_kgroups();
 template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st);
 template <typename TI> int tile_bm();
