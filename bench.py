@@ -151,6 +151,34 @@ def main():
         "layer": "BF16 relu(conv2d(x, w) + b), x[64,56,56,64] w[3,3,64,128]",
     }
 
+    # the FP8 e4m3 forward of the same layer (per-tensor amax scales, FP8
+    # operands resident, BF16 output), quantisation outside the timed loop
+    import ctypes
+    xq = torch.empty(xb.numel(), dtype=torch.uint8, device=dev)
+    wq = torch.empty(wb.numel(), dtype=torch.uint8, device=dev)
+    sx = torch.empty(1, dtype=torch.float32, device=dev)
+    sw = torch.empty(1, dtype=torch.float32, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for t, q, sc in ((xb, xq, sx), (wb, wq, sw)):
+        _lib.check(_lib.lib().owl_amd_fp8_quantize(_lib.BF16, ctypes.c_void_p(t.data_ptr()), t.numel(),
+                                                   ctypes.c_void_p(q.data_ptr()), ctypes.c_void_p(sc.data_ptr()),
+                                                   stream))
+    y8 = torch.empty(B, H, W, CO, dtype=torch.bfloat16, device=dev)
+    g8 = nd._Conv(B, [H, W], CI, [K, K], CO, [H, W], [1, 1], [1, 1], [1, 1]).desc()
+    fp8_call = lambda: _lib.check(_lib.lib().owl_amd_conv_fwd_fp8(
+        ctypes.byref(g8), _lib.BF16, ctypes.c_void_p(xq.data_ptr()), ctypes.c_void_p(wq.data_ptr()),
+        ctypes.c_void_p(sx.data_ptr()), ctypes.c_void_p(sw.data_ptr()), ctypes.c_void_p(y8.data_ptr()), stream))
+    t8 = t_loop(fp8_call)
+    y8_rel = float(torch.linalg.vector_norm(y8[:2].double() - nd.conv2d(x32[:2].contiguous(), w32, [1, 1], SAME)
+                                            .double()) / torch.linalg.vector_norm(
+        nd.conv2d(x32[:2].contiguous(), w32, [1, 1], SAME).double()))
+    fp8 = {"fwd_us": round(t8, 1), "tflops": round(FLOP_PASS / (t8 * 1e-6) / 1e12, 1),
+           "frac_of_fp8_peak": round(FLOP_PASS / (t8 * 1e-6) / 5.0e15, 4),
+           "rel_l2_vs_fp32_conv_2_images": round(y8_rel, 4),
+           "what": "FP8 e4m3 forward of the VGG layer (generic implicit-GEMM kernel, f8f6f4 MFMA), "
+                   "quantised operands resident, BF16 output"}
+    del xq, wq, y8
+
     # a VGG block as a computation graph (BF16): conv + bias + relu twice,
     # then a 2x2 max-pool; eager unfused ndarray calls, the optimised graph
     # (both layers fused) evaluated eagerly, and its HIP-graph replay
@@ -312,6 +340,7 @@ def main():
         "conv_bias_relu_bf16": fused,
         "cgraph_vgg_block_bf16": cgraph_block,
         "dropin_fp32_host_buffers": dropin,
+        "fp8_e4m3_fwd": fp8,
         "kernels_bf16": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
