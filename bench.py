@@ -175,8 +175,8 @@ def main():
     fp8 = {"fwd_us": round(t8, 1), "tflops": round(FLOP_PASS / (t8 * 1e-6) / 1e12, 1),
            "frac_of_fp8_peak": round(FLOP_PASS / (t8 * 1e-6) / 5.0e15, 4),
            "rel_l2_vs_fp32_conv_2_images": round(y8_rel, 4),
-           "what": "FP8 e4m3 forward of the VGG layer (generic implicit-GEMM kernel, f8f6f4 MFMA), "
-                   "quantised operands resident, BF16 output"}
+           "what": "FP8 e4m3 forward of the VGG layer (halo kernel, 32x32x64 f8f6f4 MFMA, kernel repacked to "
+                   "[tap][co][ci] per call, included), quantised operands resident, BF16 output"}
     del xq, wq, y8
 
     # a VGG block as a computation graph (BF16): conv + bias + relu twice,

@@ -184,6 +184,7 @@ struct DeviceState {
   DevBuf ws;                     // split-K workspace
   DevBuf red;                    // bias-gradient partial sums
   DevBuf cplx;                   // complex convolutions: expanded kernel / real weight gradient
+  DevBuf f8w;                    // FP8 halo forward: kernel repacked to [tap][co][ci]
   DevBuf slot[4];                // drop-in staging buffers
 };
 
@@ -480,6 +481,16 @@ owl_amd_conv_desc widen(const owl_amd_conv_desc *d) {
   r.Ci *= 2;
   r.Co *= 2;
   return r;
+}
+int f8w_scratch(size_t bytes, void **out, hipStream_t st) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  DeviceState *s;
+  int rc = ensure_state(&s);
+  if (rc) return rc;
+  rc = ensure_buf(s->f8w, bytes, st);
+  if (rc) return rc;
+  *out = s->f8w.ptr;
+  return OWL_AMD_OK;
 }
 int cplx_scratch(size_t bytes, void **out, hipStream_t st) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -812,6 +823,22 @@ extern "C" int owl_amd_conv_fwd_fp8(const owl_amd_conv_desc *d, int out_t, const
   p.scale_b = w_scale;
   bool aligned = p.Ci % 16 == 0 && p.Co % 16 == 0 && aligned16(xq, wq, nullptr);
   hipStream_t st = (hipStream_t)stream;
+  // 2-D stride-1 layers with Ci % 64 == 0 and Co % 8 == 0:
+  // the halo kernel with FP8 MFMAs, on the kernel repacked to [tap][co][ci]
+  // in a per-device buffer (calls that overlap on one device race on it)
+  if (fast_path_enabled() && halo_enabled() && (out_t == OWL_AMD_F32 || out_t == OWL_AMD_BF16) &&
+      aligned16(xq, wq, y) && halo_fits_fp8(p, (int)dtype_size(out_t))) {
+    const int64_t T = (int64_t)p.kh * p.kw * p.kd;
+    void *wp = nullptr;
+    rc = f8w_scratch((size_t)(T * p.Ci * p.Co), &wp, st);
+    if (rc) return rc;
+    run_fp8_pack_krows((const unsigned char *)wq, (unsigned char *)wp, T, p.Ci, p.Co, st);
+    const bool ok = out_t == OWL_AMD_F32 ? run_halo_fp8<float>(p, wp, st) : run_halo_fp8<bf16_t>(p, wp, st);
+    if (ok) {
+      OWL_CHECK_HIP(hipGetLastError());
+      return OWL_AMD_OK;
+    }
+  }
   KTimer kt(OWL_AMD_KT_FWD, "conv_igemm_kernel", st);
   if (out_t == OWL_AMD_F32) run_igemm<0, fp8e4m3_t, float>(p, aligned, st);
   else if (out_t == OWL_AMD_BF16) run_igemm<0, fp8e4m3_t, bf16_t>(p, aligned, st);

@@ -57,6 +57,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, ui
 constexpr uint32_t OOB = 0x80000000u;  // >= every num_records used here (tensors < 2 GiB)
 
 typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint4 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
   return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 0, 0));
 }
@@ -793,6 +794,8 @@ struct HaloGeom {
   const float *bias;
   void *pre;
   int act;
+  // FP8 forward: per-tensor scales of x and w (device floats), else null
+  const float *sa, *sb;
 };
 
 // MFMA of the halo kernel by fragment edge MS: 32 -> v_mfma_f32_32x32x16
@@ -839,9 +842,21 @@ template <> struct HaloMfma<f16_t, 16> {
 // [tap][k][n]) the tile is copied in that natural layout and the fragments
 // are read with the transposing LDS load (ds_read_b64_tr_b16, as in the
 // weight-gradient kernel), so no pass rearranges the weights first.
-template <int PASS, typename TI, typename TO, int MS, int FM, int FN, int WGN, bool KROWS>
+// F8: FP8 e4m3 forward (PASS 0, MS 32, KROWS).  The operands are bytes, and
+// every offset is counted in pairs of them, so a K-step of 32 "channels" is
+// 64 FP8 channels (64 bytes per halo pixel, as for 32 16-bit channels) and
+// each lane's two 16-byte A and B chunks are exactly the 32 bytes one
+// v_mfma_scale_f32_32x32x64_f8f6f4 takes: the two 32x32x16 MFMAs of a step
+// become one.  A and B give each lane group the same bytes in the same
+// order, so the products pair up whatever order the instruction sums them
+// in.  The per-tensor scales multiply the sums in the epilogue.  Waves hold
+// 2 x 2 fragments (tiles of 256 pixels x 64 channels): the scaled FP8 MFMA
+// keeps its result apart from its accumulator input, and 2 x 4 fragments
+// then spilled about 400 bytes per lane.
+template <int PASS, typename TI, typename TO, int MS, int FM, int FN, int WGN, bool KROWS, bool F8 = false>
 __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI *__restrict__ in,
                                                            const TI *__restrict__ wt, TO *__restrict__ out) {
+  static_assert(!F8 || (PASS == 0 && MS == 32 && KROWS && FM == 2 && FN == 2), "FP8: forward, 32-edge, k-rows kernel tile");
   typedef HaloMfma<TI, MS> MF;
   typedef typename MF::acc_t acc_t;
   constexpr int NR = MS == 32 ? 16 : 4;  // accumulators per lane and fragment
@@ -1135,6 +1150,26 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
           for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i], fb[j], acc[i][j]);
         __builtin_amdgcn_sched_barrier(0);
       }
+    } else if constexpr (F8) {
+      // one 32-byte operand per fragment: the two 16-byte reads land in the
+      // halves of one 8-register vector
+      i32x8 fa[FM], fb[FN];
+#pragma unroll
+      for (int f = 0; f < FM; ++f) {
+        const uint4 *src = (const uint4 *)&hcur[hoff[f] + toff];
+        fa[f] = __builtin_shufflevector(__builtin_bit_cast(i32x4, src[0]), __builtin_bit_cast(i32x4, src[1]), 0, 1, 2,
+                                        3, 4, 5, 6, 7);
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        fb[j] = __builtin_shufflevector(__builtin_bit_cast(i32x4, read_b(bs, j, 0)),
+                                        __builtin_bit_cast(i32x4, read_b(bs, j, 1)), 0, 1, 2, 3, 4, 5, 6, 7);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          // cbsz = blgp = 0: both operands OCP e4m3; block scales 2^0
+          acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0, 127, 0, 127);
     } else {
       uint4 fa[FM][2], fb[FN][2];
 #pragma unroll
@@ -1204,8 +1239,10 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     const int n = n0 + wn0 + j * MS + frow;
     bj[j] = (epi && g.bias && n < g.N) ? g.bias[n] : 0.0f;
   }
+  const float fsc = F8 ? g.sa[0] * g.sb[0] : 1.0f;
   auto val = [&](int i, int j, int r, bool act) __attribute__((always_inline)) -> float {
     float v = acc[i][j][r];
+    if (F8) v *= fsc;
     if (epi) {
       if (g.bias) v += bj[j];
       if (act && g.act == OWL_AMD_ACT_RELU) v = fmaxf(v, 0.0f);
@@ -1293,7 +1330,7 @@ static bool halo_rows_fit(const HaloGeom &g, int tr) {
 // while 32-edge tiles leave 12.5% of their MFMA work on padding.
 // OWL_AMD_HALO_MS = 16 or 32 forces an edge where it fits (tests, tuning).
 template <int PASS>
-static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom &g) {
+static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom &g, int force_ms = 0) {
   if (in_bytes != 2) return false;
   if (p.D != 1 || p.kd != 1 || p.Do != 1) return false;
   if (p.sh != 1 || p.sw != 1) return false;
@@ -1311,12 +1348,13 @@ static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom
   g.bias = PASS == 0 ? (const float *)p.bias : nullptr;
   g.pre = PASS == 0 ? p.pre : nullptr;
   g.act = PASS == 0 ? p.act : OWL_AMD_ACT_NONE;
+  g.sa = g.sb = nullptr;
   if (g.C % 32 != 0 || g.N % 8 != 0 || g.Wo > 256 || g.Ho < 1) return false;
   if ((int64_t)g.B * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * out_bytes >= (1LL << 31) ||
       (int64_t)g.kh * g.kw * g.C * g.N * 2 >= (1LL << 31))
     return false;  // 32-bit buffer offsets
   const char *fe = getenv("OWL_AMD_HALO_MS");
-  const int forced = fe ? atoi(fe) : 0;
+  const int forced = force_ms ? force_ms : fe ? atoi(fe) : 0;
   g.ms = 0;
   double best = -1.0;
   for (int ms : {32, 16}) {
@@ -1362,6 +1400,42 @@ bool run_halo(const ConvParams &p, hipStream_t st) {
   }
   return true;
 }
+// FP8: the layer seen as one over pairs of input channels (Ci / 2 of them,
+// 16-bit units), 32-edge tiles of 256 pixels x 64 channels.
+static bool halo_geom_fp8(const ConvParams &p, int out_bytes, HaloGeom &g) {
+  if (p.Ci % 64 != 0 || p.Co % 8 != 0) return false;
+  ConvParams q = p;
+  q.Ci = p.Ci / 2;
+  q.bias = nullptr;
+  q.pre = nullptr;
+  q.act = OWL_AMD_ACT_NONE;
+  if (!halo_geom<0>(q, 2, out_bytes, g, 32) || g.ms != 32) return false;
+  g.TR = 256 / g.Wo;
+  if (g.TR > g.Ho) g.TR = g.Ho;
+  if (!halo_rows_fit(g, g.TR)) return false;
+  g.sa = p.scale_a;
+  g.sb = p.scale_b;
+  return true;
+}
+
+bool halo_fits_fp8(const ConvParams &p, int out_bytes) {
+  HaloGeom g;
+  return halo_geom_fp8(p, out_bytes, g);
+}
+
+template <typename TO>
+bool run_halo_fp8(const ConvParams &p, const void *wp, hipStream_t st) {
+  HaloGeom g;
+  if (!halo_geom_fp8(p, (int)sizeof(TO), g)) return false;
+  const int64_t blocks = (int64_t)g.B * ((g.Ho + g.TR - 1) / g.TR) * ((g.N + 63) / 64);
+  KTimer kt(0, "conv_halo_kernel", st);
+  launch_k((conv_halo_kernel<0, bf16_t, TO, 32, 2, 2, 1, true, true>), dim3((unsigned)blocks), dim3(256), st, g,
+           (const bf16_t *)p.a, (const bf16_t *)wp, (TO *)p.c);
+  return true;
+}
+template bool run_halo_fp8<float>(const ConvParams &, const void *, hipStream_t);
+template bool run_halo_fp8<bf16_t>(const ConvParams &, const void *, hipStream_t);
+
 #define OWL_HALO(PASS, TI, TO) template bool run_halo<PASS, TI, TO>(const ConvParams &, hipStream_t);
 OWL_HALO(0, bf16_t, bf16_t)
 OWL_HALO(0, bf16_t, float)

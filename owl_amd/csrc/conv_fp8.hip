@@ -82,6 +82,27 @@ void run_e4m3_encode(const float *src, int64_t n, unsigned char *dst, hipStream_
   launch_k(e4m3_encode_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), st, src, n, dst);
 }
 
+// FP8 kernel [tap][ci][co] -> [tap][co][ci] (the rows of ci the halo forward
+// reads with 16-byte loads); one thread per output byte, writes coalesced.
+__global__ __launch_bounds__(256) void fp8_pack_krows_kernel(const unsigned char *__restrict__ w,
+                                                             unsigned char *__restrict__ wp, int64_t n, int Ci,
+                                                             int Co) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const int ci = (int)(e % Ci);
+    const int64_t r = e / Ci;
+    const int co = (int)(r % Co);
+    const int64_t t = r / Co;
+    wp[e] = w[(t * Ci + ci) * Co + co];
+  }
+}
+
+void run_fp8_pack_krows(const unsigned char *w, unsigned char *wp, int64_t T, int Ci, int Co, hipStream_t st) {
+  const int64_t n = T * Ci * Co;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  launch_k(fp8_pack_krows_kernel, dim3((unsigned)blocks), dim3(256), st, w, wp, n, Ci, Co);
+}
+
 template void run_fp8_quantize<float>(const float *, int64_t, unsigned char *, float *, hipStream_t);
 template void run_fp8_quantize<bf16_t>(const bf16_t *, int64_t, unsigned char *, float *, hipStream_t);
 

@@ -62,6 +62,14 @@ int fast_bk(int elem_bytes);
 template <int PASS, typename TI, typename TO>
 bool run_halo(const ConvParams &p, hipStream_t st);
 bool halo_fits(const ConvParams &p, int pass, int in_bytes, int out_bytes);
+// FP8 e4m3 forward on the halo kernel (conv_fast.hip): x as [..][Ci] bytes,
+// wp the kernel repacked to [tap][co][ci] (conv_fp8.hip), per-tensor scales
+// on the device; false when the layer does not fit (2-D, stride 1,
+// Ci % 64 == 0, Co % 8 == 0).
+bool halo_fits_fp8(const ConvParams &p, int out_bytes);
+template <typename TO>
+bool run_halo_fp8(const ConvParams &p, const void *wp, hipStream_t st);
+void run_fp8_pack_krows(const unsigned char *w, unsigned char *wp, int64_t T, int Ci, int Co, hipStream_t st);
 // conv_fast.hip: backward kernel for 16-bit storage via transposing LDS reads
 // (tile 128 x 128 x 32 pixels; p.split / p.k_per_split as for run_igemm<2>).
 int wgrad_tile_m(int64_t M);

@@ -88,3 +88,50 @@ def test_fp8_conv_forward(shape):
     rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
     assert rel(yg, rdq) < 1e-4
     assert rel(yg, ry) < 8e-2
+
+
+# Layers the FP8 halo kernel takes (2-D, stride 1, Ci % 64 == 0): the VGG
+# layer's shape, a second 64-channel slice only partly used, a 1x1 kernel,
+# dilation with VALID padding and a short last row tile; BF16 and FP32 output.
+HALO_FP8 = [([2, 12, 56, 64], [3, 3, 64, 128], 1, SAME),
+            ([2, 9, 20, 128], [3, 3, 128, 72], 1, VALID),
+            ([1, 7, 30, 64], [1, 1, 64, 64], 1, SAME),
+            ([2, 12, 20, 64], [3, 3, 64, 96], 2, VALID),
+            ([3, 13, 56, 64], [3, 3, 64, 128], 1, SAME)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(HALO_FP8)))
+@pytest.mark.parametrize("out", ["f32", "bf16"])
+def test_fp8_halo_forward(case, out):
+    xs, ks, rate, pad = HALO_FP8[case]
+    print(f"fp8 halo case {case} out {out} x{xs} w{ks} rate {rate}", flush=True)
+    rng = np.random.default_rng(40 + case)
+    x = rng.standard_normal(xs)
+    w = rng.standard_normal(ks) * np.sqrt(2.0 / np.prod(ks[:3]))
+    xt = torch.from_numpy(x).float().to(DEV).contiguous()
+    wt = torch.from_numpy(w).float().to(DEV).contiguous()
+    xq, sx = quantize(xt)
+    wq, sw = quantize(wt)
+    from owl_amd import ndarray as nd
+    g = nd.conv_problem(xs, ks, [1, 1], [rate, rate], pad)
+    ry = on.dilated_conv2d(x, w, [1, 1], [rate, rate], pad)
+    assert list(ry.shape[1:3]) == list(g.sp_out)
+    td, ot = (torch.float32, _lib.F32) if out == "f32" else (torch.bfloat16, _lib.BF16)
+    y = torch.full(ry.shape, float("nan"), dtype=td, device=DEV)
+    d = g.desc()
+    _lib.check(_lib.lib().owl_amd_conv_fwd_fp8(ctypes.byref(d), ot, ctypes.c_void_p(xq.data_ptr()),
+                                               ctypes.c_void_p(wq.data_ptr()), ctypes.c_void_p(sx.data_ptr()),
+                                               ctypes.c_void_p(sw.data_ptr()), ctypes.c_void_p(y.data_ptr()), _s()))
+    y2 = torch.full(ry.shape, float("nan"), dtype=td, device=DEV)
+    _lib.check(_lib.lib().owl_amd_conv_fwd_fp8(ctypes.byref(d), ot, ctypes.c_void_p(xq.data_ptr()),
+                                               ctypes.c_void_p(wq.data_ptr()), ctypes.c_void_p(sx.data_ptr()),
+                                               ctypes.c_void_p(sw.data_ptr()), ctypes.c_void_p(y2.data_ptr()), _s()))
+    torch.cuda.synchronize()
+    yg = y.double().cpu().numpy()
+    assert np.isfinite(yg).all()
+    assert torch.equal(y, y2)  # deterministic, every output written
+    rdq = on.dilated_conv2d(dequant(xq, sx), dequant(wq, sw), [1, 1], [rate, rate], pad)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(yg, rdq) < (1e-4 if out == "f32" else 1e-2), rel(yg, rdq)
+    assert rel(yg, ry) < 8e-2
