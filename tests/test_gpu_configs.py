@@ -108,6 +108,52 @@ def test_config3_resnet_stack(dtype, tol):
 
 
 @pytest.mark.gpu
+def test_config3_resnet_stack_fp8_forward():
+    """Config 3's FP8 e4m3 forward at full size: per-tensor amax scales, the
+    3x3 256->256 layer (FP8 halo kernel) and the 1x1 256->512 stride-2 layer
+    (generic kernel), FP32 and BF16 output.  On a 2-image slice: within 1e-4
+    (FP32 output; 1e-2 for BF16) of the FP64 oracle on the dequantised
+    operands, the MFMA's accumulation being the only error, and rel-L2 < 8e-2
+    against the FP64 oracle on the unquantised inputs (e4m3 keeps 3 mantissa
+    bits)."""
+    import ctypes
+    from owl_amd import _lib
+    x, w, g = layer([128, 28, 28, 256], [3, 3, 256, 256], [1, 1], torch.float32, 6)
+    w2 = torch.randn([1, 1, 256, 512], device=DEV, generator=g) * (2.0 / 256) ** 0.5
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def quant(t):
+        q = torch.empty(t.numel(), dtype=torch.uint8, device=DEV)
+        sc = torch.empty(1, dtype=torch.float32, device=DEV)
+        _lib.check(_lib.lib().owl_amd_fp8_quantize(_lib.F32, ctypes.c_void_p(t.data_ptr()), t.numel(),
+                                                   ctypes.c_void_p(q.data_ptr()), ctypes.c_void_p(sc.data_ptr()), s))
+        return q.view(t.shape), sc
+
+    def deq(q, sc):
+        return (q.view(torch.float8_e4m3fn).float().double() * float(sc.item())).cpu().numpy()
+
+    xq, sx = quant(x)
+    n = 2
+    for ww, stride in ((w, [1, 1]), (w2, [2, 2])):
+        wq, sw = quant(ww)
+        geo = nd.conv_problem(list(x.shape), list(ww.shape), stride, None, SAME)
+        d = geo.desc()
+        rdq = on.conv2d(deq(xq[:n].contiguous(), sx), deq(wq, sw), stride, SAME)
+        ry = on.conv2d(h64(x[:n]), h64(ww), stride, SAME)
+        for td, ot, tol in ((torch.float32, _lib.F32, 1e-4), (torch.bfloat16, _lib.BF16, 1e-2)):
+            y = torch.empty([128] + geo.sp_out + [ww.shape[3]], dtype=td, device=DEV)
+            _lib.check(_lib.lib().owl_amd_conv_fwd_fp8(ctypes.byref(d), ot, ctypes.c_void_p(xq.data_ptr()),
+                                                       ctypes.c_void_p(wq.data_ptr()), ctypes.c_void_p(sx.data_ptr()),
+                                                       ctypes.c_void_p(sw.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                                       s))
+            torch.cuda.synchronize()
+            assert np.isfinite(h64(y)).all()
+            ys = h64(y[:n])
+            assert rel(ys, rdq) < tol, (stride, td, rel(ys, rdq))
+            assert rel(ys, ry) < 8e-2, (stride, td, rel(ys, ry))
+
+
+@pytest.mark.gpu
 def test_config5_conv3d_stack_fp32():
     """conv3d [8,32,32,32,16] (*) [3,3,3,16,32] SAME then (*) [3,3,3,32,64], avg_pool3d 2x2x2."""
     x, w1, g = layer([8, 32, 32, 32, 16], [3, 3, 3, 16, 32], [1, 1, 1], torch.float32, 5)
