@@ -51,8 +51,9 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print(f"bench.py: the benchmark measures one GPU; --gpus {args.gpus} is not supported "
+              "(the layer would shard over the batch, with one all-reduce of dw; that is not implemented)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
