@@ -151,6 +151,20 @@ def main():
                             1),
         "layer": "BF16 relu(conv2d(x, w) + b), x[64,56,56,64] w[3,3,64,128]",
     }
+    # the same layer followed by the 2x2 stride-2 max-pool: pooled in the
+    # forward kernel's epilogue (with y, and without it for inference)
+    # against the fused layer then the pooling kernel
+    hb = nd.conv2d_bias_act(xb, wb, bias, [1, 1], SAME)
+    fused_pool = {
+        "fused_us": round(t_loop(lambda: nd.conv2d_bias_act_maxpool(xb, wb, bias, [1, 1], SAME)), 1),
+        "fused_no_y_us": round(t_loop(lambda: nd.conv2d_bias_act_maxpool(xb, wb, bias, [1, 1], SAME,
+                                                                          keep_y=False)), 1),
+        "unfused_us": round(t_loop(lambda: nd.max_pool2d(nd.conv2d_bias_act(xb, wb, bias, [1, 1], SAME), [2, 2],
+                                                         [2, 2], VALID)), 1),
+        "pool_alone_us": round(t_loop(lambda: nd.max_pool2d(hb, [2, 2], [2, 2], VALID)), 1),
+        "layer": "BF16 max_pool2x2(relu(conv2d(x, w) + b)), x[64,56,56,64] w[3,3,64,128]",
+    }
+    del hb
 
     # the FP8 e4m3 forward of the same layer (per-tensor amax scales, FP8
     # operands resident, BF16 output), quantisation outside the timed loop
@@ -339,6 +353,7 @@ def main():
         "errors": errs,
         "pool_max2x2s2": pool,
         "conv_bias_relu_bf16": fused,
+        "conv_bias_relu_maxpool_bf16": fused_pool,
         "cgraph_vgg_block_bf16": cgraph_block,
         "dropin_fp32_host_buffers": dropin,
         "fp8_e4m3_fwd": fp8,

@@ -122,6 +122,21 @@ int owl_amd_conv_fwd_bias_act(const owl_amd_conv_desc *d, int in_t, int out_t,
                               const void *x, const void *w, const void *bias, int act,
                               void *y, void *pre, void *stream);
 
+/* The conv layer followed by a 2x2 stride-2 VALID max-pool layer, the pair
+ * of a VGG block's end: Conv2D.run then the activation, then MaxPool2D.run =
+ * max_pool2d VALID [2;2] [2;2] (owl_neural_neuron.ml:1253, :2024;
+ * owl_dense_ndarray_generic.ml:6905; owl_ndarray_pool_impl.h:9-92).  Writes
+ * what owl_amd_conv_fwd_bias_act writes, and pooled[B][Ho/2][Wo/2][Co] =
+ * the max of each window of y, bit-identical to owl_amd_pool_fwd over y.
+ * 2-D only (D = Do = kd = 1).  y may be null when only the pooled layer is
+ * wanted (inference).  For BF16 / F16 stride-1 layers the halo kernel pools
+ * its output tile before it leaves the chip (y is then not read back);
+ * otherwise the convolution runs, then the pooling kernel (into a library
+ * buffer when y is null; calls on one device that overlap then share it). */
+int owl_amd_conv_fwd_bias_act_pool(const owl_amd_conv_desc *d, int in_t, int out_t,
+                                   const void *x, const void *w, const void *bias, int act,
+                                   void *y, void *pre, void *pooled, void *stream);
+
 /* 2-D nearest-neighbour upsampling of NHWC tensors by integer scales
  * (sh along H = Owl's cols, sw along W = rows).  Forward:
  * y[b,i,j,c] = x[b, min(i/sh, H-1), min(j/sw, W-1), c] -- Owl's upsampling2d is

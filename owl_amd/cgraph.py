@@ -298,7 +298,7 @@ class Graph:
             else:
                 blk = self.blocks[self.plan[n.id]]
                 self.views[n.id] = blk[: n.nbytes()].view(n.dtype).view(n.shape)
-            if n.op == "conv2d_bias_act" and n.parents[2] is not None:
+            if n.op in ("conv2d_bias_act", "conv2d_bias_act_maxpool") and n.parents[2] is not None:
                 acc = torch.float64 if n.dtype == torch.float64 else torch.float32
                 n.params["bias32"] = torch.empty(n.parents[2].shape, dtype=acc, device=device)
         self.workspace = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=device)
@@ -313,8 +313,10 @@ def make_graph(input, output, name):
 
 def optimise(graph):
     """Fuse conv2d -> add bias -> relu (and conv2d -> add bias) into one node
-    run by the fused conv kernel.  A conv or add whose result has another
-    consumer, or is a graph output, is left alone."""
+    run by the fused conv kernel, and such a node followed by a 2x2 stride-2
+    VALID max_pool2d into one that writes only the pooled layer.  A conv, add
+    or fused layer whose result has another consumer, or is a graph output,
+    is left alone."""
     consumers = {}
     for n in graph.order:
         for p in n.parents:
@@ -343,6 +345,22 @@ def optimise(graph):
         n.parents = [c.parents[0], c.parents[1], b]
         changed = True
     if changed:
+        graph._rebuild()
+    consumers = {}
+    for n in graph.order:
+        for p in n.parents:
+            consumers.setdefault(p.id, []).append(n)
+    pooled = False
+    for n in list(graph.order):
+        c = n.parents[0] if n.parents else None
+        if (n.op != "max_pool2d" or list(n.params["kernel"]) != [2, 2] or list(n.params["stride"]) != [2, 2]
+                or n.params["padding"] != VALID or c.op != "conv2d_bias_act" or c.parents[2] is None
+                or not only_use(c, n)):
+            continue
+        n.op, n.params = "conv2d_bias_act_maxpool", dict(c.params)
+        n.parents = list(c.parents)
+        pooled = True
+    if pooled:
         graph._rebuild()
     return graph
 
@@ -413,6 +431,11 @@ def _run(graph):
             b32 = P["bias32"]
             b32.copy_(a[2])
             nd.conv2d_bias_act(a[0], a[1], b32, P["stride"], P["padding"], act=P["act"], out=out)
+        elif op == "conv2d_bias_act_maxpool":
+            b32 = P["bias32"]
+            b32.copy_(a[2])
+            nd.conv2d_bias_act_maxpool(a[0], a[1], b32, P["stride"], P["padding"], act=P["act"], keep_y=False,
+                                       out=out)
         else:  # pragma: no cover
             raise InvalidArgument(f"eval_graph: unknown operator {op}")
 

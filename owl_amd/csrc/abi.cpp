@@ -185,6 +185,7 @@ struct DeviceState {
   DevBuf red;                    // bias-gradient partial sums
   DevBuf cplx;                   // complex convolutions: expanded kernel / real weight gradient
   DevBuf f8w;                    // FP8 halo forward: kernel repacked to [tap][co][ci]
+  DevBuf ytmp;                   // fused conv + pool without y, when the pooling runs apart
   DevBuf slot[4];                // drop-in staging buffers
 };
 
@@ -576,6 +577,59 @@ extern "C" int owl_amd_conv_fwd_bias_act(const owl_amd_conv_desc *d, int in_t, i
     return OWL_AMD_OK;
   }
   return dispatch_igemm<0>(p, in_t, out_t, aligned, (hipStream_t)stream);
+}
+
+extern "C" int owl_amd_conv_fwd_bias_act_pool(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x,
+                                              const void *w, const void *bias, int act, void *y, void *pre,
+                                              void *pooled, void *stream) {
+  int rc = check_conv(d);
+  if (rc) return rc;
+  if (is_complex(in_t) || is_complex(out_t)) {
+    owl_amd_set_error("the fused bias / activation / pooling epilogue takes real kinds");
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  if (d->D != 1 || d->Do != 1 || d->kd != 1) {
+    owl_amd_set_error("the fused pooling takes 2-D convolutions");
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  if (act != OWL_AMD_ACT_NONE && act != OWL_AMD_ACT_RELU) ARG_FAIL("unknown activation %d", act);
+  if (pre && pre == y) ARG_FAIL("pre must not alias y");
+  const int64_t Hp = d->Ho / 2, Wp = d->Wo / 2;
+  if (d->B * d->Ho * d->Wo * d->Co == 0) return OWL_AMD_OK;
+  if (!x || !w || (!pooled && Hp * Wp > 0)) ARG_FAIL("null pointer");
+  if (pooled && (pooled == y || pooled == pre)) ARG_FAIL("pooled must not alias y or pre");
+  hipStream_t st = (hipStream_t)stream;
+  // 16-bit stride-1 layers: the halo kernel pools its staged output tile
+  if (in_t == out_t && (in_t == OWL_AMD_BF16 || in_t == OWL_AMD_F16) && Hp * Wp > 0 && fast_path_enabled() &&
+      halo_enabled() && d->Ci % 8 == 0 && d->Co % 8 == 0 && aligned16(x, w, y) && aligned16(pre, pooled, nullptr)) {
+    ConvParams p = make_params(d);
+    p.M = d->B * d->Ho * d->Wo;
+    p.N = p.Co;
+    p.K = d->kh * d->kw * d->Ci;
+    p.a = x; p.b = w; p.c = y; p.ldc = p.Co;
+    p.bias = bias; p.pre = pre; p.act = act; p.pool = pooled;
+    const bool done = in_t == OWL_AMD_BF16 ? run_halo<0, bf16_t, bf16_t>(p, st) : run_halo<0, f16_t, f16_t>(p, st);
+    if (done) {
+      OWL_CHECK_HIP(hipGetLastError());
+      return OWL_AMD_OK;
+    }
+  }
+  // otherwise the convolution, then the pooling kernel over y (a library
+  // buffer when the caller does not want y)
+  void *yy = y;
+  if (!yy) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DeviceState *s;
+    rc = ensure_state(&s);
+    if (rc) return rc;
+    rc = ensure_buf(s->ytmp, (size_t)(d->B * d->Ho * d->Wo * d->Co) * dtype_size(out_t), st);
+    if (rc) return rc;
+    yy = s->ytmp.ptr;
+  }
+  rc = owl_amd_conv_fwd_bias_act(d, in_t, out_t, x, w, bias, act, yy, pre, stream);
+  if (rc || Hp * Wp == 0) return rc;
+  const owl_amd_pool_desc q = {d->B, d->Ho, d->Wo, 1, d->Co, 2, 2, 1, Hp, Wp, 1, 2, 2, 1, 0, 0, 0};
+  return owl_amd_pool_fwd(&q, out_t, 1, yy, pooled, stream);
 }
 
 extern "C" int owl_amd_conv_bwd_input(const owl_amd_conv_desc *d, int in_t, int out_t,

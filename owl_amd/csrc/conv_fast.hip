@@ -800,6 +800,9 @@ struct HaloGeom {
   const float *bias;
   void *pre;
   int act;
+  // forward, 16-bit output only: 2x2 stride-2 max-pool of y written here
+  // from the staged tile (TR even, or one row tile), else null
+  void *pool;
   // FP8 forward: per-tensor scales of x and w (device floats), else null
   const float *sa, *sb;
 };
@@ -809,6 +812,11 @@ struct HaloGeom {
 // v_mfma_f32_16x16x32 (one per K-step, 4 accumulators per lane).  Both run
 // at the same rate; 16-pixel fragments tile rows whose pixel count is not a
 // multiple of 32 with less padding.
+// compile-time flag for the staged epilogue's variants
+template <bool V> struct HaloFlag {
+  static constexpr bool value = V;
+};
+
 template <typename TI, int MS> struct HaloMfma;
 template <> struct HaloMfma<bf16_t, 32> {
   typedef f32x16 acc_t;
@@ -1258,30 +1266,92 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   // staged output row: BN values, plus 16 B of padding when that still fits
   constexpr int ORS_PAD = BN * (int)sizeof(TO) + 16;
   constexpr int ORS = CAP * ORS_PAD <= LDS_BYTES ? ORS_PAD : BN * (int)sizeof(TO);
+  // dst may be null when the tile only feeds the fused pooling
   auto emit = [&](TO *dst, bool act) __attribute__((always_inline)) {
     const __amdgpu_buffer_rsrc_t orr = make_rsrc(dst, o_bytes);
     if constexpr (CAP * ORS <= LDS_BYTES) {
       // Stage the tile in LDS as [pixel][channel] rows (the LDS is free after
-      // the K loop's last barrier), then store whole 16-byte chunks.
+      // the K loop's last barrier), then store whole 16-byte chunks.  The
+      // bias and relu choices are made once, outside the unrolled loops:
+      // tested per value they cost 3.5 us on a 64x56x56x64 -> 128 layer.
+      auto stage = [&](auto HB, auto HA) __attribute__((always_inline)) {
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          char *rowp = smem + (wm * FM * MS + i * MS + 4 * fkg) * ORS + (wn0 + j * MS + frow) * (int)sizeof(TO);
+          for (int j = 0; j < FN; ++j) {
+            char *rowp = smem + (wm * FM * MS + i * MS + 4 * fkg) * ORS + (wn0 + j * MS + frow) * (int)sizeof(TO);
 #pragma unroll
-          for (int r = 0; r < NR; ++r) *(TO *)(rowp + rofs(r) * ORS) = (TO)val(i, j, r, act);
-        }
+            for (int r = 0; r < NR; ++r) {
+              float v = acc[i][j][r];
+              if (F8) v *= fsc;
+              if constexpr (decltype(HB)::value) v += bj[j];
+              if constexpr (decltype(HA)::value) v = fmaxf(v, 0.0f);
+              *(TO *)(rowp + rofs(r) * ORS) = (TO)v;
+            }
+          }
+      };
+      const bool hb = epi && g.bias, ha = epi && act && g.act == OWL_AMD_ACT_RELU;
+      if (hb && ha) stage(HaloFlag<true>(), HaloFlag<true>());
+      else if (hb) stage(HaloFlag<true>(), HaloFlag<false>());
+      else if (ha) stage(HaloFlag<false>(), HaloFlag<true>());
+      else stage(HaloFlag<false>(), HaloFlag<false>());
       __syncthreads();
       constexpr int CPR = BN * (int)sizeof(TO) / 16;  // 16-byte chunks per row
       constexpr int EPC = 16 / (int)sizeof(TO);       // values per chunk
-      for (int c = tid; c < P * CPR; c += 256) {
-        const int pix = c / CPR, ch = c - (c / CPR) * CPR;
-        const int n = n0 + ch * EPC;
-        const uint4 v = *(const uint4 *)(smem + pix * ORS + ch * 16);
-        const uint32_t vo = n < g.N ? (uint32_t)((obase + pix) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), orr, (int)vo, 0, ST_NT);
+      if (dst) {
+        for (int c = tid; c < P * CPR; c += 256) {
+          const int pix = c / CPR, ch = c - (c / CPR) * CPR;
+          const int n = n0 + ch * EPC;
+          const uint4 v = *(const uint4 *)(smem + pix * ORS + ch * 16);
+          const uint32_t vo = n < g.N ? (uint32_t)((obase + pix) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
+          __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), orr, (int)vo, 0, ST_NT);
+        }
       }
-    } else {
+      if constexpr (PASS == 0 && sizeof(TO) == 2) {
+        if (act && g.pool) {
+          // Fused 2x2 stride-2 max-pool of y (the VALID windows, Hp = Ho / 2,
+          // Wp = Wo / 2).  The tile starts at an even row, so its windows
+          // are whole; each thread reads a window's four staged chunks and
+          // keeps Owl's max (owl_ndarray_pool_impl.h:9-92 with the stub's
+          // max macros): acc = -inf, then "if (acc < t) acc = t" over
+          // (h, w), (h, w + 1), (h + 1, w), (h + 1, w + 1), compared in FP32
+          // -- the first strict maximum, NaN never taken.
+          const int Wp = g.Wo >> 1, Hp = g.Ho >> 1;
+          const int PQ = (TRv >> 1) * Wp;
+          const uint32_t p_bytes = (uint32_t)((int64_t)g.B * Hp * Wp * g.N * sizeof(TO));
+          const __amdgpu_buffer_rsrc_t prr = make_rsrc(g.pool, p_bytes);
+          const int pbase = (b * Hp + (i0 >> 1)) * Wp;
+          for (int c = tid; c < PQ * CPR; c += 256) {
+            const int pq = c / CPR, ch = c - (c / CPR) * CPR;
+            const int pr = pq / Wp, pc = pq - (pq / Wp) * Wp;
+            const char *s0 = smem + (2 * pr * g.Wo + 2 * pc) * ORS + ch * 16;
+            const int wofs[4] = {0, ORS, g.Wo * ORS, (g.Wo + 1) * ORS};
+            float m[EPC];
+#pragma unroll
+            for (int e = 0; e < EPC; ++e) m[e] = -__builtin_inff();
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const uint4 q = *(const uint4 *)(s0 + wofs[k]);
+              TO tv[EPC];
+              __builtin_memcpy(tv, &q, 16);
+#pragma unroll
+              for (int e = 0; e < EPC; ++e) {
+                const float v = (float)tv[e];
+                if (m[e] < v) m[e] = v;
+              }
+            }
+            TO o[EPC];
+#pragma unroll
+            for (int e = 0; e < EPC; ++e) o[e] = (TO)m[e];
+            uint4 ov;
+            __builtin_memcpy(&ov, o, 16);
+            const int n = n0 + ch * EPC;
+            const uint32_t vo = n < g.N ? (uint32_t)((pbase + pq) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, ov), prr, (int)vo, 0, ST_NT);
+          }
+        }
+      }
+    } else if (dst) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -1354,7 +1424,10 @@ static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom
   g.bias = PASS == 0 ? (const float *)p.bias : nullptr;
   g.pre = PASS == 0 ? p.pre : nullptr;
   g.act = PASS == 0 ? p.act : OWL_AMD_ACT_NONE;
+  g.pool = PASS == 0 ? p.pool : nullptr;
   g.sa = g.sb = nullptr;
+  // the fused pooling reads the staged 16-bit tile
+  if (g.pool && out_bytes != 2) return false;
   if (g.C % 32 != 0 || g.N % 8 != 0 || g.Wo > 256 || g.Ho < 1) return false;
   if ((int64_t)g.B * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * out_bytes >= (1LL << 31) ||
       (int64_t)g.kh * g.kw * g.C * g.N * 2 >= (1LL << 31))
@@ -1364,7 +1437,10 @@ static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom
   g.ms = 0;
   double best = -1.0;
   for (int ms : {32, 16}) {
-    const int cap = halo_cap(g.N, ms), tr = halo_rows(g, cap);
+    const int cap = halo_cap(g.N, ms);
+    int tr = halo_rows(g, cap);
+    // fused pooling: tiles start at even rows (whole 2x2 windows)
+    if (g.pool && tr < g.Ho) tr &= ~1;
     if (!halo_rows_fit(g, tr)) continue;
     const double use = (double)g.Ho * g.Wo / ((double)((g.Ho + tr - 1) / tr) * cap);
     const bool take = forced == 16 || forced == 32 ? ms == forced || g.ms == 0 : use > best + 1e-9;
@@ -1415,6 +1491,7 @@ static bool halo_geom_fp8(const ConvParams &p, int out_bytes, HaloGeom &g) {
   q.bias = nullptr;
   q.pre = nullptr;
   q.act = OWL_AMD_ACT_NONE;
+  q.pool = nullptr;
   if (!halo_geom<0>(q, 2, out_bytes, g, 32) || g.ms != 32) return false;
   g.TR = 256 / g.Wo;
   if (g.TR > g.Ho) g.TR = g.Ho;

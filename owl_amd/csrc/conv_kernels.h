@@ -31,6 +31,9 @@ struct ConvParams {
   const void *bias;
   void *pre;
   int act;  // OWL_AMD_ACT_*
+  // forward, 16-bit halo kernel only: 2x2 stride-2 max-pool of y written
+  // from the epilogue (owl_amd_conv_fwd_bias_act_pool), else null
+  void *pool;
 };
 
 template <int PASS, typename TI, typename TO>

@@ -622,6 +622,40 @@ def conv2d_bias_act(input, kernel, bias, stride, padding=SAME, act="relu", keep_
                           out_dtype, out, pre_out)
 
 
+def conv2d_bias_act_maxpool(input, kernel, bias, stride, padding=SAME, act="relu", keep_y=True, keep_pre=False,
+                            out_dtype=None, out=None):
+    """Owl's conv layer followed by a 2x2 stride-2 VALID max-pool layer (a VGG
+    block's end: owl_neural_neuron.ml:1253 Conv2D.run and the activation, then
+    :2024 MaxPool2D.run = ``max_pool2d ~padding:VALID x [|2;2|] [|2;2|]``).
+    Returns (y, pooled), or (y, pooled, pre) when keep_pre; y is None when
+    keep_y is False (inference: only the pooled layer leaves the chip).
+    pooled equals ``max_pool2d(y, [2, 2], [2, 2], VALID)`` bit for bit; for
+    BF16 / FP16 stride-1 layers it is taken from the convolution's output
+    tile before that is stored.  ``out`` is an optional array for pooled."""
+    name = "conv2d_bias_act_maxpool"
+    _verify(_is_torch(input) and _is_torch(kernel), f"{name}: the fused layer ops take device tensors")
+    _check_conv_args(name, 2, input, kernel, stride, None)
+    x, k = _contig(input), _contig(kernel)
+    g = conv_problem(list(x.shape), list(k.shape), stride, None, padding)
+    ys = [g.B, *g.sp_out, g.Co]
+    y = _out_buf(None, x, ys, out_dtype, name, (x, k)) if keep_y else None
+    odt = y.dtype if y is not None else (out_dtype or x.dtype)
+    pre = _out_buf(None, x, ys, odt, name, (x, k)) if keep_pre else None
+    pooled = _out_buf(out, x, [g.B, g.sp_out[0] // 2, g.sp_out[1] // 2, g.Co], odt, name, (x, k))
+    acc_dt = torch.float64 if odt == torch.float64 else torch.float32
+    if bias is not None:
+        _verify(_is_torch(bias) and tuple(bias.shape) == (g.Co,),
+                f"{name}: bias must be a device tensor of shape [{g.Co}]")
+        _verify(bias.dtype == acc_dt, f"{name}: bias must be {acc_dt} for {odt} output")
+        bias = _contig(bias)
+    d = g.desc()
+    _lib.check(_lib.lib().owl_amd_conv_fwd_bias_act_pool(
+        ctypes.byref(d), _tdt(x), _tdt(pooled), _ptr(x), _ptr(k), _ptr(bias) if bias is not None else None,
+        _act_code(act), _ptr(y) if y is not None else None, _ptr(pre) if pre is not None else None,
+        _ptr(pooled), _stream()))
+    return (y, pooled, pre) if keep_pre else (y, pooled)
+
+
 def conv3d_bias_act(input, kernel, bias, stride, padding=SAME, act="relu", keep_pre=False, out_dtype=None,
                     out=None, pre_out=None):
     """The 3-D form of ``conv2d_bias_act`` (Owl's Conv3D layer)."""

@@ -182,3 +182,30 @@ def test_out_argument_checks():
     with pytest.raises(InvalidArgument):   # 1-D out must be contiguous
         nd.conv1d_(np.zeros([1, 6, 10], np.float32)[..., ::2], x[:, 0], w[0], [1])
     assert nd.conv2d_.__doc__ and "2402" in nd.conv2d_.__doc__
+
+
+def vgg_block(pool_padding=VALID, extra_out=False):
+    x = G.var_arr("x", [2, 8, 8, 16])
+    w1, b1 = G.var_arr("w1", [3, 3, 16, 32]), G.var_arr("b1", [32])
+    w2, b2 = G.var_arr("w2", [3, 3, 32, 32]), G.var_arr("b2", [32])
+    h1 = G.relu(G.add(G.conv2d(x, w1, [1, 1]), b1))
+    h2 = G.relu(G.add(G.conv2d(h1, w2, [1, 1]), b2))
+    y = G.max_pool2d(h2, [2, 2], [2, 2], pool_padding)
+    return G.make_graph([x, w1, b1, w2, b2], [y, h2] if extra_out else [y], "vgg_block")
+
+
+def test_optimise_fuses_conv_bias_relu_maxpool():
+    g = G.optimise(vgg_block())
+    ops = [n.op for n in g.order]
+    assert ops.count("conv2d_bias_act") == 1 and ops.count("conv2d_bias_act_maxpool") == 1
+    assert not {"max_pool2d", "add", "relu", "conv2d"} & set(ops)
+    last = next(n for n in g.order if n.op == "conv2d_bias_act_maxpool")
+    first = next(n for n in g.order if n.op == "conv2d_bias_act")
+    assert last.parents[0] is first and [p.name for p in last.parents[1:]] == ["w2", "b2"]
+    assert last.shape == [2, 4, 4, 32] and last.params["act"] == "relu" and g.outputs[0] is last
+    check_plan(g)
+    # a SAME-padded pool, or a layer output that is also a graph output, stays apart
+    for gg in (G.optimise(vgg_block(pool_padding=SAME)), G.optimise(vgg_block(extra_out=True))):
+        ops = [n.op for n in gg.order]
+        assert "conv2d_bias_act_maxpool" not in ops and ops.count("conv2d_bias_act") == 2 and "max_pool2d" in ops
+        check_plan(gg)
