@@ -1266,8 +1266,14 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   // staged output row: BN values, plus 16 B of padding when that still fits
   constexpr int ORS_PAD = BN * (int)sizeof(TO) + 16;
   constexpr int ORS = CAP * ORS_PAD <= LDS_BYTES ? ORS_PAD : BN * (int)sizeof(TO);
-  // dst may be null when the tile only feeds the fused pooling
-  auto emit = [&](TO *dst, bool act) __attribute__((always_inline)) {
+  // dst may be null when the tile only feeds the fused pooling.  dst_pre
+  // (staged 16-bit outputs): the tile is staged once, before the activation,
+  // and stored to dst_pre; y is formed from it per 16-bit value on the way
+  // out.  relu commutes with rounding to 16 bits, and the integer form
+  // below maps negatives, -0 and NaN to +0 as fmaxf(v, 0) does before
+  // rounding (v is never -0 there: the sums start from +0), so y has the
+  // bits of the two-pass epilogue.
+  auto emit = [&](TO *dst, bool act, TO *dst_pre) __attribute__((always_inline)) {
     const __amdgpu_buffer_rsrc_t orr = make_rsrc(dst, o_bytes);
     if constexpr (CAP * ORS <= LDS_BYTES) {
       // Stage the tile in LDS as [pixel][channel] rows (the LDS is free after
@@ -1298,17 +1304,34 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       __syncthreads();
       constexpr int CPR = BN * (int)sizeof(TO) / 16;  // 16-byte chunks per row
       constexpr int EPC = 16 / (int)sizeof(TO);       // values per chunk
-      if (dst) {
+      const bool relu_y = dst_pre && g.act == OWL_AMD_ACT_RELU;
+      // relu of eight 16-bit values: sign bit or NaN -> +0
+      auto relu8 = [](uint4 v) __attribute__((always_inline)) -> uint4 {
+        constexpr unsigned INF16 = __is_same(TO, f16_t) ? 0x7c00u : 0x7f80u;
+        unsigned short h[8];
+        __builtin_memcpy(h, &v, 16);
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if ((h[e] & 0x8000u) || (h[e] & 0x7fffu) > INF16) h[e] = 0;
+        __builtin_memcpy(&v, h, 16);
+        return v;
+      };
+      if (dst || dst_pre) {
+        const __amdgpu_buffer_rsrc_t qrr = make_rsrc(dst_pre, o_bytes);
         for (int c = tid; c < P * CPR; c += 256) {
           const int pix = c / CPR, ch = c - (c / CPR) * CPR;
           const int n = n0 + ch * EPC;
-          const uint4 v = *(const uint4 *)(smem + pix * ORS + ch * 16);
+          uint4 v = *(const uint4 *)(smem + pix * ORS + ch * 16);
           const uint32_t vo = n < g.N ? (uint32_t)((obase + pix) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
-          __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), orr, (int)vo, 0, ST_NT);
+          if (dst_pre) __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), qrr, (int)vo, 0, ST_NT);
+          if (dst) {
+            if (relu_y) v = relu8(v);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v), orr, (int)vo, 0, ST_NT);
+          }
         }
       }
       if constexpr (PASS == 0 && sizeof(TO) == 2) {
-        if (act && g.pool) {
+        if ((act || dst_pre) && g.pool) {
           // Fused 2x2 stride-2 max-pool of y (the VALID windows, Hp = Ho / 2,
           // Wp = Wo / 2).  The tile starts at an even row, so its windows
           // are whole; each thread reads a window's four staged chunks and
@@ -1331,7 +1354,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
             for (int e = 0; e < EPC; ++e) m[e] = -__builtin_inff();
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-              const uint4 q = *(const uint4 *)(s0 + wofs[k]);
+              uint4 q = *(const uint4 *)(s0 + wofs[k]);
+              if (relu_y) q = relu8(q);
               TO tv[EPC];
               __builtin_memcpy(tv, &q, 16);
 #pragma unroll
@@ -1374,11 +1398,17 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
         }
     }
   };
-  if (PASS == 0 && g.pre) {
-    emit((TO *)g.pre, false);
-    __syncthreads();  // the staging area is reused for y
+  if constexpr (sizeof(TO) == 2 && CAP * ORS <= LDS_BYTES) {
+    // one staging pass also when the pre-activation is kept
+    TO *pre = PASS == 0 ? (TO *)g.pre : nullptr;
+    emit(out, pre == nullptr, pre);
+  } else {
+    if (PASS == 0 && g.pre) {
+      emit((TO *)g.pre, false, nullptr);
+      __syncthreads();  // the staging area is reused for y
+    }
+    emit(out, true, nullptr);
   }
-  emit(out, true);
 }
 
 // Pixel capacity of a halo tile: 128-wide channel slices (N > 64) run 4

@@ -12,7 +12,9 @@ Either way:
     both run the same MFMA fragment edge (tile rows do not change any
     output's summation), and y is within 1e-2 (16-bit output) of the FP64
     oracle on the rounded inputs;
-  * without y (inference) the pooled result is the same.
+  * without y (inference) the pooled result is the same;
+  * y is the same whether or not pre is kept (keeping it, the 16-bit
+    epilogue forms y from the staged pre-activation).
 Shapes cover 224- and 448-pixel tiles, odd output heights and widths (the
 last row / column is dropped), a partial channel slice, a layer whose tiles
 are one row (the pooling kernel runs apart) and NaN windows."""
@@ -38,6 +40,7 @@ CASES = [
     ([1, 9, 200, 32], [3, 3, 32, 128], SAME, "bf16", "relu", False),   # one-row tiles: pooling kernel apart
     ([2, 28, 28, 64], [1, 1, 64, 64], SAME, "f16", "relu", False),     # 1x1 kernel
     ([2, 14, 18, 32], [3, 3, 32, 64], SAME, "bf16", "none", True),     # NaN windows, no activation
+    ([2, 14, 18, 32], [3, 3, 32, 64], SAME, "f16", "relu", True),      # NaN windows under relu
 ]
 
 
@@ -88,6 +91,7 @@ def test_conv_bias_act_maxpool(case, edge, monkeypatch):
 
     y, pooled, pre = nd.conv2d_bias_act_maxpool(xd, wd, bd, [1, 1], pad, act=act, keep_pre=True)
     y_ref, pre_ref = nd.conv2d_bias_act(xd, wd, bd, [1, 1], pad, act=act, keep_pre=True)
+    y_plain = nd.conv2d_bias_act(xd, wd, bd, [1, 1], pad, act=act)
     none_y, pooled2 = nd.conv2d_bias_act_maxpool(xd, wd, bd, [1, 1], pad, act=act, keep_y=False)
     pooled_dev = nd.max_pool2d(y, [2, 2], [2, 2], VALID)
     torch.cuda.synchronize()
@@ -100,6 +104,8 @@ def test_conv_bias_act_maxpool(case, edge, monkeypatch):
     assert torch.equal(bits(pooled2), bits(pooled))
     ref_pool = on.max_pool2d(host(y), [2, 2], [2, 2], VALID)
     np.testing.assert_array_equal(host(pooled), ref_pool)
+    # keeping pre, y is formed from the staged pre-activation: the same bits
+    assert torch.equal(bits(y_plain), bits(y_ref))
     # y and pre as the unfused layer computes them
     # (the same edge runs both when the forced one takes the fused pooling,
     # or when no edge does and the plain convolution runs)
@@ -112,8 +118,10 @@ def test_conv_bias_act_maxpool(case, edge, monkeypatch):
     ref_y = np.fmax(ref_pre, 0.0) if act == "relu" else ref_pre
     assert rel(host(y), ref_y) < 1e-2
     assert rel(host(pre), ref_pre) < 1e-2
-    if nans:
+    if nans and act == "none":
         assert np.isneginf(host(pooled)).any() and not np.isnan(host(pooled)).any()
+    if nans and act == "relu":
+        assert not np.isnan(host(y)).any() and np.isnan(host(pre)).any()
 
 
 @pytest.mark.gpu
