@@ -162,9 +162,16 @@ def main():
         "unfused_us": round(t_loop(lambda: nd.max_pool2d(nd.conv2d_bias_act(xb, wb, bias, [1, 1], SAME), [2, 2],
                                                          [2, 2], VALID)), 1),
         "pool_alone_us": round(t_loop(lambda: nd.max_pool2d(hb, [2, 2], [2, 2], VALID)), 1),
-        "layer": "BF16 max_pool2x2(relu(conv2d(x, w) + b)), x[64,56,56,64] w[3,3,64,128]",
     }
-    del hb
+    # its backward up to dz and db: the max-pool backward and the relu / bias
+    # adjoint in one pass against the two calls
+    hy, hp, hpre = nd.conv2d_bias_act_maxpool(xb, wb, bias, [1, 1], SAME, keep_pre=True)
+    dpool = torch.randn(hp.shape, device=dev, generator=g).bfloat16()
+    fused_pool["bwd_fused_us"] = round(t_loop(lambda: nd.maxpool_bias_act_backward(hy, hpre, dpool)), 1)
+    fused_pool["bwd_unfused_us"] = round(t_loop(lambda: nd.bias_act_backward(
+        hpre, nd.max_pool2d_backward(VALID, hy, [2, 2], [2, 2], dpool), act="relu")), 1)
+    fused_pool["layer"] = "BF16 max_pool2x2(relu(conv2d(x, w) + b)), x[64,56,56,64] w[3,3,64,128]"
+    del hb, hy, hp, hpre, dpool
 
     # the FP8 e4m3 forward of the same layer (per-tensor amax scales, FP8
     # operands resident, BF16 output), quantisation outside the timed loop

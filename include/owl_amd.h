@@ -159,6 +159,22 @@ int owl_amd_upsampling2d_bwd(int dt, int64_t B, int64_t H, int64_t W, int64_t C,
 int owl_amd_bias_act_bwd(int dt, int act, int64_t npix, int64_t C, const void *pre,
                          const void *dy, void *dz, void *db, void *stream);
 
+/* Adjoint of pooled = max_pool2d VALID [2;2] [2;2] (relu(z + bias)), the
+ * backward of owl_amd_conv_fwd_bias_act_pool: Owl's max-pool backward
+ * (owl_ndarray_pool_impl.h:102-202: each window's gradient to its first
+ * strict maximum in y) followed by the relu adjoint and bias gradient of
+ * owl_amd_bias_act_bwd, in one pass without the intermediate dy.  y =
+ * relu(pre) and pre are [B][H][W][C] with H and W even, dp is
+ * [B][H/2][W/2][C].  dz is bit-identical to owl_amd_pool_bwd then
+ * owl_amd_bias_act_bwd (F16: up to the sign of zeros, IEEE's -0 for a
+ * negative gradient times a zero factor being kept here); db is summed in
+ * another fixed order (each thread takes a window's four pixels), so it
+ * agrees to rounding and is the same on every call.  act: RELU only.
+ * dt: F32, F64, BF16, F16. */
+int owl_amd_maxpool_bias_act_bwd(int dt, int act, int64_t B, int64_t H, int64_t W, int64_t C,
+                                 const void *y, const void *pre, const void *dp, void *dz,
+                                 void *db, void *stream);
+
 /* FP8 e4m3 forward with per-tensor scales: x = x_q * x_scale, w = w_q * w_scale
  * (the scales are what an amax pass produced; owl_amd_fp8_quantize below).
  * out_t: F32 or BF16. */

@@ -100,6 +100,7 @@ def lib():
             "owl_amd_conv_fwd_bias_act": ([cd, i32, i32, vp, vp, vp, i32, vp, vp, vp], i32),
             "owl_amd_conv_fwd_bias_act_pool": ([cd, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp], i32),
             "owl_amd_bias_act_bwd": ([i32, i32, i64, i64, vp, vp, vp, vp, vp], i32),
+            "owl_amd_maxpool_bias_act_bwd": ([i32, i32, i64, i64, i64, i64, vp, vp, vp, vp, vp, vp], i32),
             "owl_amd_upsampling2d_fwd": ([i32] + [i64] * 8 + [vp, vp, vp], i32),
             "owl_amd_upsampling2d_bwd": ([i32] + [i64] * 8 + [vp, vp, vp], i32),
             "owl_amd_conv_bwd_input": ([cd, i32, i32, vp, vp, vp, vp], i32),

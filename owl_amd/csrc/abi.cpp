@@ -752,6 +752,59 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
 }
 
 // ------------------------------------------------------------------ fused epilogue adjoint
+extern "C" int owl_amd_maxpool_bias_act_bwd(int dt, int act, int64_t B, int64_t H, int64_t W, int64_t C,
+                                           const void *y, const void *pre, const void *dp, void *dz, void *db,
+                                           void *stream) {
+  if (act != OWL_AMD_ACT_RELU) {
+    owl_amd_set_error("the max-pool / relu adjoint takes relu layers (y >= 0: every window has a maximum)");
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  if (B < 0 || H < 0 || W < 0 || C <= 0 || C > INT32_MAX || H > INT32_MAX || W > INT32_MAX)
+    ARG_FAIL("bad sizes B=%lld H=%lld W=%lld C=%lld", (long long)B, (long long)H, (long long)W, (long long)C);
+  if (H % 2 || W % 2) {
+    owl_amd_set_error("the max-pool / relu adjoint takes even H and W (every pixel in one window)");
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  if (dt != OWL_AMD_F32 && dt != OWL_AMD_F64 && dt != OWL_AMD_BF16 && dt != OWL_AMD_F16) {
+    owl_amd_set_error("unsupported dtype %d", dt);
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nwin = B * (H / 2) * (W / 2);
+  if (nwin == 0) {
+    if (db) OWL_CHECK_HIP(hipMemsetAsync(db, 0, (size_t)C * (dt == OWL_AMD_F64 ? 8 : 4), st));
+    return OWL_AMD_OK;
+  }
+  if (!y || !pre || !dp || !dz) ARG_FAIL("null pointer");
+  if (dz == y || dz == pre || dz == dp) ARG_FAIL("dz must not alias y, pre or dp");
+  void *partial = nullptr;
+  if (db) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DeviceState *s;
+    int rc = ensure_state(&s);
+    if (rc) return rc;
+    rc = ensure_buf(s->red, (size_t)maxpool_relu_bwd_chunks(nwin) * (size_t)C * (dt == OWL_AMD_F64 ? 8 : 4), st);
+    if (rc) return rc;
+    partial = s->red.ptr;
+  }
+  KTimer kt(OWL_AMD_KT_AUX, "maxpool_relu_bwd_kernel", st);
+  const int h = (int)H, w = (int)W, c = (int)C;
+  if (dt == OWL_AMD_F32)
+    run_maxpool_relu_bwd<float, float>(B, h, w, c, (const float *)y, (const float *)pre, (const float *)dp,
+                                       (float *)dz, (float *)partial, (float *)db, st);
+  else if (dt == OWL_AMD_F64)
+    run_maxpool_relu_bwd<double, double>(B, h, w, c, (const double *)y, (const double *)pre, (const double *)dp,
+                                         (double *)dz, (double *)partial, (double *)db, st);
+  else if (dt == OWL_AMD_BF16)
+    run_maxpool_relu_bwd<bf16_t, float>(B, h, w, c, (const bf16_t *)y, (const bf16_t *)pre, (const bf16_t *)dp,
+                                        (bf16_t *)dz, (float *)partial, (float *)db, st);
+  else
+    run_maxpool_relu_bwd<f16_t, float>(B, h, w, c, (const f16_t *)y, (const f16_t *)pre, (const f16_t *)dp,
+                                       (f16_t *)dz, (float *)partial, (float *)db, st);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
 extern "C" int owl_amd_bias_act_bwd(int dt, int act, int64_t npix, int64_t C, const void *pre, const void *dy,
                                     void *dz, void *db, void *stream) {
   if (act != OWL_AMD_ACT_NONE && act != OWL_AMD_ACT_RELU) ARG_FAIL("unknown activation %d", act);

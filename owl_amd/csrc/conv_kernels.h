@@ -90,6 +90,10 @@ int bias_act_chunks(int64_t npix);
 template <typename T, typename A>
 void run_bias_act_bwd(int act, int64_t npix, int C, const T *pre, const T *dy, T *dz, A *partial, A *db,
                       hipStream_t st);
+int maxpool_relu_bwd_chunks(int64_t nwin);
+template <typename T, typename A>
+void run_maxpool_relu_bwd(int64_t B, int H, int W, int C, const T *y, const T *pre, const T *dp, T *dz,
+                          A *partial, A *db, hipStream_t st);
 
 // 2-D nearest-neighbour upsampling (resample.hip): backward = false maps
 // src = x [B,H,W,C] -> dst = y [B,Ho,Wo,C]; true maps dy -> dx.

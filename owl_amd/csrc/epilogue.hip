@@ -116,6 +116,118 @@ void run_bias_act_bwd(int act, int64_t npix, int C, const T *pre, const T *dy, T
     launch_k((bias_grad_sum_kernel<A>), dim3((unsigned)C), dim3(256), st, partial, nchunks, C, db);
 }
 
+// Adjoint of pooled = max_pool2d 2x2 stride 2 VALID (relu(pre)) for even H
+// and W: a thread per (window, 16-byte channel vector) reads the window's
+// four y and pre vectors and its dp vector, sends dp to the window's first
+// strict maximum of y in Owl's order ((h, w), (h, w + 1), (h + 1, w),
+// (h + 1, w + 1); dx starts at 0, so the cell receives 0 + dp), multiplies by
+// Owl's relu factor and stores the four dz vectors; y = relu(pre) >= 0, so
+// every window has a maximum.  Bias sums: per thread in window order (the
+// four cells in that order), over lanes in lane order, then over chunks of
+// WCH windows as in the plain adjoint.
+template <typename T, typename A, int VEC>
+__global__ __launch_bounds__(256) void maxpool_relu_bwd_kernel(int64_t nwin, int Wp, int C, int CW, int WCH,
+                                                               const T *__restrict__ y, const T *__restrict__ pre,
+                                                               const T *__restrict__ dp, T *__restrict__ dz,
+                                                               A *__restrict__ partial) {
+  __shared__ A red[256 * VEC];
+  const int t = threadIdx.x;
+  const int cw = t % CW, pl = t / CW, PL = 256 / CW;
+  const int cv = blockIdx.y * CW + cw;
+  const int nv = C / VEC;
+  const int64_t w0 = (int64_t)blockIdx.x * WCH;
+  int64_t w1 = w0 + WCH;
+  if (w1 > nwin) w1 = nwin;
+  const int W = 2 * Wp;
+  A s[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) s[v] = (A)0;
+  if (pl < PL && cv < nv) {
+    const int64_t c0 = (int64_t)cv * VEC;
+    for (int64_t wi = w0 + pl; wi < w1; wi += PL) {
+      const int64_t bi = wi / Wp;  // b * Hp + i
+      const int j = (int)(wi - bi * Wp);
+      const int64_t q0 = (2 * bi) * W + 2 * j;  // (b, 2i, 2j): H = 2 Hp
+      const int64_t cell[4] = {q0, q0 + 1, q0 + W, q0 + W + 1};
+      T yv[4][VEC], zv[4][VEC], d[VEC];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        __builtin_memcpy(yv[k], y + cell[k] * C + c0, sizeof(yv[k]));
+        __builtin_memcpy(zv[k], pre + cell[k] * C + c0, sizeof(zv[k]));
+      }
+      __builtin_memcpy(d, dp + wi * C + c0, sizeof(d));
+      int arg[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        A best = -__builtin_inf();
+        arg[v] = -1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (best < (A)yv[k][v]) {
+            best = (A)yv[k][v];
+            arg[v] = k;
+          }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        T o[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          const A g = arg[v] == k ? (A)(T)((A)0 + (A)d[v]) : (A)0;
+          o[v] = (T)(g * relu_factor<A>((A)zv[k][v]));
+          s[v] += (A)o[v];
+        }
+        __builtin_memcpy(dz + cell[k] * C + c0, o, sizeof(o));
+      }
+    }
+  }
+  if (!partial) return;
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) red[t * VEC + v] = s[v];
+  __syncthreads();
+  if (pl == 0 && cv < nv) {
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      A a = (A)0;
+      for (int l = 0; l < PL; ++l) a += red[(l * CW + cw) * VEC + v];
+      partial[(int64_t)blockIdx.x * C + (int64_t)cv * VEC + v] = a;
+    }
+  }
+}
+
+int maxpool_relu_bwd_chunks(int64_t nwin) { return (int)((nwin + 63) / 64); }
+
+template <typename T, typename A>
+void run_maxpool_relu_bwd(int64_t B, int H, int W, int C, const T *y, const T *pre, const T *dp, T *dz, A *partial,
+                          A *db, hipStream_t st) {
+  constexpr int VEC = 16 / sizeof(T);
+  const int WCH = 64;
+  const int64_t nwin = B * (int64_t)(H / 2) * (W / 2);
+  const int nchunks = maxpool_relu_bwd_chunks(nwin);
+  const bool vec_ok =
+      C % VEC == 0 && ((((uintptr_t)pre | (uintptr_t)y | (uintptr_t)dp | (uintptr_t)dz) & 15) == 0);
+  const int nv = C / (vec_ok ? VEC : 1);
+  int CW = nv < 256 ? nv : 256;
+  while (256 % CW != 0) --CW;
+  dim3 grid((unsigned)nchunks, (unsigned)((nv + CW - 1) / CW));
+  if (vec_ok)
+    launch_k((maxpool_relu_bwd_kernel<T, A, VEC>), grid, dim3(256), st, nwin, W / 2, C, CW, WCH, y, pre, dp, dz,
+             db ? partial : (A *)nullptr);
+  else
+    launch_k((maxpool_relu_bwd_kernel<T, A, 1>), grid, dim3(256), st, nwin, W / 2, C, CW, WCH, y, pre, dp, dz,
+             db ? partial : (A *)nullptr);
+  if (db) launch_k((bias_grad_sum_kernel<A>), dim3((unsigned)C), dim3(256), st, partial, nchunks, C, db);
+}
+
+template void run_maxpool_relu_bwd<float, float>(int64_t, int, int, int, const float *, const float *,
+                                                 const float *, float *, float *, float *, hipStream_t);
+template void run_maxpool_relu_bwd<double, double>(int64_t, int, int, int, const double *, const double *,
+                                                   const double *, double *, double *, double *, hipStream_t);
+template void run_maxpool_relu_bwd<bf16_t, float>(int64_t, int, int, int, const bf16_t *, const bf16_t *,
+                                                  const bf16_t *, bf16_t *, float *, float *, hipStream_t);
+template void run_maxpool_relu_bwd<f16_t, float>(int64_t, int, int, int, const f16_t *, const f16_t *,
+                                                 const f16_t *, f16_t *, float *, float *, hipStream_t);
+
 template void run_bias_act_bwd<float, float>(int, int64_t, int, const float *, const float *, float *, float *,
                                              float *, hipStream_t);
 template void run_bias_act_bwd<double, double>(int, int64_t, int, const double *, const double *, double *,

@@ -688,6 +688,34 @@ def bias_act_backward(pre, output_grad, act="relu", bias_grad=True):
     return dz, db
 
 
+def maxpool_bias_act_backward(y, pre, pooled_grad, bias_grad=True):
+    """Adjoint of ``pooled = max_pool2d(y, [2, 2], [2, 2], VALID)`` with
+    ``y = relu(pre)`` (the backward of ``conv2d_bias_act_maxpool``): returns
+    (dz, db).  dz has the bits of ``max_pool2d_backward`` into dy followed by
+    ``bias_act_backward(pre, dy)``; for even H and W one pass forms it without
+    the intermediate dy (db then summed in another fixed order), otherwise
+    those two calls run.  y and pre are [B, H, W, C], pooled_grad
+    [B, H // 2, W // 2, C]; device tensors only."""
+    _verify(_is_torch(y) and _is_torch(pre) and _is_torch(pooled_grad),
+            "maxpool_bias_act_backward takes device tensors")
+    _verify(y.dim() == 4 and pre.shape == y.shape and pre.dtype == y.dtype and pooled_grad.dtype == y.dtype,
+            "maxpool_bias_act_backward: y and pre must be [B, H, W, C] of one dtype")
+    B, H, W, C = y.shape
+    _verify(list(pooled_grad.shape) == [B, H // 2, W // 2, C],
+            f"maxpool_bias_act_backward: pooled_grad shape {list(pooled_grad.shape)} should be {[B, H // 2, W // 2, C]}")
+    if H % 2 or W % 2:
+        return bias_act_backward(pre, max_pool2d_backward(VALID, y, [2, 2], [2, 2], pooled_grad), "relu", bias_grad)
+    y, pre, dp = _contig(y), _contig(pre), _contig(pooled_grad)
+    dz = torch.empty_like(y)
+    db = None
+    if bias_grad:
+        db = torch.empty(C, dtype=torch.float64 if y.dtype == torch.float64 else torch.float32, device=y.device)
+    _lib.check(_lib.lib().owl_amd_maxpool_bias_act_bwd(
+        _tdt(y), _act_code("relu"), B, H, W, C, _ptr(y), _ptr(pre), _ptr(dp), _ptr(dz),
+        _ptr(db) if db is not None else None, _stream()))
+    return dz, db
+
+
 # ----------------------------------------------------------------------------- upsampling
 def _check_upsampling(name, x, size):
     _verify(len(x.shape) == 4 and len(size) == 2,

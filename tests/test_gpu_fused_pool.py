@@ -164,3 +164,58 @@ def test_cgraph_vgg_block_pools_in_the_conv_epilogue():
         ref = eager(v)
         torch.cuda.synchronize()
         assert torch.equal(bits(got), bits(ref)), rep
+
+
+BWD_TD = {"f32": torch.float32, "f64": torch.float64, "bf16": torch.bfloat16, "f16": torch.float16}
+IBITS = {torch.float32: torch.int32, torch.float64: torch.int64, torch.bfloat16: torch.int16,
+         torch.float16: torch.int16}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtn", ["f64", "f32", "bf16", "f16"])
+@pytest.mark.parametrize("shape", [[2, 56, 56, 64], [3, 8, 10, 20], [2, 7, 9, 24]])
+def test_maxpool_bias_act_backward(shape, dtn):
+    # against max_pool2d_backward into dy then bias_act_backward: dz bit for
+    # bit (odd H / W run those two calls), db within rounding and the same on
+    # every call; FP64 also against the oracle's max-pool backward times Owl's
+    # relu factor (exact) and the FP64 sum of dz
+    print(f"maxpool_bias_act_backward {shape} {dtn}", flush=True)
+    td = BWD_TD[dtn]
+    B, H, W, C = shape
+    rng = np.random.default_rng(90 + H + C)
+    pre_h = rng.standard_normal(shape)
+    flat = pre_h.reshape(-1)
+    flat[rng.choice(flat.size, flat.size // 50, replace=False)] = 0.0      # factor 1/2
+    npix = flat.size // C
+    pre_h.reshape(npix, C)[rng.choice(npix, max(npix // 100, 1), replace=False), 0] = np.nan  # db[0] is NaN
+    pre = dev(pre_h, td)
+    y = torch.fmax(pre, torch.zeros((), dtype=td, device=DEV))
+    dp_h = rng.standard_normal([B, H // 2, W // 2, C])
+    dp_h.reshape(-1)[::97] = -0.0  # Owl adds the gradient to a zeroed dx: -0 arrives as +0
+    dp = dev(dp_h, td)
+
+    dz, db = nd.maxpool_bias_act_backward(y, pre, dp)
+    dz2, db2 = nd.maxpool_bias_act_backward(y, pre, dp)
+    dy = nd.max_pool2d_backward(VALID, y, [2, 2], [2, 2], dp)
+    dz_ref, db_ref = nd.bias_act_backward(pre, dy, act="relu")
+    torch.cuda.synchronize()
+    ib, ibd = IBITS[td], IBITS[db.dtype]
+    if td == torch.float16:
+        # FP16: the two-call path's compiled adjoint gives +0 for a negative
+        # gradient times a zero factor where this pass keeps IEEE's -0; the
+        # values and NaNs agree
+        assert torch.equal(torch.isnan(dz), torch.isnan(dz_ref))
+        assert torch.equal(torch.nan_to_num(dz), torch.nan_to_num(dz_ref))
+    else:
+        assert torch.equal(dz.view(ib), dz_ref.view(ib))
+    assert torch.equal(dz2.view(ib), dz.view(ib)) and torch.equal(db2.view(ibd), db.view(ibd))
+    dbh, dbr = host(db), host(db_ref)
+    ok = ~np.isnan(dbr)
+    assert np.array_equal(np.isnan(dbh), ~ok)
+    tol = 1e-12 if dtn == "f64" else 1e-5
+    assert np.abs(dbh[ok] - dbr[ok]).max() <= tol * max(np.abs(dbr[ok]).max(), 1.0)
+    if dtn == "f64":
+        fac = (np.sign(pre_h) + 1.0) / 2.0
+        ref = on.max_pool2d_backward(VALID, host(y), [2, 2], [2, 2], host(dp)) * fac
+        np.testing.assert_array_equal(host(dz), ref)
+        np.testing.assert_allclose(dbh[ok], ref.reshape(-1, C).sum(axis=0)[ok], rtol=1e-12, atol=1e-12)
