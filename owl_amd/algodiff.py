@@ -203,3 +203,33 @@ def conv2d_bias_relu(x, w, b, stride, padding=SAME):
                 out.append(db.to(bb.dtype))
         return out
     return build_aiso("conv2d_bias_relu", ff, dr)(x, w, b)
+
+
+def conv2d_bias_relu_maxpool(x, w, b, stride, padding=SAME):
+    """The conv layer with a relu followed by Owl's 2x2 stride-2 VALID
+    max-pool layer (owl_neural_neuron.ml:1253, :2024), as one ``build_aiso``
+    op: the forward pools in the fused conv's epilogue (y and the
+    pre-activation kept for the adjoint); the adjoint is
+    ``maxpool_bias_act_backward`` (max-pool backward, relu factor and bias
+    gradient in one pass) then the two conv backward passes."""
+    saved = {}
+
+    def ff(ps):
+        a, k, bb = ps
+        y, pooled, pre = nd.conv2d_bias_act_maxpool(a, k, bb, stride, padding, act="relu", keep_pre=True)
+        saved["y"], saved["pre"] = y, pre
+        return pooled
+
+    def dr(idxs, ps, cp, ca):
+        a, k, bb = ps
+        dz, db = nd.maxpool_bias_act_backward(saved["y"], saved["pre"], ca, bias_grad=2 in idxs)
+        out = []
+        for i in idxs:
+            if i == 0:
+                out.append(nd.conv2d_backward_input(a, k, stride, dz))
+            elif i == 1:
+                out.append(nd.conv2d_backward_kernel(a, k, stride, dz, out_dtype=k.dtype))
+            else:
+                out.append(db.to(bb.dtype))
+        return out
+    return build_aiso("conv2d_bias_relu_maxpool", ff, dr)(x, w, b)

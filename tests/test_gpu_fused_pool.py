@@ -219,3 +219,29 @@ def test_maxpool_bias_act_backward(shape, dtn):
         ref = on.max_pool2d_backward(VALID, host(y), [2, 2], [2, 2], host(dp)) * fac
         np.testing.assert_array_equal(host(dz), ref)
         np.testing.assert_allclose(dbh[ok], ref.reshape(-1, C).sum(axis=0)[ok], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hw", [(8, 10), (9, 7)])
+def test_algodiff_conv_bias_relu_maxpool_vs_torch_autograd(hw):
+    # gradients of sum_sqr(maxpool(relu(conv(x, w) + b))) against torch's FP64
+    # autograd of the same layers (NCHW); even sizes take the one-pass
+    # max-pool / relu adjoint, odd ones the two calls
+    from owl_amd import algodiff as ad
+    H, W = hw
+    rng = np.random.default_rng(5 + H)
+    x = rng.standard_normal([2, H, W, 6])
+    w = rng.standard_normal([3, 3, 6, 5]) / 5
+    b = rng.standard_normal(5) * 0.3
+    xt, wt, bt = (dev(a, torch.float64) for a in (x, w, b))
+    gx, gw, gb = ad.grad(lambda a, k, bb: ad.sum_sqr(ad.conv2d_bias_relu_maxpool(a, k, bb, [1, 1], SAME)),
+                         xt, wt, bt)
+    X = torch.tensor(x.transpose(0, 3, 1, 2).copy(), requires_grad=True)
+    Wt = torch.tensor(w.transpose(3, 2, 0, 1).copy(), requires_grad=True)
+    Bt = torch.tensor(b.copy(), requires_grad=True)
+    y = torch.nn.functional.max_pool2d(torch.relu(torch.nn.functional.conv2d(X, Wt, Bt, padding=1)), 2, 2)
+    (y * y).sum().backward()
+    r = lambda a, b_: np.linalg.norm(a - b_) / np.linalg.norm(b_)
+    assert r(host(gx), X.grad.numpy().transpose(0, 2, 3, 1)) < 1e-12
+    assert r(host(gw), Wt.grad.numpy().transpose(2, 3, 1, 0)) < 1e-12
+    assert r(host(gb), Bt.grad.numpy()) < 1e-12
