@@ -2,14 +2,16 @@
 
 Same signatures as ``owl_amd.ndarray`` (numpy in, numpy out).  Geometry
 follows Owl's wrappers (owl_dense_ndarray_generic.ml) with the formulas of
-``owl_amd.shapes``; the arithmetic is the oracle's FP64 direct loops (conv)
-or the bit-exact pooling restatement in the input's own dtype.
+``owl_geometry`` (restated apart from the product's ``owl_amd.shapes``); the
+arithmetic is the oracle's FP64 direct loops (conv) or the bit-exact pooling
+restatement in the input's own dtype.
 Test infrastructure only.
 """
 import numpy as np
 
 import oracle
-from owl_amd.shapes import SAME, c_pad, calc_conv2d_padding, conv_out_dim, k_eff, transpose_out_dim
+from owl_geometry import SAME, dilated_extent as k_eff, lead_pad as c_pad, out_dim as conv_out_dim, \
+    transpose_dim as transpose_out_dim, wrapper_pads_2d as calc_conv2d_padding
 
 
 def _geom(B, sp_in, Ci, ksp, Co, sp_out, stride, rate, pads):

@@ -10,7 +10,8 @@ import pytest
 
 import oracle
 import oracle_ndarray as on
-from owl_amd.shapes import SAME, VALID, c_pad, conv_out_dim, k_eff
+from owl_geometry import SAME, VALID, dilated_extent as k_eff, lead_pad as c_pad, out_dim as conv_out_dim
+from owl_geometry import transpose_dim, wrapper_pads_2d
 
 
 def brute_conv(x, w, stride, rate, pads, out_sp):
@@ -183,3 +184,35 @@ def test_oracle_pool_ties_nan_and_orphans():
     dx = on.max_pool2d_backward(VALID, x, [2, 2], [2, 2], dy).ravel()
     assert dx[1] == 1 and dx[6] == 2 and dx[15] == 8 and dx[0] == 4
     assert dx.sum() == 15
+
+
+def test_geometry_matches_product():
+    """The product's shape / pad formulas (owl_amd.shapes, Owl's float form)
+    against the test side's integer restatement, over every small size,
+    kernel, stride and rate; plus the textbook special cases: SAME at stride 1
+    keeps the size and pads (k_eff - 1) / 2 in front for odd extents; VALID at
+    stride 1 gives n - k_eff + 1 and no pad."""
+    from owl_amd import shapes as P
+    for n in range(1, 40):
+        for k in range(1, 8):
+            for s in range(1, 6):
+                for r in range(1, 4):
+                    ke = k_eff(k, r)
+                    assert P.k_eff(k, r) == ke
+                    for pad in (SAME, VALID):
+                        o = conv_out_dim(pad, n, ke, s)
+                        assert P.conv_out_dim(pad, n, ke, s) == o, (pad, n, ke, s)
+                        assert P.transpose_out_dim(pad, n, k, s) == transpose_dim(pad, n, k, s)
+                        if o > 0:
+                            assert P.c_pad(s, o, ke, n) == c_pad(s, o, ke, n), (n, ke, s, o)
+                    assert P.calc_conv2d_padding(n, n + 1, k, ke, conv_out_dim(SAME, n, k, s),
+                                                 conv_out_dim(SAME, n + 1, ke, s + 1), s + 1, s) == \
+                        wrapper_pads_2d(n, n + 1, k, ke, conv_out_dim(SAME, n, k, s),
+                                        conv_out_dim(SAME, n + 1, ke, s + 1), s + 1, s)
+                    if s == 1:
+                        assert conv_out_dim(SAME, n, ke, 1) == n
+                        if ke % 2 == 1:
+                            assert c_pad(1, n, ke, n) == (ke - 1) // 2
+                        if n >= ke:
+                            assert conv_out_dim(VALID, n, ke, 1) == n - ke + 1
+                            assert c_pad(1, n - ke + 1, ke, n) == 0
