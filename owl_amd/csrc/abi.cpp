@@ -193,6 +193,10 @@ struct DeviceState {
 // again for the split-K workspace and the pooling flag.
 std::recursive_mutex g_mu;
 DeviceState g_dev[64];
+// Library scratch buffers replaced by larger ones.  A HIP graph captured
+// earlier may still hold their addresses, so they are only freed by
+// owl_amd_cache_release (which invalidates such graphs); guarded by g_mu.
+std::vector<void *> g_retired;
 
 int current_device(int *dev) {
   hipError_t e = hipGetDevice(dev);
@@ -220,16 +224,26 @@ int ensure_state(DeviceState **out) {
   return OWL_AMD_OK;
 }
 
-// Grow-only device buffer.  Growing waits for the stream that used it.
-int ensure_buf(DevBuf &b, size_t bytes, hipStream_t st) {
+// Grow-only device buffer; the caller holds g_mu.  A scratch buffer that
+// device-API calls hand to kernels (retire = true) is kept alive when it is
+// replaced -- a captured graph may still use it -- and grows at least 2x so
+// that a run of slowly growing calls retires few buffers.  The drop-ins'
+// staging slots (used only inside one synchronous call) are freed at once,
+// after the stream that used them has finished.
+int ensure_buf(DevBuf &b, size_t bytes, hipStream_t st, bool retire = true) {
   if (b.bytes >= bytes) return OWL_AMD_OK;
+  size_t want = bytes < 256 ? 256 : bytes;
   if (b.ptr) {
-    OWL_CHECK_HIP(hipStreamSynchronize(st));
-    OWL_CHECK_HIP(hipFree(b.ptr));
+    if (retire) {
+      g_retired.push_back(b.ptr);
+      if (want < 2 * b.bytes) want = 2 * b.bytes;
+    } else {
+      OWL_CHECK_HIP(hipStreamSynchronize(st));
+      OWL_CHECK_HIP(hipFree(b.ptr));
+    }
     b.ptr = nullptr;
     b.bytes = 0;
   }
-  size_t want = bytes < 256 ? 256 : bytes;
   hipError_t e = hipMalloc(&b.ptr, want);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -455,17 +469,22 @@ extern "C" int owl_amd_cache_stats(int64_t *n, int64_t *bytes) {
 
 extern "C" void owl_amd_cache_release(void) {
   std::lock_guard<std::recursive_mutex> lk(g_mu);
-  for (auto &s : g_dev) {
+  int cur = -1;
+  (void)hipGetDevice(&cur);
+  for (int dev = 0; dev < 64; ++dev) {
+    DeviceState &s = g_dev[dev];
     if (!s.ready) continue;
+    if (hipSetDevice(dev) != hipSuccess) continue;
     (void)hipStreamSynchronize(s.stream);
     (void)hipDeviceSynchronize();
-    for (auto &b : s.slot) {
-      if (b.ptr) (void)hipFree(b.ptr);
-      b = DevBuf();
+    for (DevBuf *b : {&s.slot[0], &s.slot[1], &s.slot[2], &s.slot[3], &s.ws, &s.red, &s.cplx, &s.f8w, &s.ytmp}) {
+      if (b->ptr) (void)hipFree(b->ptr);
+      *b = DevBuf();
     }
-    if (s.ws.ptr) (void)hipFree(s.ws.ptr);
-    s.ws = DevBuf();
   }
+  for (void *p : g_retired) (void)hipFree(p);
+  g_retired.clear();
+  if (cur >= 0) (void)hipSetDevice(cur);
 }
 
 // ------------------------------------------------------------------ complex kinds
@@ -691,7 +710,8 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   if (p.M == 0 || p.N == 0) return OWL_AMD_OK;
   if (!x || !dy || !dw) ARG_FAIL("null pointer");
   if (p.K == 0) {
-    OWL_CHECK_HIP(hipMemsetAsync(dw, 0, (size_t)p.M * p.N * dtype_size(out_t), st));
+    run_zero_fill(dw, (size_t)p.M * p.N * dtype_size(out_t), st);
+    OWL_CHECK_HIP(hipGetLastError());
     return OWL_AMD_OK;
   }
   plan_splitk(p, in_t, &p.split, &p.k_per_split);
@@ -772,7 +792,8 @@ extern "C" int owl_amd_maxpool_bias_act_bwd(int dt, int act, int64_t B, int64_t 
   hipStream_t st = (hipStream_t)stream;
   const int64_t nwin = B * (H / 2) * (W / 2);
   if (nwin == 0) {
-    if (db) OWL_CHECK_HIP(hipMemsetAsync(db, 0, (size_t)C * (dt == OWL_AMD_F64 ? 8 : 4), st));
+    if (db) run_zero_fill(db, (size_t)C * (dt == OWL_AMD_F64 ? 8 : 4), st);
+    OWL_CHECK_HIP(hipGetLastError());
     return OWL_AMD_OK;
   }
   if (!y || !pre || !dp || !dz) ARG_FAIL("null pointer");
@@ -813,12 +834,13 @@ extern "C" int owl_amd_bias_act_bwd(int dt, int act, int64_t npix, int64_t C, co
     owl_amd_set_error("unsupported dtype %d", dt);
     return OWL_AMD_ERR_UNSUPPORTED;
   }
-  if (!dy || !dz || (act == OWL_AMD_ACT_RELU && !pre)) ARG_FAIL("null pointer");
   hipStream_t st = (hipStream_t)stream;
-  if (npix == 0) {
-    if (db) OWL_CHECK_HIP(hipMemsetAsync(db, 0, (size_t)C * (dt == OWL_AMD_F64 ? 8 : 4), st));
+  if (npix == 0) {  // an empty layer: pre, dy and dz may be null
+    if (db) run_zero_fill(db, (size_t)C * (dt == OWL_AMD_F64 ? 8 : 4), st);
+    OWL_CHECK_HIP(hipGetLastError());
     return OWL_AMD_OK;
   }
+  if (!dy || !dz || (act == OWL_AMD_ACT_RELU && !pre)) ARG_FAIL("null pointer");
   void *partial = nullptr;
   if (db) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
@@ -866,7 +888,8 @@ static int upsample(bool backward, int dt, int64_t B, int64_t H, int64_t W, int6
   if (!dst || (!src && (backward ? B * Ho * Wo * C : B * H * W * C) > 0)) ARG_FAIL("null pointer");
   if (H == 0 || W == 0) ARG_FAIL("empty input image with a non-empty output");
   if (backward && (Ho == 0 || Wo == 0)) {
-    OWL_CHECK_HIP(hipMemsetAsync(dst, 0, (size_t)n_dst * dtype_size(dt), st));
+    run_zero_fill(dst, (size_t)n_dst * dtype_size(dt), st);
+    OWL_CHECK_HIP(hipGetLastError());
     return OWL_AMD_OK;
   }
   KTimer kt(OWL_AMD_KT_AUX, backward ? "upsample_bwd_kernel" : "upsample_fwd_kernel", st);
@@ -1110,7 +1133,7 @@ int run_dropin(std::initializer_list<HostArg> args, int out_slot, void *out_host
   void *dev[4] = {nullptr, nullptr, nullptr, nullptr};
   int i = 0;
   for (const HostArg &a : args) {
-    rc = ensure_buf(s->slot[i], a.bytes, s->stream);
+    rc = ensure_buf(s->slot[i], a.bytes, s->stream, false);
     if (rc) return rc;
     dev[i] = s->slot[i].ptr;
     if (a.src && a.bytes)

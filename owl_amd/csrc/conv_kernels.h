@@ -84,6 +84,10 @@ void run_wgrad(const ConvParams &p, hipStream_t st);
 template <typename TI> int tile_bm();
 template <typename TI> int tile_bk();
 
+// Zero fill of bytes at p on st (epilogue.hip): a kernel, not a memset node
+// when captured.
+void run_zero_fill(void *p, size_t bytes, hipStream_t st);
+
 // Adjoint of the fused bias / activation epilogue (epilogue.hip); partial
 // holds bias_act_chunks(npix) x C values of A when db is wanted.
 int bias_act_chunks(int64_t npix);

@@ -238,3 +238,31 @@ template void run_bias_act_bwd<f16_t, float>(int, int64_t, int, const f16_t *, c
                                              float *, hipStream_t);
 
 }  // namespace owl_amd
+
+namespace owl_amd {
+// Zero fill of [p, p + bytes): 16-byte stores from the first 16-byte boundary,
+// single bytes before it and after the last whole chunk.  Degenerate-shape
+// results are cleared with it instead of hipMemsetAsync, so a HIP graph that
+// captures such a call records a kernel node, never a memset node.
+__global__ void zero_fill_kernel(unsigned char *p, uint64_t head, uint64_t chunks, uint64_t bytes) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t c = i; c < chunks; c += n) *(uint4 *)(p + head + 16 * c) = make_uint4(0, 0, 0, 0);
+  // the head bytes and the tail bytes (fewer than 16 each)
+  const uint64_t tail0 = head + 16 * chunks;
+  if (i < head) p[i] = 0;
+  if (i < bytes - tail0) p[tail0 + i] = 0;
+}
+
+void run_zero_fill(void *p, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return;
+  unsigned char *b = (unsigned char *)p;
+  uint64_t head = (16 - ((uintptr_t)b & 15)) & 15;
+  if (head > bytes) head = bytes;
+  const uint64_t chunks = (bytes - head) / 16;
+  uint64_t blocks = (chunks + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(zero_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, st, b, head, chunks, (uint64_t)bytes);
+}
+}  // namespace owl_amd

@@ -201,3 +201,84 @@ def test_pooling_16bit_rounds_the_fp32_result_once(dt):
         ya, arg = nd.max_pool2d_argmax(x, kern, stride, pad)
         ra, rarg = nd.max_pool2d_argmax(x.float(), kern, stride, pad)
         assert torch.equal(ya, ra.to(dt)) and torch.equal(arg, rarg)
+
+
+@pytest.mark.gpu
+def test_captured_fused_pool_survives_larger_library_calls():
+    """An FP32 conv + bias + relu + 2x2 max-pool layer without y runs the
+    convolution into a library scratch buffer and then the pooling kernel, so
+    a captured graph of it holds that buffer's address.  Larger calls of the
+    same kind made after the capture need a larger buffer; the captured one
+    must stay valid: every replay equals the eager layer bit for bit."""
+    dt = torch.float32
+    gx, gw, gb = G.var_arr("x", [2, 16, 16, 32], dt), G.var_arr("w", [3, 3, 32, 64], dt), G.var_arr("b", [64], dt)
+    gy = G.max_pool2d(G.relu(G.add(G.conv2d(gx, gw, [1, 1]), gb)), [2, 2], [2, 2], VALID)
+    g = G.optimise(G.make_graph([gx, gw, gb], [gy], "layer"))
+    assert "conv2d_bias_act_maxpool" in [n.op for n in g.order]
+    assign([gx, gw, gb], values(dt, 40)[:3])
+    G.capture_graph(g)
+    gen = torch.Generator(device=DEV)
+    gen.manual_seed(41)
+    for rep in range(3):
+        # a larger keep_y=False layer between replays, growing each time
+        big = torch.randn(4 + 4 * rep, 32, 32, 32, device=DEV, generator=gen)
+        wv, bv = values(dt, 42)[1:3]
+        _, pb = nd.conv2d_bias_act_maxpool(big, wv, bv, [1, 1], SAME, keep_y=False)
+        ref_big = nd.max_pool2d(nd.conv2d_bias_act(big, wv, bv, [1, 1], SAME), [2, 2], [2, 2], VALID)
+        vals = values(dt, 50 + rep)[:3]
+        assign([gx, gw, gb], vals)
+        got = G.replay_graph(g)[0].clone()
+        ref = nd.max_pool2d(nd.conv2d_bias_act(vals[0], vals[1], vals[2], [1, 1], SAME), [2, 2], [2, 2], VALID)
+        torch.cuda.synchronize()
+        assert torch.equal(pb, ref_big), rep
+        assert torch.equal(got, ref), rep
+
+
+def _block(dt, with_h2):
+    x, w1, b1 = G.var_arr("x", [2, 14, 14, 16], dt), G.var_arr("w1", [3, 3, 16, 32], dt), G.var_arr("b1", [32], dt)
+    w2, b2 = G.var_arr("w2", [3, 3, 32, 32], dt), G.var_arr("b2", [32], dt)
+    h1 = G.relu(G.add(G.conv2d(x, w1, [1, 1]), b1))
+    h2 = G.relu(G.add(G.conv2d(h1, w2, [1, 1]), b2))
+    p = G.max_pool2d(h2, [2, 2], [2, 2], VALID)
+    return [x, w1, b1, w2, b2], ([p, h2] if with_h2 else [p])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dt,tol", [(torch.float32, 1e-5), (torch.float64, 1e-12)], ids=["fp32", "fp64"])
+@pytest.mark.parametrize("opt", [False, True], ids=["built", "optimised"])
+def test_graph_block_matches_oracle(dt, tol, opt):
+    """A VGG block (conv + bias + relu twice, then a 2x2 stride-2 VALID
+    max-pool) evaluated as a graph, as built and after ``optimise`` (which
+    fuses the layers and, when h2 is not an output, the pooling), against the
+    FP64 oracle's composition (owl_computation_cpu_eval.ml:180-190 dispatches
+    the same operators to the CPU kernels): the pooled output within tol of
+    the oracle; and, with h2 kept as an output, the pooled output equal bit
+    for bit to the oracle's max-pool of the GPU's h2."""
+    import oracle_ndarray as on
+    g = torch.Generator(device=DEV)
+    g.manual_seed(60)
+    r = lambda *s, scale=1.0: (torch.randn(*s, device=DEV, generator=g, dtype=torch.float64) * scale).to(dt)
+    vals = [r(2, 14, 14, 16), r(3, 3, 16, 32, scale=0.12), r(32, scale=0.2), r(3, 3, 32, 32, scale=0.08),
+            r(32, scale=0.2)]
+    h = [v.double().cpu().numpy() for v in vals]
+    r1 = np.maximum(on.conv2d(h[0], h[1], [1, 1], SAME) + h[2], 0.0)
+    r2 = np.maximum(on.conv2d(r1, h[3], [1, 1], SAME) + h[4], 0.0)
+    rp = on.max_pool2d(r2, [2, 2], [2, 2], VALID)
+    for with_h2 in (False, True):
+        ins, outs = _block(dt, with_h2)
+        graph = G.make_graph(ins, outs, "block")
+        if opt:
+            G.optimise(graph)
+            ops = [n.op for n in graph.order]
+            assert ops.count("conv2d_bias_act") == (2 if with_h2 else 1), ops
+            assert ("conv2d_bias_act_maxpool" in ops) == (not with_h2), ops
+        assign(ins, vals)
+        got = G.eval_graph(graph)
+        torch.cuda.synchronize()
+        p = got[0].cpu().numpy()
+        assert p.dtype == np.dtype(str(dt).split(".")[1])
+        assert float(np.linalg.norm(p - rp) / np.linalg.norm(rp)) < tol, with_h2
+        if with_h2:
+            h2 = got[1].cpu().numpy()
+            assert float(np.linalg.norm(h2 - r2) / np.linalg.norm(r2)) < tol
+            assert np.array_equal(p, on.max_pool2d(h2, [2, 2], [2, 2], VALID))
