@@ -380,42 +380,78 @@ def main():
     print(json.dumps(out))
 
 
-def cpu_baseline(budget_s=12.0):
-    """Owl's CPU algorithm (im2col + OpenBLAS SGEMM, restated in oracle/) on
-    the whole layer (all 64 images), all host threads: full steps repeated
-    until about budget_s seconds of CPU work, median step time."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(budget_s=10.0, budget_1t_s=10.0):
+    """Owl's FP32 CPU algorithm as the oracle restates it: for this layer
+    (im2col 200704 x 576 >= 512 Ki elements) the cache-blocked GEBP loops
+    with Owl's block sizes from the host's CPUID caches, serial packing and
+    scatter, and OpenBLAS SGEMM per block.  Whole layer (64 images) on all
+    host threads, full steps repeated for about budget_s seconds, median;
+    then one BLAS thread on an 8-image sample (still the GEBP branch) for
+    about budget_1t_s seconds, median, in TFLOP/s of that sample."""
+    import ctypes
     import oracle
     rng = np.random.default_rng(0)
     x = rng.standard_normal([B, H, W, CI]).astype(np.float32)
     w = (rng.standard_normal([K, K, CI, CO]) * (2.0 / (K * K * CI)) ** 0.5).astype(np.float32)
     dy = rng.standard_normal([B, H, W, CO]).astype(np.float32)
-    g = oracle.conv_geom([B, H, W, CI], [K, K, CI, CO], [H, W], [1, 1], [1, 1], [1, 1])
     lib, _, path = oracle.openblas_sgemm()
-    threads = None
-    for sym in ("scipy_openblas_get_num_threads", "openblas_get_num_threads"):
-        f = getattr(lib, sym, None)
-        if f is not None:
-            threads = int(f())
-            break
+    get_t = getattr(lib, "scipy_openblas_get_num_threads", None) or getattr(lib, "openblas_get_num_threads", None)
+    set_t = getattr(lib, "scipy_openblas_set_num_threads", None) or getattr(lib, "openblas_set_num_threads", None)
+    cfg = getattr(lib, "scipy_openblas_get_config", None) or getattr(lib, "openblas_get_config", None)
+    blas = None
+    if cfg is not None:
+        cfg.restype = ctypes.c_char_p
+        blas = cfg().decode(errors="replace")
+    threads = int(get_t()) if get_t is not None else os.cpu_count()
 
-    def run():
-        oracle.conv_fwd_owl_f32(g, x, w)
-        oracle.conv_bwd_input_owl_f32(g, w, dy)
-        oracle.conv_bwd_kernel_owl_f32(g, x, dy)
-    run()
-    ts = []
-    t_start = time.perf_counter()
-    while time.perf_counter() - t_start < budget_s or len(ts) < 3:
-        t0 = time.perf_counter()
+    def bench(nimg, budget):
+        g = oracle.conv_geom([nimg, H, W, CI], [K, K, CI, CO], [H, W], [1, 1], [1, 1], [1, 1])
+        xs, dys = x[:nimg], dy[:nimg]
+        br = []
+
+        def run():
+            oracle.conv_fwd_owl_f32(g, xs, w, None, br)
+            oracle.conv_bwd_input_owl_f32(g, w, dys, None, br)
+            oracle.conv_bwd_kernel_owl_f32(g, xs, dys, None, br)
         run()
-        ts.append(time.perf_counter() - t0)
-    t = statistics.median(ts)
-    flop = 3 * FLOP_PASS
-    return {"value": round(flop / t / 1e12, 4), "unit": "TFLOP/s", "cores": threads or os.cpu_count(),
-            "kind": "port", "sample": f"the whole layer (64 images), fwd+bwd_input+bwd_kernel, FP32 im2col + "
-                                      f"OpenBLAS sgemm ({os.path.basename(path)}), {len(ts)} steps in "
-                                      f"{sum(ts):.1f} s, median step",
-            "seconds_per_step": round(t, 3)}
+        ts = []
+        t_start = time.perf_counter()
+        while time.perf_counter() - t_start < budget or len(ts) < 3:
+            t0 = time.perf_counter()
+            run()
+            ts.append(time.perf_counter() - t0)
+        t = statistics.median(ts)
+        flop = 3 * FLOP_PASS * nimg / B
+        return flop / t / 1e12, t, len(ts), sum(ts), sorted(set(br))
+
+    tf, t, n, tot, br = bench(B, budget_s)
+    one = None
+    if set_t is not None:
+        set_t(1)
+        try:
+            tf1, t1, n1, tot1, br1 = bench(8, budget_1t_s)
+        finally:
+            set_t(threads)
+        one = {"value": round(tf1, 5), "unit": "TFLOP/s", "cores": 1, "seconds_per_step": round(t1, 3),
+               "sample": f"8 images of the layer ({'/'.join(br1)} branch), fwd+bwd_input+bwd_kernel, "
+                         f"{n1} steps in {tot1:.1f} s, median step"}
+    return {"value": round(tf, 4), "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"the whole layer (64 images), fwd+bwd_input+bwd_kernel, Owl's FP32 {'/'.join(br)} branch "
+                      f"(owl_ndarray_conv_impl.h:340-452, 553-663, 774-887; block sizes from CPUID caches "
+                      f"{oracle.host_cache_sizes()}) with OpenBLAS sgemm ({os.path.basename(path)}), "
+                      f"{n} steps in {tot:.1f} s, median step",
+            "seconds_per_step": round(t, 3), "one_thread": one, "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "blas": blas}
 
 
 if __name__ == "__main__":

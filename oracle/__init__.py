@@ -48,6 +48,13 @@ def lib():
                   "oracle_conv_bwd_kernel_owl_f32"):
             getattr(L, n).argtypes = [ctypes.POINTER(ConvGeom), vp, vp, vp, vp]
             getattr(L, n).restype = ctypes.c_int
+            getattr(L, n + "_cache").argtypes = [ctypes.POINTER(ConvGeom), vp, vp, vp, vp, vp]
+            getattr(L, n + "_cache").restype = ctypes.c_int
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_owl_block_sizes.argtypes = [ip, ip, ip] + [ctypes.c_int] * 4
+        L.oracle_owl_block_sizes.restype = None
+        L.oracle_host_cache_sizes.argtypes = [ip, ip, ip]
+        L.oracle_host_cache_sizes.restype = None
         for s in ("f32", "f64"):
             getattr(L, "oracle_pool_fwd_" + s).argtypes = [ctypes.POINTER(PoolGeom), ctypes.c_int, vp, vp]
             getattr(L, "oracle_pool_argmax_" + s).argtypes = [ctypes.POINTER(PoolGeom), vp, vp, vp]
@@ -142,31 +149,61 @@ def openblas_sgemm():
     return _SGEMM
 
 
-def conv_fwd_owl_f32(g, x, w):
+def _cache_arg(cache):
+    return None if cache is None else (ctypes.c_int * 3)(*cache)
+
+
+def conv_fwd_owl_f32(g, x, w, cache=None, branch=None):
+    """Owl's FP32 forward (im2col + SGEMM, or the GEBP blocks for large 2-D
+    layers).  cache: (l1, l2, l3) bytes for the block sizes, else the host's
+    CPUID values.  branch: a list that receives "gebp" or "im2col"."""
     x = np.ascontiguousarray(x, np.float32)
     w = np.ascontiguousarray(w, np.float32)
     y = np.zeros(g.B * g.Ho * g.Wo * g.Do * g.Co, np.float32)
-    rc = lib().oracle_conv_fwd_owl_f32(ctypes.byref(g), _p(x), _p(w), _p(y), openblas_sgemm()[1])
-    assert rc == 0
+    rc = lib().oracle_conv_fwd_owl_f32_cache(ctypes.byref(g), _p(x), _p(w), _p(y), openblas_sgemm()[1],
+                                             _cache_arg(cache))
+    assert rc >= 0
+    if branch is not None:
+        branch.append("gebp" if rc == 1 else "im2col")
     return y
 
 
-def conv_bwd_input_owl_f32(g, w, dy):
+def conv_bwd_input_owl_f32(g, w, dy, cache=None, branch=None):
     w = np.ascontiguousarray(w, np.float32)
     dy = np.ascontiguousarray(dy, np.float32)
     dx = np.zeros(g.B * g.H * g.W * g.D * g.Ci, np.float32)
-    rc = lib().oracle_conv_bwd_input_owl_f32(ctypes.byref(g), _p(w), _p(dy), _p(dx), openblas_sgemm()[1])
-    assert rc == 0
+    rc = lib().oracle_conv_bwd_input_owl_f32_cache(ctypes.byref(g), _p(w), _p(dy), _p(dx), openblas_sgemm()[1],
+                                                   _cache_arg(cache))
+    assert rc >= 0
+    if branch is not None:
+        branch.append("gebp" if rc == 1 else "im2col")
     return dx
 
 
-def conv_bwd_kernel_owl_f32(g, x, dy):
+def conv_bwd_kernel_owl_f32(g, x, dy, cache=None, branch=None):
     x = np.ascontiguousarray(x, np.float32)
     dy = np.ascontiguousarray(dy, np.float32)
     dw = np.zeros(g.kh * g.kw * g.kd * g.Ci * g.Co, np.float32)
-    rc = lib().oracle_conv_bwd_kernel_owl_f32(ctypes.byref(g), _p(x), _p(dy), _p(dw), openblas_sgemm()[1])
-    assert rc == 0
+    rc = lib().oracle_conv_bwd_kernel_owl_f32_cache(ctypes.byref(g), _p(x), _p(dy), _p(dw), openblas_sgemm()[1],
+                                                    _cache_arg(cache))
+    assert rc >= 0
+    if branch is not None:
+        branch.append("gebp" if rc == 1 else "im2col")
     return dw
+
+
+def owl_block_sizes(k, m, n, typesize, l1, l2, l3):
+    """Owl's compute_block_sizes for (k, m, n) at the given cache sizes."""
+    a, b, c = ctypes.c_int(k), ctypes.c_int(m), ctypes.c_int(n)
+    lib().oracle_owl_block_sizes(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), typesize, l1, l2, l3)
+    return a.value, b.value, c.value
+
+
+def host_cache_sizes():
+    """(l1, l2, l3) bytes as Owl's query_cache_sizes reads them on this host."""
+    v = [ctypes.c_int() for _ in range(3)]
+    lib().oracle_host_cache_sizes(*[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
 
 
 # ---------------------------------------------------------------- pooling
