@@ -44,7 +44,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-check", action="store_true", help="skip the FP64 oracle error check")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 error checks")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other workloads (ResNet stack, ragged "
+                    "batch, conv3d, small-layer latency)")
     return ap.parse_args()
 
 
@@ -339,6 +341,13 @@ def main():
                       "FP32 kernels + D2H copies per call, median of 5 steps"}
     del xh, wh, dyh
 
+    configs = None
+    if not args.no_configs:
+        configs = {"config1_latency": bench_config1(args.steps),
+                   "config3_resnet_stack": bench_config3(args.steps, check=not args.no_check),
+                   "config4_ragged": bench_config4(args.steps),
+                   "config5_conv3d": bench_config5(args.steps)}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline()
@@ -374,10 +383,353 @@ def main():
         "dropin_fp32_host_buffers": dropin,
         "fp8_e4m3_fwd": fp8,
         "kernels_bf16": kernels,
+        "configs": configs,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _ev_time(f, reps, warm=3):
+    """Average milliseconds of f() over reps back-to-back calls between two
+    events on the current stream, after warm untimed calls."""
+    import torch
+    for _ in range(warm):
+        f()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        f()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _kernel_avgs(f, reps):
+    """The library's per-kernel timing (HIP events stamped with each main
+    kernel's own start and end) over reps calls of f: {class: (avg ms, name)}."""
+    import torch
+    from owl_amd import _lib
+    _lib.kernel_timing(True)
+    for _ in range(reps):
+        f()
+    torch.cuda.synchronize()
+    out = {}
+    for cls, tag in ((_lib.KT_FWD, "fwd"), (_lib.KT_BWD_INPUT, "bwd_input"), (_lib.KT_BWD_KERNEL, "bwd_kernel"),
+                     (_lib.KT_AUX, "aux")):
+        ms, n, name = _lib.kernel_time(cls)
+        if n:
+            out[tag] = (ms / n, name)
+    _lib.kernel_timing(False)
+    return out
+
+
+def _rel(a, b):
+    import torch
+    a, b = a.double(), b.double()
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def _maxabs(a, b):
+    return float((a.double() - b.double()).abs().max())
+
+
+def _dw_samples(x, dy, ksp, stride, pads, idx):
+    """FP64 weight-gradient entries idx [(a, c, q, o)] of a 2-D convolution
+    (rate 1), each a dot product over every output pixel, on the device."""
+    import torch.nn.functional as F
+    kh, kw = ksp
+    sh, sw = stride
+    Ho, Wo = dy.shape[1:3]
+    H, W = x.shape[1:3]
+    ph, pw = pads
+    bot = max(0, sh * (Ho - 1) + kh - H - ph)
+    rgt = max(0, sw * (Wo - 1) + kw - W - pw)
+    xp = F.pad(x.double(), (0, 0, pw, rgt, ph, bot))
+    g = dy.double()
+    import torch
+    return torch.stack([(xp[:, a:a + sh * (Ho - 1) + 1:sh, c:c + sw * (Wo - 1) + 1:sw, q] * g[..., o]).sum()
+                        for a, c, q, o in idx])
+
+
+def bench_config1(steps):
+    """[8,32,32,3] (*) [3,3,3,16] SAME, uniform[-1,1], FP32: latency of the
+    device call and of the Owl drop-in on host buffers."""
+    import torch
+    from owl_amd import ndarray as nd
+    from owl_amd.shapes import SAME
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    x = torch.rand(8, 32, 32, 3, device=dev, generator=g) * 2 - 1
+    w = torch.rand(3, 3, 3, 16, device=dev, generator=g) * 2 - 1
+    t = _ev_time(lambda: nd.conv2d(x, w, [1, 1], SAME), max(steps, 50))
+    xh, wh = x.cpu().numpy(), w.cpu().numpy()
+    nd.conv2d(xh, wh, [1, 1], SAME)
+    hs = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        nd.conv2d(xh, wh, [1, 1], SAME)
+        hs.append(time.perf_counter() - t0)
+    flop = 2.0 * 8 * 32 * 32 * 16 * 27
+    return {"fwd_device_us": round(t * 1e3, 2), "tflops": round(flop / (t * 1e-3) / 1e12, 3),
+            "dropin_host_us": round(statistics.median(hs) * 1e6, 1),
+            "what": "FP32 conv2d [8,32,32,3] (*) [3,3,3,16] SAME: device call (events, back to back) and the "
+                    "float32 drop-in on host buffers (copies in and out, median of 20 calls)"}
+
+
+def bench_config3(steps, check=True):
+    """ResNet-style stack on [128,28,28,256]: four 3x3 256->256 stride-1 SAME
+    convolutions and one 1x1 256->512 stride-2, fwd + bwd_input + bwd_kernel
+    of every layer (1440.3 GFLOP per step) in FP32, BF16 and FP16 (FP32
+    accumulate), and the FP8 e4m3 forward with per-tensor amax scales
+    (480.1 GFLOP).  Errors per layer and pass on the layer's own stored
+    inputs against FP64 (torch on the device): y and dx on 2 images, dw on
+    64 sampled entries over every pixel; the forward chain against FP64 from
+    the FP32 input."""
+    import ctypes
+    import torch
+    import torch.nn.functional as F
+    from owl_amd import _lib
+    from owl_amd import ndarray as nd
+    from owl_amd.shapes import SAME
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(33)
+    Bc, Hc, C3 = 128, 28, 256
+    x32 = torch.randn(Bc, Hc, Hc, C3, device=dev, generator=g)
+    ws32 = [torch.randn(3, 3, C3, C3, device=dev, generator=g) * (2.0 / (9 * C3)) ** 0.5 for _ in range(4)]
+    ws32.append(torch.randn(1, 1, C3, 512, device=dev, generator=g) * (2.0 / C3) ** 0.5)
+    strides = [[1, 1]] * 4 + [[2, 2]]
+    dy_last32 = torch.randn(Bc, 14, 14, 512, device=dev, generator=g)
+    flops = [2.0 * Bc * Hc * Hc * C3 * 9 * C3] * 4 + [2.0 * Bc * 14 * 14 * 512 * C3]
+    step_flop = 3 * sum(flops)
+
+    def stack(x, ws, dy_last, dt_w):
+        acts = [x]
+        for w, st in zip(ws, strides):
+            acts.append(nd.conv2d(acts[-1], w, st, SAME))
+        dy = dy_last
+        dws, dys = [None] * 5, [None] * 5
+        for l in range(4, -1, -1):
+            dys[l] = dy
+            dws[l] = nd.conv2d_backward_kernel(acts[l], ws[l], strides[l], dy, out_dtype=dt_w)
+            dy = nd.conv2d_backward_input(acts[l], ws[l], strides[l], dy)
+        return acts, dys, dws, dy  # dy: the input's gradient
+
+    out = {"workload": "[128,28,28,256]: 4 x conv 3x3 256->256 s1 SAME + conv 1x1 256->512 s2 SAME, "
+                       "fwd+bwd_input+bwd_kernel per layer", "gflop_per_step": round(step_flop / 1e9, 1)}
+    gi = torch.Generator()
+    gi.manual_seed(5)
+    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16), ("fp16", torch.float16)):
+        x, ws, dyl = x32.to(dt), [w.to(dt) for w in ws32], dy_last32.to(dt)
+        run = lambda: stack(x, ws, dyl, torch.float32)
+        n = max(3, steps // (4 if dt == torch.float32 else 1))
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            run()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / n * 1e3
+        peak = PEAK["f32"] if dt == torch.float32 else PEAK["bf16"]
+        rec = {"ms_per_step": round(ms, 3), "tflops": round(step_flop / (ms * 1e-3) / 1e12, 1),
+               "frac_of_peak": round(step_flop / (ms * 1e-3) / peak, 4)}
+        # the dominant kernel of a 3x3 layer's three passes
+        a0 = x
+        kt = _kernel_avgs(lambda: (nd.conv2d(a0, ws[0], [1, 1], SAME),
+                                   nd.conv2d_backward_input(a0, ws[0], [1, 1], a0),
+                                   nd.conv2d_backward_kernel(a0, ws[0], [1, 1], a0, out_dtype=torch.float32)), 5)
+        dom = max(("fwd", "bwd_input", "bwd_kernel"), key=lambda k: kt[k][0])
+        rec["roofline_3x3"] = {"kernel": f"{kt[dom][1]} ({dom})", "avg_launch_ms": round(kt[dom][0], 4),
+                               "achieved": round(flops[0] / (kt[dom][0] * 1e-3) / 1e12, 1),
+                               "frac": round(flops[0] / (kt[dom][0] * 1e-3) / peak, 4),
+                               "kernels_ms": {k: round(v[0], 4) for k, v in kt.items()}}
+        if check:
+            acts, dys, dws, dx0 = run()
+            torch.cuda.synchronize()
+            errs = []
+            ref = x32[:2].double()
+            for l in range(5):
+                st, w64 = strides[l], ws[l].double().permute(3, 2, 0, 1).contiguous()
+                pad = 1 if l < 4 else 0
+                a_in = acts[l][:2].double().permute(0, 3, 1, 2)
+                # SAME pads 1 for the 3x3 stride-1 layers, nothing for the 1x1 stride-2 one
+                ry = F.conv2d(a_in, w64, stride=st, padding=pad).permute(0, 2, 3, 1)
+                g64 = dys[l][:2].double().permute(0, 3, 1, 2)
+                rdx = torch.nn.grad.conv2d_input(a_in.shape, w64, g64, stride=st, padding=pad).permute(0, 2, 3, 1)
+                ks = list(ws[l].shape)
+                flat = torch.randperm(ks[0] * ks[1] * ks[2] * ks[3], generator=gi)[:64]
+                idx = torch.stack([flat // (ks[1] * ks[2] * ks[3]), flat // (ks[2] * ks[3]) % ks[1],
+                                   flat // ks[3] % ks[2], flat % ks[3]], 1).tolist()
+                rdw = _dw_samples(acts[l], dys[l], ks[:2], st, [pad, pad], idx)
+                gdw = torch.stack([dws[l][a, c, q, o] for a, c, q, o in idx])
+                ref = F.conv2d(ref.permute(0, 3, 1, 2), ws32[l].double().permute(3, 2, 0, 1).contiguous(),
+                               stride=st, padding=pad).permute(0, 2, 3, 1)
+                dx = (dys[l - 1] if l > 0 else dx0)[:2]
+                errs.append({"layer": l, "fwd_rel_l2": _rel(acts[l + 1][:2], ry), "fwd_max_abs": _maxabs(acts[l + 1][:2], ry),
+                             "bwd_input_rel_l2": _rel(dx, rdx), "bwd_input_max_abs": _maxabs(dx, rdx),
+                             "bwd_kernel_rel_l2_sampled": _rel(gdw, rdw), "bwd_kernel_max_abs_sampled": _maxabs(gdw, rdw),
+                             "fwd_chain_rel_l2_vs_fp32_input": _rel(acts[l + 1][:2], ref)})
+            rec["errors"] = errs
+        out[name] = rec
+        del x, ws, dyl
+    # FP8 e4m3 forward of the stack: each layer quantises its BF16 input and
+    # its weights (per-tensor amax scales), output BF16
+    st_ = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L = _lib.lib()
+
+    def quant(t):
+        q = torch.empty(t.numel(), dtype=torch.uint8, device=dev)
+        sc = torch.empty(1, dtype=torch.float32, device=dev)
+        _lib.check(L.owl_amd_fp8_quantize(_lib.BF16, ctypes.c_void_p(t.data_ptr()), t.numel(),
+                                          ctypes.c_void_p(q.data_ptr()), ctypes.c_void_p(sc.data_ptr()), st_))
+        return q, sc
+    wq = [quant(w.bfloat16().contiguous()) for w in ws32]
+    geos = []
+    shp = [Bc, Hc, Hc, C3]
+    for w, stv in zip(ws32, strides):
+        geo = nd.conv_problem(shp, list(w.shape), stv, None, SAME)
+        geos.append(geo)
+        shp = [Bc, *geo.sp_out, w.shape[3]]
+    ys8 = [torch.empty([Bc, *g_.sp_out, w.shape[3]], dtype=torch.bfloat16, device=dev) for g_, w in zip(geos, ws32)]
+    descs = [g_.desc() for g_ in geos]
+
+    def fp8_stack():
+        a = x32.bfloat16()
+        for l in range(5):
+            q, sc = quant(a)
+            _lib.check(L.owl_amd_conv_fwd_fp8(ctypes.byref(descs[l]), _lib.BF16, ctypes.c_void_p(q.data_ptr()),
+                                              ctypes.c_void_p(wq[l][0].data_ptr()), ctypes.c_void_p(sc.data_ptr()),
+                                              ctypes.c_void_p(wq[l][1].data_ptr()),
+                                              ctypes.c_void_p(ys8[l].data_ptr()), st_))
+            a = ys8[l]
+    fp8_stack()
+    torch.cuda.synchronize()
+    n = max(3, steps)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        fp8_stack()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / n * 1e3
+    fwd_flop = sum(flops)
+    rec = {"ms_per_fwd": round(ms, 3), "tflops": round(fwd_flop / (ms * 1e-3) / 1e12, 1),
+           "frac_of_fp8_peak": round(fwd_flop / (ms * 1e-3) / 5.0e15, 4),
+           "what": "forward of the stack, per-layer quantisation of the BF16 input included"}
+    if check:
+        errs = []
+        a_prev = x32.bfloat16()
+        for l in range(5):
+            pad = 1 if l < 4 else 0
+            ry = F.conv2d(a_prev[:2].double().permute(0, 3, 1, 2), ws32[l].double().permute(3, 2, 0, 1).contiguous(),
+                          stride=strides[l], padding=pad).permute(0, 2, 3, 1)
+            errs.append({"layer": l, "fwd_rel_l2": _rel(ys8[l][:2], ry), "fwd_max_abs": _maxabs(ys8[l][:2], ry)})
+            a_prev = ys8[l]
+        rec["errors_vs_fp64_on_bf16_input"] = errs
+    out["fp8_e4m3_fwd"] = rec
+    return out
+
+
+def bench_config4(steps):
+    """64 seeded ragged conv2d problems (BF16, FP32 accumulate): one
+    persistent grouped launch (autotuned), its hipGraph replay, and the same
+    64 problems as separate launches."""
+    import torch
+    from owl_amd import ndarray as nd
+    from owl_amd.group import ConvGroup, ragged_problems
+    dev = torch.device("cuda:0")
+    problems = ragged_problems(seed=4, n=64)
+    g = torch.Generator(device=dev)
+    g.manual_seed(44)
+    xs = [(torch.rand(p[0], device=dev, generator=g) * 2 - 1).bfloat16() for p in problems]
+    ws = [(torch.rand(p[1], device=dev, generator=g) * 2 - 1).bfloat16() for p in problems]
+    grp = ConvGroup(problems, torch.bfloat16, torch.bfloat16)
+    ys = grp.empty_outputs(dev)
+    grp.autotune(xs, ws, ys)
+    flop = 0.0
+    for geo in grp.geoms:
+        flop += 2.0 * geo.B * int(np.prod(geo.sp_out)) * geo.Co * int(np.prod(geo.ksp)) * geo.Ci
+    reps = max(steps, 10)
+    t_group = _ev_time(lambda: grp.run(xs, ws, ys), reps)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    grp.capture(xs, ws, ys, s)
+    torch.cuda.synchronize()
+    t_graph = _ev_time(lambda: grp.replay(torch.cuda.current_stream()), reps)
+    sep = lambda: [nd.dilated_conv2d(x, w, p[2], p[3], p[4]) for x, w, p in zip(xs, ws, problems)]
+    t_sep = _ev_time(sep, max(3, reps // 2))
+    t_sep_wall = None
+    sep()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        sep()
+    torch.cuda.synchronize()
+    t_sep_wall = (time.perf_counter() - t0) / 3 * 1e3
+    edges, tiles = grp.tile_edges()
+    return {"gflop": round(flop / 1e9, 2), "group_ms": round(t_group, 4), "graph_replay_ms": round(t_graph, 4),
+            "separate_launches_ms": round(t_sep, 4), "separate_launches_wall_ms": round(t_sep_wall, 3),
+            "group_tflops": round(flop / (t_group * 1e-3) / 1e12, 1),
+            "graph_tflops": round(flop / (t_graph * 1e-3) / 1e12, 1),
+            "separate_tflops": round(flop / (t_sep * 1e-3) / 1e12, 1),
+            "group_frac_of_bf16_peak": round(flop / (t_group * 1e-3) / PEAK["bf16"], 4),
+            "tiles": tiles, "tile_edges": sorted(set(edges)),
+            "what": "64 conv2d problems (seed 4): B 1-32, H=W 7-112, Ci/Co 16-512, k 1/3/5, s 1/2, rate 1/2, "
+                    "SAME/VALID; one persistent launch vs the same calls one by one"}
+
+
+def bench_config5(steps):
+    """conv3d [8,32,32,32,16] (*) [3,3,3,16,32] (*) [3,3,3,32,64] SAME, then
+    avg_pool3d 2x2x2: fwd + bwd_input + bwd_kernel of both convolutions
+    (108.7 GFLOP) in FP32 and BF16, and the average pool's GB/s."""
+    import torch
+    from owl_amd import ndarray as nd
+    from owl_amd.shapes import SAME, VALID
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(55)
+    x32 = torch.randn(8, 32, 32, 32, 16, device=dev, generator=g)
+    w1 = torch.randn(3, 3, 3, 16, 32, device=dev, generator=g) * (2.0 / 432) ** 0.5
+    w2 = torch.randn(3, 3, 3, 32, 64, device=dev, generator=g) * (2.0 / 864) ** 0.5
+    flop = 3 * 2.0 * 8 * 32 ** 3 * 27 * (16 * 32 + 32 * 64)
+    s1 = [1, 1, 1]
+    out = {"gflop_per_step": round(flop / 1e9, 2)}
+    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
+        x, a, b = x32.to(dt), w1.to(dt), w2.to(dt)
+        y1 = nd.conv3d(x, a, s1, SAME)
+        y2 = nd.conv3d(y1, b, s1, SAME)
+        dy2 = torch.randn(y2.shape, device=dev, generator=g).to(dt)
+
+        def step():
+            h1 = nd.conv3d(x, a, s1, SAME)
+            h2 = nd.conv3d(h1, b, s1, SAME)
+            nd.conv3d_backward_kernel(h1, b, s1, dy2, out_dtype=torch.float32)
+            d1 = nd.conv3d_backward_input(h1, b, s1, dy2)
+            nd.conv3d_backward_kernel(x, a, s1, d1, out_dtype=torch.float32)
+            nd.conv3d_backward_input(x, a, s1, d1)
+            return h2
+        ms = _ev_time(step, max(3, steps // 2))
+        peak = PEAK["f32"] if dt == torch.float32 else PEAK["bf16"]
+        kt = _kernel_avgs(lambda: (nd.conv3d(y1, b, s1, SAME), nd.conv3d_backward_input(y1, b, s1, dy2),
+                                   nd.conv3d_backward_kernel(y1, b, s1, dy2, out_dtype=torch.float32)), 3)
+        f2 = 2.0 * 8 * 32 ** 3 * 27 * 32 * 64
+        dom = max(("fwd", "bwd_input", "bwd_kernel"), key=lambda k: kt[k][0])
+        out[name] = {"ms_per_step": round(ms, 3), "tflops": round(flop / (ms * 1e-3) / 1e12, 1),
+                     "frac_of_peak": round(flop / (ms * 1e-3) / peak, 4),
+                     "roofline_layer2": {"kernel": f"{kt[dom][1]} ({dom})", "avg_launch_ms": round(kt[dom][0], 4),
+                                         "frac": round(f2 / (kt[dom][0] * 1e-3) / peak, 4)}}
+    y2 = nd.conv3d(nd.conv3d(x32, w1, s1, SAME), w2, s1, SAME)
+    pool_f = _ev_time(lambda: nd.avg_pool3d(y2, [2, 2, 2], [2, 2, 2], VALID), 20)
+    p = nd.avg_pool3d(y2, [2, 2, 2], [2, 2, 2], VALID)
+    gp = torch.randn(p.shape, device=dev, generator=g)
+    pool_b = _ev_time(lambda: nd.avg_pool3d_backward(VALID, y2, [2, 2, 2], [2, 2, 2], gp), 20)
+    nin, nout = y2.numel() * 4, p.numel() * 4
+    out["avg_pool3d_fp32"] = {"fwd_GBps": round((nin + nout) / (pool_f * 1e-3) / 1e9, 1),
+                              "bwd_GBps": round((nout + nin) / (pool_b * 1e-3) / 1e9, 1),
+                              "fwd_frac_hbm": round((nin + nout) / (pool_f * 1e-3) / HBM_PEAK, 3),
+                              "bwd_frac_hbm": round((nout + nin) / (pool_b * 1e-3) / HBM_PEAK, 3)}
+    return out
 
 
 def _cpu_model():
