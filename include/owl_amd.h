@@ -203,6 +203,22 @@ int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int out_t,
                             const void *x, const void *dy, void *dw,
                             void *workspace, size_t ws_bytes, void *stream);
 
+/* The backward-kernel pass for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, kernels and
+ * results as owl_amd_conv_bwd_kernel without a workspace, but its split-K
+ * partials go to a file-scope device array of
+ * OWL_AMD_NOALLOC_SCRATCH_BYTES (32 MiB) that every GPU the library runs on
+ * holds until the process ends; the call itself allocates nothing (no
+ * hipMalloc / hipFree, no library buffer).  A problem whose partials
+ * (owl_amd_conv_bwd_kernel_workspace) exceed the scratch is rejected with
+ * OWL_AMD_ERR_ARG.  Real kinds only (F32, F64, BF16, F16).  Every call on one
+ * GPU shares that GPU's scratch: calls that overlap on one GPU (different
+ * streams or host threads) race. */
+#define OWL_AMD_NOALLOC_SCRATCH_BYTES (32 * 1024 * 1024)
+int owl_amd_conv_bwd_kernel_noalloc(const owl_amd_conv_desc *d, int in_t, int out_t,
+                                    const void *x, const void *dy, void *dw, void *stream);
+
 /* Ragged batches of forward convolutions in one persistent launch (no
  * counterpart in Owl, which loops over calls; this is the batched form of
  * owl_amd_conv_fwd).  create: validates the n descriptors and picks a tile

@@ -771,6 +771,63 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   return OWL_AMD_OK;
 }
 
+// ------------------------------------------------------------------ backward kernel without allocation
+// The split-K partials of owl_amd_conv_bwd_kernel_noalloc live in this
+// file-scope device array: OWL_AMD_NOALLOC_SCRATCH_BYTES per GPU, held by
+// every GPU the library runs on until the process ends.  In host code the
+// array's name is only a stand-in for it: its device address, one per GPU,
+// comes from hipGetSymbolAddress and is what the kernels receive.
+static __device__ __align__(256) unsigned char scratch[OWL_AMD_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+void *g_scratch_ptr[64];  // device address of scratch on each device (guarded by g_mu)
+
+// Device address of this device's scratch array, looked up once per device
+// with that device current.
+int scratch_ptr(void **out) {
+  int dev;
+  int rc = current_device(&dev);
+  if (rc) return rc;
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      owl_amd_set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      return OWL_AMD_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return OWL_AMD_OK;
+}
+}  // namespace
+
+extern "C" int owl_amd_conv_bwd_kernel_noalloc(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x,
+                                               const void *dy, void *dw, void *stream) {
+  if (is_complex(in_t) || is_complex(out_t)) {
+    owl_amd_set_error("owl_amd_conv_bwd_kernel_noalloc takes real kinds (complex kinds expand the kernel "
+                      "into a library buffer)");
+    return OWL_AMD_ERR_UNSUPPORTED;
+  }
+  int rc = check_conv(d);
+  if (rc) return rc;
+  const size_t need = owl_amd_conv_bwd_kernel_workspace(d, in_t);
+  if (need > (size_t)OWL_AMD_NOALLOC_SCRATCH_BYTES)
+    ARG_FAIL("split-K partials need %zu bytes, above the %d-byte scratch of owl_amd_conv_bwd_kernel_noalloc; "
+             "call owl_amd_conv_bwd_kernel with a workspace", need, (int)OWL_AMD_NOALLOC_SCRATCH_BYTES);
+  // every partial the kernels read is written first by the same call, so
+  // nothing in the scratch needs clearing between calls
+  void *ws = nullptr;
+  if (need > 0) {
+    rc = scratch_ptr(&ws);
+    if (rc) return rc;
+  }
+  return owl_amd_conv_bwd_kernel(d, in_t, out_t, x, dy, dw, ws, need > 0 ? (size_t)OWL_AMD_NOALLOC_SCRATCH_BYTES : 0,
+                                 stream);
+}
+
 // ------------------------------------------------------------------ fused epilogue adjoint
 extern "C" int owl_amd_maxpool_bias_act_bwd(int dt, int act, int64_t B, int64_t H, int64_t W, int64_t C,
                                            const void *y, const void *pre, const void *dp, void *dz, void *db,
