@@ -161,3 +161,27 @@ def test_pool_nan_windows_and_flat_index_zero():
     x2 = np.arange(16, dtype=np.float32).reshape(1, 4, 4, 1)
     dx2 = nd.max_pool2d_backward(VALID, x2, [2, 2], [2, 2], dy)
     assert _same_bits(dx2, on.max_pool2d_backward(VALID, x2, [2, 2], [2, 2], dy))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nd_", [2, 3])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_pool_backward_tiled_windows_bit_exact(nd_, dt):
+    """Windows that tile the input exactly (kernel == stride, VALID, sizes a
+    multiple of the stride) take the one-thread-per-window backward kernels;
+    the results keep the oracle's bits, including -0 and NaN gradients (Owl
+    adds them to 0: -0 becomes +0), with 4-wide and odd channel counts."""
+    rng = np.random.default_rng(11 + nd_)
+    for k, C in (([2] * nd_, 16), ([3, 2, 2][:nd_], 5), ([1, 3, 2][:nd_], 8)):
+        sp = [kk * int(rng.integers(2, 5)) for kk in k]
+        x = rng.standard_normal([2, *sp, C]).astype(dt)
+        x[0, (0,) * nd_] = np.nan
+        for name in ("max_pool%dd" % nd_, "avg_pool%dd" % nd_):
+            yo = getattr(on, name)(x, k, k, VALID)
+            dy = rng.standard_normal(yo.shape).astype(dt)
+            dy.ravel()[::7] = -0.0
+            dy.ravel()[3] = np.nan
+            td = torch.float32 if dt == np.float32 else torch.float64
+            dxg = getattr(nd, name + "_backward")(VALID, t(x, td), k, k, t(dy, td))
+            dxo = getattr(on, name + "_backward")(VALID, x, k, k, dy)
+            assert _same_bits(dxg.cpu().numpy(), dxo), (name, k, C)
