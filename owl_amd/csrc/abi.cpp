@@ -7,8 +7,10 @@
 #include <string.h>
 
 #include <atomic>
+#include <condition_variable>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -186,7 +188,6 @@ struct DeviceState {
   DevBuf cplx;                   // complex convolutions: expanded kernel / real weight gradient
   DevBuf f8w;                    // FP8 halo forward: kernel repacked to [tap][co][ci]
   DevBuf ytmp;                   // fused conv + pool without y, when the pooling runs apart
-  DevBuf slot[4];                // drop-in staging buffers
 };
 
 // Recursive: the drop-ins hold it while calling the device API, which takes it
@@ -452,40 +453,6 @@ extern "C" int owl_amd_init(int device) {
 }
 
 extern "C" int owl_amd_complex_supported(void) { return 1; }
-
-extern "C" int owl_amd_cache_stats(int64_t *n, int64_t *bytes) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  int64_t cnt = 0, tot = 0;
-  for (auto &s : g_dev) {
-    if (!s.ready) continue;
-    for (auto &b : s.slot)
-      if (b.ptr) { cnt++; tot += (int64_t)b.bytes; }
-    if (s.ws.ptr) { cnt++; tot += (int64_t)s.ws.bytes; }
-  }
-  if (n) *n = cnt;
-  if (bytes) *bytes = tot;
-  return OWL_AMD_OK;
-}
-
-extern "C" void owl_amd_cache_release(void) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  int cur = -1;
-  (void)hipGetDevice(&cur);
-  for (int dev = 0; dev < 64; ++dev) {
-    DeviceState &s = g_dev[dev];
-    if (!s.ready) continue;
-    if (hipSetDevice(dev) != hipSuccess) continue;
-    (void)hipStreamSynchronize(s.stream);
-    (void)hipDeviceSynchronize();
-    for (DevBuf *b : {&s.slot[0], &s.slot[1], &s.slot[2], &s.slot[3], &s.ws, &s.red, &s.cplx, &s.f8w, &s.ytmp}) {
-      if (b->ptr) (void)hipFree(b->ptr);
-      *b = DevBuf();
-    }
-  }
-  for (void *p : g_retired) (void)hipFree(p);
-  g_retired.clear();
-  if (cur >= 0) (void)hipSetDevice(cur);
-}
 
 // ------------------------------------------------------------------ complex kinds
 // A complex convolution is the real one over the interleaved channels
@@ -1128,12 +1095,14 @@ static bool axis_has_empty(int64_t n, int64_t k, int64_t s, int64_t p, int64_t o
   return (k - p <= 0) || ((o - 1) * s - p >= n);
 }
 
-extern "C" int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max, const void *x,
-                                const void *dy, void *dx, void *stream) {
+// The pooling backward with its orphan flag (an int that is zero between
+// calls) given, or the device's own when flag is null.
+static int pool_bwd_flag(const owl_amd_pool_desc *d, int t, int is_max, const void *x, const void *dy, void *dx,
+                         int *flag, void *stream) {
   if (d && is_complex(t)) {
     owl_amd_pool_desc r;
     int rc = complex_pool(d, t, is_max, &r);
-    return rc ? rc : owl_amd_pool_bwd(&r, real_of(t), 0, x, dy, dx, stream);
+    return rc ? rc : pool_bwd_flag(&r, real_of(t), 0, x, dy, dx, flag, stream);
   }
   int rc = check_pool(d, t);
   if (rc) return rc;
@@ -1141,8 +1110,7 @@ extern "C" int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max, c
   hipStream_t st = (hipStream_t)stream;
   if ((int64_t)p.B * p.H * p.W * p.D * p.C == 0) return OWL_AMD_OK;
   if (!x || !dy || !dx) ARG_FAIL("null pointer");
-  int *flag = nullptr;
-  {
+  if (!flag) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     DeviceState *s;
     rc = ensure_state(&s);
@@ -1162,6 +1130,11 @@ extern "C" int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max, c
   return OWL_AMD_OK;
 }
 
+extern "C" int owl_amd_pool_bwd(const owl_amd_pool_desc *d, int t, int is_max, const void *x,
+                                const void *dy, void *dx, void *stream) {
+  return pool_bwd_flag(d, t, is_max, x, dy, dx, nullptr, stream);
+}
+
 // ------------------------------------------------------------------ Owl drop-ins
 namespace {
 
@@ -1178,37 +1151,329 @@ struct HostArg {
   size_t bytes;
 };
 
-// Runs one drop-in call: uploads inputs into the device's staging slots,
-// enqueues `launch`, downloads `out_slot` into `out_host` and waits.
-template <typename F>
-int run_dropin(std::initializer_list<HostArg> args, int out_slot, void *out_host, size_t out_bytes,
-               F launch, int extra_out_slot = -1, void *extra_host = nullptr, size_t extra_bytes = 0) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  DeviceState *s;
-  int rc = ensure_state(&s);
-  if (rc) return rc;
-  void *dev[4] = {nullptr, nullptr, nullptr, nullptr};
-  int i = 0;
-  for (const HostArg &a : args) {
-    rc = ensure_buf(s->slot[i], a.bytes, s->stream, false);
-    if (rc) return rc;
-    dev[i] = s->slot[i].ptr;
-    if (a.src && a.bytes)
-      OWL_CHECK_HIP(hipMemcpyAsync(dev[i], a.src, a.bytes, hipMemcpyHostToDevice, s->stream));
-    ++i;
+// ---- drop-in staging.  A drop-in call copies Owl's (pageable) Bigarray
+// data through pinned host chunks: the host copies one chunk into pinned
+// memory with several threads while the DMA engine moves the previous one,
+// and the other way round for results.  Each call takes a staging context
+// of its own (stream, device buffers, split-K workspace, pooling flag,
+// pinned chunks, copy threads), so calls from different host threads on one
+// device run concurrently; the library lock is held only to take and return
+// a context.  Complex kinds keep the lock for the whole call (their
+// expanded kernels live in one per-device buffer).
+constexpr size_t kChunk = 8u << 20;  // pinned chunk (two per context)
+constexpr int kMaxCtx = 8;           // staging contexts per device
+
+// Parallel memcpy: the caller and up to nw worker threads each copy one
+// stripe of at least 1 MiB.
+class CopyPool {
+ public:
+  explicit CopyPool(int nw) {
+    for (int i = 0; i < nw; ++i) th_.emplace_back([this, i] { work(i); });
   }
-  rc = launch(dev, s->stream);
-  if (rc) {
-    (void)hipStreamSynchronize(s->stream);
-    return rc;
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
   }
-  if (out_bytes)
-    OWL_CHECK_HIP(hipMemcpyAsync(out_host, dev[out_slot], out_bytes, hipMemcpyDeviceToHost, s->stream));
-  if (extra_out_slot >= 0 && extra_bytes)
-    OWL_CHECK_HIP(hipMemcpyAsync(extra_host, dev[extra_out_slot], extra_bytes, hipMemcpyDeviceToHost, s->stream));
-  OWL_CHECK_HIP(hipStreamSynchronize(s->stream));
+  void copy(void *dst, const void *src, size_t n) {
+    const size_t stripe_min = 1u << 20;
+    int parts = (int)(n / stripe_min);
+    if (parts > (int)th_.size() + 1) parts = (int)th_.size() + 1;
+    if (parts <= 1) {
+      memcpy(dst, src, n);
+      return;
+    }
+    const size_t per = (n / parts + 63) & ~(size_t)63;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      dst_ = (char *)dst;
+      src_ = (const char *)src;
+      n_ = n;
+      per_ = per;
+      parts_ = parts;
+      pending_ = parts - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    memcpy(dst, src, per < n ? per : n);  // stripe 0
+    std::unique_lock<std::mutex> lk(m_);
+    done_.wait(lk, [this] { return pending_ == 0; });
+  }
+
+ private:
+  void work(int i) {
+    uint64_t seen = 0;
+    for (;;) {
+      char *d;
+      const char *s;
+      size_t n, per;
+      int parts;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        d = dst_; s = src_; n = n_; per = per_; parts = parts_;
+      }
+      const int part = i + 1;
+      if (part < parts) {
+        const size_t off = (size_t)part * per;
+        if (off < n) memcpy(d + off, s + off, (off + per < n && part + 1 < parts) ? per : n - off);
+        std::lock_guard<std::mutex> lk(m_);
+        if (--pending_ == 0) done_.notify_all();
+      }
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  bool stop_ = false;
+  uint64_t gen_ = 0;
+  char *dst_ = nullptr;
+  const char *src_ = nullptr;
+  size_t n_ = 0, per_ = 0;
+  int parts_ = 0, pending_ = 0;
+};
+
+struct DropinCtx {
+  int dev = -1;
+  bool busy = false;
+  hipStream_t stream = nullptr;
+  int *flag = nullptr;
+  DevBuf slot[4];
+  DevBuf ws;
+  void *pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  CopyPool *copier = nullptr;
+};
+std::vector<DropinCtx *> g_ctx;  // guarded by g_mu
+std::condition_variable_any g_ctx_free;
+
+int copy_threads() {
+  static const int n = [] {
+    const int env = env_int("OWL_AMD_COPY_THREADS", 0);
+    if (env > 0) return env > 32 ? 32 : env;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int t = hw >= 16 ? 4 : (hw >= 8 ? 3 : 1);
+    return t;
+  }();
+  return n;
+}
+
+int make_ctx(int dev, DropinCtx **out) {
+  DropinCtx *c = new DropinCtx();
+  c->dev = dev;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->flag, 256);
+  if (e == hipSuccess) e = hipMemset(c->flag, 0, 256);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipHostMalloc(&c->pin[i], kChunk, hipHostMallocDefault);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    owl_amd_set_error("drop-in staging set-up: %s", hipGetErrorString(e));
+    for (int i = 0; i < 2; ++i) {
+      if (c->pin[i]) (void)hipHostFree(c->pin[i]);
+      if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    }
+    if (c->flag) (void)hipFree(c->flag);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return OWL_AMD_ERR_HIP;
+  }
+  const int nt = copy_threads();
+  c->copier = new CopyPool(nt > 1 ? nt - 1 : 0);
+  *out = c;
   return OWL_AMD_OK;
 }
+
+// Take a free staging context of the current device (creating one, up to
+// kMaxCtx; otherwise wait for one to be returned).
+int acquire_ctx(DropinCtx **out) {
+  int dev;
+  int rc = current_device(&dev);
+  if (rc) return rc;
+  std::unique_lock<std::recursive_mutex> lk(g_mu);
+  for (;;) {
+    int count = 0;
+    for (DropinCtx *c : g_ctx) {
+      if (c->dev != dev) continue;
+      ++count;
+      if (!c->busy) {
+        c->busy = true;
+        *out = c;
+        return OWL_AMD_OK;
+      }
+    }
+    if (count < kMaxCtx) {
+      DropinCtx *c;
+      rc = make_ctx(dev, &c);
+      if (rc) return rc;
+      c->busy = true;
+      g_ctx.push_back(c);
+      *out = c;
+      return OWL_AMD_OK;
+    }
+    g_ctx_free.wait(lk);
+  }
+}
+
+void release_ctx(DropinCtx *c) {
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    c->busy = false;
+  }
+  g_ctx_free.notify_one();
+}
+
+// host -> device through the pinned chunks (the host copy of chunk k + 1
+// overlaps the DMA of chunk k)
+int upload(DropinCtx &c, void *dst, const void *src, size_t n) {
+  for (size_t off = 0, k = 0; off < n; off += kChunk, ++k) {
+    const size_t len = n - off < kChunk ? n - off : kChunk;
+    const int b = (int)(k & 1);
+    OWL_CHECK_HIP(hipEventSynchronize(c.ev[b]));  // the DMA that last read this chunk is done
+    c.copier->copy(c.pin[b], (const char *)src + off, len);
+    OWL_CHECK_HIP(hipMemcpyAsync((char *)dst + off, c.pin[b], len, hipMemcpyHostToDevice, c.stream));
+    OWL_CHECK_HIP(hipEventRecord(c.ev[b], c.stream));
+  }
+  return OWL_AMD_OK;
+}
+
+// device -> host, after the work already on the stream (the DMA of chunk
+// k + 1 overlaps the host copy of chunk k)
+int download(DropinCtx &c, void *dst, const void *src, size_t n) {
+  size_t prev_off = 0, prev_len = 0;
+  int prev_b = -1;
+  for (size_t off = 0, k = 0; off < n; off += kChunk, ++k) {
+    const size_t len = n - off < kChunk ? n - off : kChunk;
+    const int b = (int)(k & 1);
+    OWL_CHECK_HIP(hipMemcpyAsync(c.pin[b], (const char *)src + off, len, hipMemcpyDeviceToHost, c.stream));
+    OWL_CHECK_HIP(hipEventRecord(c.ev[b], c.stream));
+    if (prev_b >= 0) {
+      OWL_CHECK_HIP(hipEventSynchronize(c.ev[prev_b]));
+      c.copier->copy((char *)dst + prev_off, c.pin[prev_b], prev_len);
+    }
+    prev_off = off;
+    prev_len = len;
+    prev_b = b;
+  }
+  if (prev_b >= 0) {
+    OWL_CHECK_HIP(hipEventSynchronize(c.ev[prev_b]));
+    c.copier->copy((char *)dst + prev_off, c.pin[prev_b], prev_len);
+  }
+  return OWL_AMD_OK;
+}
+
+// One synchronous drop-in call: upload the host inputs into the context's
+// device slots, run launch(slots, stream, context), download the outputs.
+template <typename F>
+int run_dropin_ctx(DropinCtx &c, std::initializer_list<HostArg> args, int out_slot, void *out_host,
+                   size_t out_bytes, F launch, int extra_out_slot, void *extra_host, size_t extra_bytes) {
+  void *dev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int i = 0, rc;
+  for (const HostArg &a : args) {
+    rc = ensure_buf(c.slot[i], a.bytes, c.stream, false);
+    if (rc) return rc;
+    dev[i] = c.slot[i].ptr;
+    if (a.src && a.bytes) {
+      rc = upload(c, dev[i], a.src, a.bytes);
+      if (rc) return rc;
+    }
+    ++i;
+  }
+  rc = launch(dev, c.stream, c);
+  if (rc) {
+    (void)hipStreamSynchronize(c.stream);
+    return rc;
+  }
+  if (out_bytes && (rc = download(c, out_host, dev[out_slot], out_bytes))) return rc;
+  if (extra_out_slot >= 0 && extra_bytes && (rc = download(c, extra_host, dev[extra_out_slot], extra_bytes)))
+    return rc;
+  OWL_CHECK_HIP(hipStreamSynchronize(c.stream));
+  return OWL_AMD_OK;
+}
+
+template <typename F>
+int run_dropin(std::initializer_list<HostArg> args, int out_slot, void *out_host, size_t out_bytes,
+               F launch, int extra_out_slot = -1, void *extra_host = nullptr, size_t extra_bytes = 0,
+               bool whole_call_locked = false) {
+  DropinCtx *c;
+  int rc = acquire_ctx(&c);
+  if (rc) return rc;
+  if (whole_call_locked) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    rc = run_dropin_ctx(*c, args, out_slot, out_host, out_bytes, launch, extra_out_slot, extra_host, extra_bytes);
+  } else {
+    rc = run_dropin_ctx(*c, args, out_slot, out_host, out_bytes, launch, extra_out_slot, extra_host, extra_bytes);
+  }
+  if (rc) (void)hipStreamSynchronize(c->stream);
+  release_ctx(c);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int owl_amd_cache_stats(int64_t *n, int64_t *bytes) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int64_t cnt = 0, tot = 0;
+  for (auto &s : g_dev) {
+    if (!s.ready) continue;
+    for (const DevBuf *b : {&s.ws, &s.red, &s.cplx, &s.f8w, &s.ytmp})
+      if (b->ptr) { cnt++; tot += (int64_t)b->bytes; }
+  }
+  for (DropinCtx *c : g_ctx)
+    for (const DevBuf *b : {&c->slot[0], &c->slot[1], &c->slot[2], &c->slot[3], &c->ws})
+      if (b->ptr) { cnt++; tot += (int64_t)b->bytes; }
+  if (n) *n = cnt;
+  if (bytes) *bytes = tot;
+  return OWL_AMD_OK;
+}
+
+extern "C" void owl_amd_cache_release(void) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  int cur = -1;
+  (void)hipGetDevice(&cur);
+  for (int dev = 0; dev < 64; ++dev) {
+    DeviceState &s = g_dev[dev];
+    if (!s.ready) continue;
+    if (hipSetDevice(dev) != hipSuccess) continue;
+    (void)hipStreamSynchronize(s.stream);
+    (void)hipDeviceSynchronize();
+    for (DevBuf *b : {&s.ws, &s.red, &s.cplx, &s.f8w, &s.ytmp}) {
+      if (b->ptr) (void)hipFree(b->ptr);
+      *b = DevBuf();
+    }
+  }
+  for (void *p : g_retired) (void)hipFree(p);
+  g_retired.clear();
+  // idle drop-in staging contexts
+  std::vector<DropinCtx *> keep;
+  for (DropinCtx *c : g_ctx) {
+    if (c->busy) {
+      keep.push_back(c);
+      continue;
+    }
+    if (hipSetDevice(c->dev) == hipSuccess) (void)hipStreamSynchronize(c->stream);
+    for (DevBuf *b : {&c->slot[0], &c->slot[1], &c->slot[2], &c->slot[3], &c->ws})
+      if (b->ptr) (void)hipFree(b->ptr);
+    for (int i = 0; i < 2; ++i) {
+      (void)hipHostFree(c->pin[i]);
+      (void)hipEventDestroy(c->ev[i]);
+    }
+    (void)hipFree(c->flag);
+    (void)hipStreamDestroy(c->stream);
+    delete c->copier;
+    delete c;
+  }
+  g_ctx.swap(keep);
+  if (cur >= 0) (void)hipSetDevice(cur);
+}
+
+namespace {
 
 template <typename T> constexpr int dt();
 template <> constexpr int dt<float>() { return OWL_AMD_F32; }
@@ -1245,23 +1510,29 @@ int conv_dropin(ConvPass pass, T *input, T *kernel, T *output, int64_t B, int64_
   const size_t nw = (size_t)(kh * kw * kd * Ci * Co) * sizeof(T);
   const size_t ny = (size_t)(B * Ho * Wo * Do * Co) * sizeof(T);
   const int t = dt<T>();
+  const bool cplx = is_complex(t);  // expanded kernels in one per-device buffer: whole call locked
   if (pass == FWD) {
     // Owl memsets the output first (owl_ndarray_conv_impl.h:285): an empty
     // GEMM (K == 0 cannot occur, M == 0 writes nothing) leaves zeros.
     return run_dropin({{input, nx}, {kernel, nw}, {nullptr, ny}}, 2, output, ny,
-                      [&](void **dv, hipStream_t st) {
+                      [&](void **dv, hipStream_t st, DropinCtx &) {
                         return owl_amd_conv_fwd(&d, t, t, dv[0], dv[1], dv[2], st);
-                      });
+                      }, -1, nullptr, 0, cplx);
   } else if (pass == BWD_IN) {
     return run_dropin({{kernel, nw}, {output, ny}, {nullptr, nx}}, 2, input, nx,
-                      [&](void **dv, hipStream_t st) {
+                      [&](void **dv, hipStream_t st, DropinCtx &) {
                         return owl_amd_conv_bwd_input(&d, t, t, dv[0], dv[1], dv[2], st);
-                      });
+                      }, -1, nullptr, 0, cplx);
   } else {
     return run_dropin({{input, nx}, {output, ny}, {nullptr, nw}}, 2, kernel, nw,
-                      [&](void **dv, hipStream_t st) {
-                        return owl_amd_conv_bwd_kernel(&d, t, t, dv[0], dv[1], dv[2], nullptr, 0, st);
-                      });
+                      [&](void **dv, hipStream_t st, DropinCtx &c) {
+                        // the split-K partials in the context's own workspace
+                        const size_t need = owl_amd_conv_bwd_kernel_workspace(&d, t);
+                        int r = ensure_buf(c.ws, need, st, false);
+                        if (r) return r;
+                        return owl_amd_conv_bwd_kernel(&d, t, t, dv[0], dv[1], dv[2], need ? c.ws.ptr : nullptr,
+                                                       need, st);
+                      }, -1, nullptr, 0, cplx);
   }
 }
 
@@ -1288,7 +1559,7 @@ int pool_dropin_fwd(bool is_max, T *input, T *output, int64_t B, int64_t H, int6
   if (!input || !output) ARG_FAIL("null pointer");
   const size_t nx = (size_t)(B * H * W * D * C) * sizeof(T);
   const size_t ny = (size_t)(B * Ho * Wo * Do * C) * sizeof(T);
-  return run_dropin({{input, nx}, {nullptr, ny}}, 1, output, ny, [&](void **dv, hipStream_t st) {
+  return run_dropin({{input, nx}, {nullptr, ny}}, 1, output, ny, [&](void **dv, hipStream_t st, DropinCtx &c) {
     return owl_amd_pool_fwd(&d, dt<T>(), is_max, dv[0], dv[1], st);
   });
 }
@@ -1316,8 +1587,8 @@ int pool_dropin_bwd(bool is_max, T *input, T *output_back, T *input_back, int64_
   // Owl's average backward is called with the output buffer as a dummy input
   // (owl_ndarray_pool.ml:85-119); it is never read for the average.
   return run_dropin({{is_max ? (const void *)input : nullptr, is_max ? nx : 16}, {output_back, ny}, {nullptr, nx}}, 2,
-                    input_back, nx, [&](void **dv, hipStream_t st) {
-                      return owl_amd_pool_bwd(&d, dt<T>(), is_max, dv[0], dv[1], dv[2], st);
+                    input_back, nx, [&](void **dv, hipStream_t st, DropinCtx &c) {
+                      return pool_bwd_flag(&d, dt<T>(), is_max, dv[0], dv[1], dv[2], c.flag, st);
                     });
 }
 
@@ -1342,7 +1613,7 @@ int pool_dropin_arg(T *input, T *output, int64_t *argmax, int64_t B, int64_t H, 
   const size_t ny = (size_t)(B * Ho * Wo * C) * sizeof(T);
   const size_t na = (size_t)(B * Ho * Wo * C) * sizeof(int64_t);
   return run_dropin({{input, nx}, {nullptr, ny}, {nullptr, na}}, 1, output, ny,
-                    [&](void **dv, hipStream_t st) {
+                    [&](void **dv, hipStream_t st, DropinCtx &c) {
                       return owl_amd_pool_argmax(&d, dt<T>(), dv[0], dv[1], (int64_t *)dv[2], st);
                     },
                     2, argmax, na);
@@ -1570,10 +1841,10 @@ int upsample_dropin(bool backward, T *input, T *output, int64_t B, int64_t H, in
   if ((backward ? nx : ny) == 0) return OWL_AMD_OK;
   if (!input || !output) ARG_FAIL("null pointer");
   if (backward)
-    return run_dropin({{output, ny}, {nullptr, nx}}, 1, input, nx, [&](void **dv, hipStream_t st) {
+    return run_dropin({{output, ny}, {nullptr, nx}}, 1, input, nx, [&](void **dv, hipStream_t st, DropinCtx &c) {
       return owl_amd_upsampling2d_bwd(dt<T>(), B, H, W, C, Ho, Wo, sh, sw, dv[0], dv[1], st);
     });
-  return run_dropin({{input, nx}, {nullptr, ny}}, 1, output, ny, [&](void **dv, hipStream_t st) {
+  return run_dropin({{input, nx}, {nullptr, ny}}, 1, output, ny, [&](void **dv, hipStream_t st, DropinCtx &c) {
     return owl_amd_upsampling2d_fwd(dt<T>(), B, H, W, C, Ho, Wo, sh, sw, dv[0], dv[1], st);
   });
 }
