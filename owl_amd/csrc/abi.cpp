@@ -694,7 +694,7 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   if (p.split <= 1) {
     p.c = dw;
     if (tr) {
-      KTimer kt(OWL_AMD_KT_BWD_KERNEL, "conv_wgrad_kernel", st);
+      KTimer kt(OWL_AMD_KT_BWD_KERNEL, wgrad_halo_fits(p) ? "conv_wgrad_halo_kernel" : "conv_wgrad_kernel", st);
       if (in_t == OWL_AMD_BF16) {
         if (out_t == OWL_AMD_BF16) run_wgrad<bf16_t, bf16_t>(p, st); else run_wgrad<bf16_t, float>(p, st);
       } else {
@@ -720,7 +720,7 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   }
   p.c = ws;
   if (tr) {
-    KTimer kt(OWL_AMD_KT_BWD_KERNEL, "conv_wgrad_kernel", st);
+    KTimer kt(OWL_AMD_KT_BWD_KERNEL, wgrad_halo_fits(p) ? "conv_wgrad_halo_kernel" : "conv_wgrad_kernel", st);
     if (in_t == OWL_AMD_BF16) run_wgrad<bf16_t, float>(p, st);
     else run_wgrad<f16_t, float>(p, st);
     OWL_CHECK_HIP(hipGetLastError());

@@ -721,6 +721,291 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
     }
 }
 
+// ---------------------------------------------------------------------------
+// Weight gradient with the input staged as a halo (16-bit storage, 2-D,
+// kw = 3, unit stride and rate along W, Ci % 64 == 0 and Ci >= 128,
+// Wo >= 32, B Ho Wo a multiple of 32).
+//
+// A tile is one kernel row a, its three taps c and a block of 64 input
+// channels (192 rows m = (a kw + c) Ci + 64 cb + q) by 128 output channels.
+// The 32 output pixels of a K-step lie in at most two output rows
+// (Wo >= 32); for each such run of n pixels the x pixels it reads through
+// the three taps, one row of n + 2, are staged in LDS once, and the tap c
+// fragment of pixel k is read at halo slot slot(k) + c, where slot(k) = k in
+// the first run and k + 2 in the second.  The im2col form (conv_wgrad_kernel)
+// stages and loads each x pixel three times, once per tap: here the A tile
+// is 36 pixels x 64 channels instead of 32 x 192.  Measured against it (one
+// box, alternating processes): 64x56x56x128 -> 128 layer 88.1 -> 73.7 us.
+// Not taken: 64 input channels (64x56x56x64 -> 128: 40.9 against 38.7 us),
+// and rows under 32 pixels (a form with three runs per step, for Wo >= 16,
+// ran a 128x28x28x256 -> 256 layer in 168 against 132 us).  The MFMA loop,
+// the two wave groups per workgroup and the FP32 split-K partials are those
+// of conv_wgrad_kernel (same split plan and tile count: kh Ci / 64 row tiles
+// of 192).  Every K-step is either inside the split or wholly past its end
+// (B Ho Wo and the split lengths are multiples of 32), and a step past the
+// end loads zeros for both operands, so the products are exactly those of
+// the pixels the split owns.
+// ---------------------------------------------------------------------------
+template <typename TI, typename TO, int KG>
+__global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_halo_kernel(ConvParams p) {
+  constexpr int FMW = 3, KW = 3, CB = 64;
+  constexpr int BM = 64 * FMW, BN = 128, BK = 32;
+  constexpr int HS = BK + 2 * (KW - 1);  // halo slots: two runs of up to BK pixels, each + KW - 1
+  constexpr int RSA = CB * 2 + 64;       // halo slot row: 64 channels + 64 bytes pad (192 B: conflict-free tr reads)
+  constexpr int RSB = BN * 2 + 64;
+  constexpr int SA = 64 * RSA, STAGE = SA + BK * RSB;  // 64 slots: HS read, the rest take dummy stores
+  constexpr int REG = FMW * 16 * 256;    // floats one group hands the other at the end (KG 2)
+  constexpr int LDS_BYTES = (KG * 2 * STAGE > 2 * REG * 4) ? KG * 2 * STAGE : 2 * REG * 4;
+  __shared__ __attribute__((aligned(16))) char lds_raw[LDS_BYTES];
+  // the group is uniform per wave: as a scalar, the run rows and bases
+  // derived from it below stay in scalar registers
+  const int tid = threadIdx.x & 255, grp = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), lane = tid & 63,
+            wave = tid >> 6;
+  char *lds_grp = lds_raw + grp * 2 * STAGE;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int M = (int)p.M, N = p.N;
+  const int n_tiles = (N + BN - 1) / BN;
+  const int cblocks = p.Ci / CB;
+  const int mn_tiles = p.kh * cblocks * n_tiles;
+  const int lid = blockIdx.x;
+  const int tile = lid % mn_tiles, split_idx = lid / mn_tiles;
+  const int mt = tile / n_tiles, n0 = (tile % n_tiles) * BN;
+  const int ta = mt / cblocks, cb = mt - (mt / cblocks) * cblocks;  // kernel row a, channel block
+  const int64_t sbeg = (int64_t)split_idx * p.k_per_split;
+  const int64_t send = sbeg + p.k_per_split < p.K ? sbeg + p.k_per_split : p.K;
+  const int64_t half = p.k_per_split / KG;
+  const int64_t kbeg = sbeg + grp * half;
+  int64_t kend = kbeg + half < send ? kbeg + half : send;
+  if (kend < kbeg) kend = kbeg;
+  const int64_t len0 = half < send - sbeg ? half : send - sbeg;
+  const int nk = (int)((len0 + BK - 1) / BK);
+
+  const uint32_t x_bytes = (uint32_t)((int64_t)p.B * p.H * p.W * p.Ci * 2);
+  const uint32_t g_bytes = (uint32_t)((int64_t)p.K * p.Co * 2);
+  const __amdgpu_buffer_rsrc_t xr = make_rsrc(p.a, x_bytes);
+  const __amdgpu_buffer_rsrc_t gr = make_rsrc(p.b, g_bytes);
+
+  // ---- loader assignment.  A: HS x 8 chunks of 16 bytes (288): thread t
+  // loads chunk t and, for t < 32, chunk 256 + t.  B: as conv_wgrad_kernel,
+  // pixel kp = t / 8 of the step, chunks c8 and c8 + 8 of its 128 columns.
+  const int c8 = tid & 7, kp = tid >> 3;
+  const int sl0 = tid >> 3, sl1 = (tid + 256) >> 3;  // halo slots of the two A chunks
+  const bool has1 = tid < HS * 8 - 256;
+  const int coff = cb * CB + 8 * c8;                  // channel of this thread's A chunks
+  const int bn = n0 + 8 * c8;
+  const bool bok0 = bn < N, bok1 = bn + 64 < N;
+  const int ah = ta * p.rh - p.ph;                    // x row offset of kernel row a
+
+  // loader position: first pixel of the step being loaded, as (image, row,
+  // column) digits advanced by BK without division (Wo >= BK: one carry)
+  int lb, li, lj;
+  {
+    int t = (int)kbeg;
+    lj = t % p.Wo;
+    t /= p.Wo;
+    li = t % p.Ho;
+    lb = t / p.Ho;
+  }
+  int lk = (int)kbeg;          // absolute pixel index of the loaded step
+  int gbase = (int)kbeg * p.Co + bn + kp * p.Co;  // dy element offset of the thread's B chunk 0
+  const int kend32 = (int)kend;
+
+  typedef uint4 Stage[4];  // A chunk 0, A chunk 1, B chunks 0 and 1
+  Stage rs0, rs1;
+  // The run rows, bases and bounds are uniform over the group (scalar
+  // work); per lane a chunk needs a select, an add and its column check.
+  // Threads without a second A chunk load zeros and store them past the
+  // last halo slot (slots HS .. 63 are never read), so no branch splits the
+  // K loop.
+  const int sc0 = sl0 * p.Ci, sc1 = sl1 * p.Ci;
+  auto a_off = [&](int slot, int sc, bool ok, int lim, int base1, int base2, bool r1, bool r2)
+      __attribute__((always_inline)) {
+    const bool s1 = slot < lim;
+    const int col = s1 ? lj - p.pw + slot : slot - lim - p.pw;
+    const bool in = ok & (s1 ? r1 : r2) & ((unsigned)col < (unsigned)p.W);
+    return in ? (uint32_t)((s1 ? base1 : base2) + sc + coff) * 2u : OOB;
+  };
+  auto load = [&](Stage &rg) __attribute__((always_inline)) {
+    const bool ok = lk < kend32;  // the whole step is inside the split or past its end
+    const int n1 = p.Wo - lj < BK ? p.Wo - lj : BK;
+    const int lim = n1 + KW - 1;
+    const bool wrap = li + 1 == p.Ho;
+    const int b1 = wrap ? lb + 1 : lb, i1 = wrap ? 0 : li + 1;
+    const int h1 = li * p.sh + ah, h2 = i1 * p.sh + ah;
+    const bool r1 = (lb < p.B) & ((unsigned)h1 < (unsigned)p.H);
+    const bool r2 = (b1 < p.B) & ((unsigned)h2 < (unsigned)p.H);
+    const int base1 = ((lb * p.H + h1) * p.W + lj - p.pw) * p.Ci;
+    const int base2 = ((b1 * p.H + h2) * p.W - p.pw - lim) * p.Ci;
+    rg[0] = bload(xr, a_off(sl0, sc0, ok, lim, base1, base2, r1, r2));
+    rg[1] = bload(xr, a_off(sl1, sc1, ok & has1, lim, base1, base2, r1, r2));
+    const uint32_t goff = (uint32_t)gbase * 2u;
+    rg[2] = bload(gr, (ok & bok0) ? goff : OOB);
+    rg[3] = bload(gr, (ok & bok1) ? goff + 128u : OOB);
+    // advance by BK pixels
+    lk += BK;
+    gbase += BK * p.Co;
+    lj += BK;
+    if (lj >= p.Wo) {
+      lj -= p.Wo;
+      if (++li == p.Ho) {
+        li = 0;
+        ++lb;
+      }
+    }
+  };
+  auto store = [&](const Stage &rg, int st) __attribute__((always_inline)) {
+    char *L = lds_grp + st * STAGE;
+    *(uint4 *)(&L[sl0 * RSA + c8 * 16]) = rg[0];
+    *(uint4 *)(&L[sl1 * RSA + c8 * 16]) = rg[1];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) *(uint4 *)(&L[SA + kp * RSB + (c8 + 8 * q) * 16]) = rg[2 + q];
+  };
+
+  f32x16 acc[FMW][2];
+#pragma unroll
+  for (int i = 0; i < FMW; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  // transposed-read sources: lane j of a 16-lane group reads 4 channels of
+  // pixel (j % 16) / 4 (+ 8 for the upper k-group, + 4 per half, + 16 per
+  // quarter) of the step
+  const int j16 = lane & 15;
+  const int tr_row = (j16 >> 2) + 8 * (lane >> 5), tr_col = 2 * (16 * ((lane >> 4) & 1) + 4 * (j16 & 3));
+  const int tr_b = SA + tr_row * RSB + tr_col;
+  // compute-side column of the step's first pixel (for the run split)
+  int cj = (int)(kbeg % p.Wo);
+
+  auto kstep = [&](int ks, Stage &rn) __attribute__((always_inline)) {
+    const char *Ls = lds_grp + (ks & 1) * STAGE;
+    const int n1 = p.Wo - cj < BK ? p.Wo - cj : BK;
+    constexpr int NQ = BK / 16;
+    v4s16 a[NQ][FMW][2], b[NQ][2][2];
+#pragma unroll
+    for (int s = 0; s < NQ; ++s)
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) {
+        const int k = tr_row + 16 * s + 4 * hf;
+        const int slot = k < n1 ? k : k + KW - 1;
+        const char *pa = Ls + slot * RSA + tr_col;
+#pragma unroll
+        for (int f = 0; f < FMW; ++f) {
+          const int F = wm * FMW + f;  // tap F / 2, channels 32 (F % 2) ..
+          a[s][f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s16 *)(pa + (F >> 1) * RSA + (F & 1) * 64));
+        }
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+          b[s][f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+              (lds_v4s16 *)(Ls + tr_b + (16 * s + 4 * hf) * RSB + 2 * (wn * 64 + f * 32)));
+      }
+#pragma unroll
+    for (int s = 0; s < NQ; ++s)
+#pragma unroll
+      for (int i = 0; i < FMW; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = mfma16<TI>(a[s][i][0], a[s][i][1], b[s][j][0], b[s][j][1], acc[i][j]);
+    store(rn, (ks & 1) ^ 1);
+    load(rn);
+    cj += BK;
+    if (cj >= p.Wo) cj -= p.Wo;
+    __builtin_amdgcn_sched_group_barrier(0x100, 2 * NQ * (FMW + 2), 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 2 * NQ * FMW, 0);
+    __syncthreads();
+  };
+
+  if (nk > 0) {
+    load(rs0);  // step 0
+    load(rs1);  // step 1
+    store(rs0, 0);
+    load(rs0);  // step 2
+    __syncthreads();
+    for (int ks = 0; ks < nk; ks += 2) {
+      kstep(ks, rs1);
+      if (ks + 1 < nk) kstep(ks + 1, rs0);
+    }
+  }
+
+  if constexpr (KG == 2) {
+    __syncthreads();
+    float *X = (float *)lds_raw;
+    float *mine = X + grp * REG;
+    const float *theirs = X + (1 - grp) * REG;
+    if (grp == 0) {
+#pragma unroll
+      for (int i = 0; i < FMW; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mine[(i * 16 + r) * 256 + tid] = acc[i][1][r];
+    } else {
+#pragma unroll
+      for (int i = 0; i < FMW; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mine[(i * 16 + r) * 256 + tid] = acc[i][0][r];
+    }
+    __syncthreads();
+    if (grp == 0) {
+#pragma unroll
+      for (int i = 0; i < FMW; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][0][r] = acc[i][0][r] + theirs[(i * 16 + r) * 256 + tid];
+    } else {
+#pragma unroll
+      for (int i = 0; i < FMW; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][1][r] = theirs[(i * 16 + r) * 256 + tid] + acc[i][1][r];
+    }
+  }
+
+  // ---- epilogue: fragment F = wm FMW + i holds tap c = F / 2, channels
+  // 32 (F % 2) .. of the block; row m = (a kw + c) Ci + 64 cb + q
+  const int frow = lane & 31, fkg = lane >> 5;
+  const bool partial = p.split > 1;
+  const uint32_t c_bytes = partial ? (uint32_t)((int64_t)M * N * 4) : (uint32_t)((int64_t)M * N * sizeof(TO));
+  char *cbase = (char *)p.c + (partial ? (int64_t)split_idx * M * N * 4 : 0);
+  const __amdgpu_buffer_rsrc_t cr = make_rsrc(cbase, c_bytes);
+#pragma unroll
+  for (int i = 0; i < FMW; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (KG == 2 && j != grp) continue;
+      const int n = n0 + wn * 64 + j * 32 + frow;
+      const int F = wm * FMW + i;
+      const int mrow = (ta * KW + (F >> 1)) * p.Ci + cb * CB + (F & 1) * 32 + 4 * fkg;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = mrow + 8 * (r >> 2) + (r & 3);
+        const bool ok = n < N;
+        const float v = acc[i][j][r];
+        if constexpr (sizeof(TO) == 2) {
+          if (!partial) {
+            const uint32_t vo = ok ? (uint32_t)(m * N + n) * 2u : OOB;
+            const TO h = (TO)v;
+            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), cr, (int)vo, 0, ST_NT);
+            continue;
+          }
+        }
+        const uint32_t vo = ok ? (uint32_t)(m * N + n) * 4u : OOB;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), cr, (int)vo, 0, ST_NT);
+      }
+    }
+}
+
+// The halo form of the weight gradient takes the problem (see
+// conv_wgrad_halo_kernel for the conditions); OWL_AMD_WGRAD_HALO=0 turns it
+// off (cross-checks against the im2col form).
+bool wgrad_halo_fits(const ConvParams &p) {
+  static const bool on = [] {
+    const char *e = getenv("OWL_AMD_WGRAD_HALO");
+    return !(e && atoi(e) == 0);
+  }();
+  return on && p.D == 1 && p.Do == 1 && p.kd == 1 && p.pd == 0 && p.kw == 3 && p.sw == 1 && p.rw == 1 &&
+         p.Ci % 64 == 0 && p.Ci >= 128 && p.Wo >= 32 && p.K % 32 == 0 && wgrad_tile_m(p.M) == 192 &&
+         p.Co % 8 == 0 &&
+         (p.split <= 1 || p.k_per_split % 64 == 0);
+}
+
 // Rows (tap, ci) per weight-gradient tile: 192 when that pads M no more than
 // 128 does (fewer LDS reads per MFMA), else 128.  OWL_AMD_WGRAD_BM = 128 or
 // 192 forces one (tuning).
@@ -747,6 +1032,11 @@ void run_wgrad(const ConvParams &p, hipStream_t st) {
   const int bm = wgrad_tile_m(p.M);
   const int kg = p.split > 1 ? wgrad_kgroups() : 1;
   dim3 grid((unsigned)(((p.M + bm - 1) / bm) * ((p.N + 127) / 128) * p.split)), blk(256 * kg);
+  if (wgrad_halo_fits(p)) {
+    if (kg == 2) launch_k((conv_wgrad_halo_kernel<TI, TO, 2>), grid, blk, st, p);
+    else launch_k((conv_wgrad_halo_kernel<TI, TO, 1>), grid, blk, st, p);
+    return;
+  }
   const bool d1 = p.D == 1 && p.Do == 1 && p.kd == 1 && p.pd == 0;
   if (kg == 2) {
     if (bm == 192) {

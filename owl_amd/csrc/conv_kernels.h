@@ -81,6 +81,8 @@ int wgrad_tile_m(int64_t M);
 int wgrad_kgroups();
 template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st);
+// True when run_wgrad takes the halo form (conv_wgrad_halo_kernel).
+bool wgrad_halo_fits(const ConvParams &p);
 template <typename TI> int tile_bm();
 template <typename TI> int tile_bk();
 
