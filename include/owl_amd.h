@@ -203,6 +203,16 @@ int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int out_t,
                             const void *x, const void *dy, void *dw,
                             void *workspace, size_t ws_bytes, void *stream);
 
+/* Both backward passes of a layer, dx and dw from x, w and dy: the pair
+ * Owl's reverse rule for a convolution computes (owl_algodiff_ops.ml:2364,
+ * conv2d_backward_input a b s ca, conv2d_backward_kernel a b s ca): the
+ * weight gradient, then the backward input, on the caller's stream, with
+ * the results of owl_amd_conv_bwd_kernel and owl_amd_conv_bwd_input (dx_t,
+ * dw_t: their out_t).  workspace as for owl_amd_conv_bwd_kernel. */
+int owl_amd_conv_bwd(const owl_amd_conv_desc *d, int in_t, int dx_t, int dw_t, const void *x,
+                     const void *w, const void *dy, void *dx, void *dw, void *workspace,
+                     size_t ws_bytes, void *stream);
+
 /* The backward-kernel pass for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  Same arguments, kernels and

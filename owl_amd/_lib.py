@@ -107,6 +107,7 @@ def lib():
             "owl_amd_conv_bwd_kernel_workspace": ([cd, i32], ctypes.c_size_t),
             "owl_amd_conv_bwd_kernel": ([cd, i32, i32, vp, vp, vp, vp, ctypes.c_size_t, vp], i32),
             "owl_amd_conv_bwd_kernel_noalloc": ([cd, i32, i32, vp, vp, vp, vp], i32),
+            "owl_amd_conv_bwd": ([cd, i32, i32, i32, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp], i32),
             "owl_amd_pool_fwd": ([pd, i32, i32, vp, vp, vp], i32),
             "owl_amd_pool_argmax": ([pd, i32, vp, vp, vp, vp], i32),
             "owl_amd_pool_bwd": ([pd, i32, i32, vp, vp, vp, vp], i32),

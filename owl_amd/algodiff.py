@@ -119,8 +119,9 @@ def conv2d(x, w, stride, padding=SAME):
     return build_piso(
         "conv2d",
         lambda a, b: nd.conv2d(a, b, stride, padding),
-        lambda a, b, cp, ca: (nd.conv2d_backward_input(a, b, stride, ca),
-                              nd.conv2d_backward_kernel(a, b, stride, ca, out_dtype=b.dtype)),
+        # both gradients (dr_ab) in one paired call: conv2d_backward_input
+        # and conv2d_backward_kernel, the reduction beside the first
+        lambda a, b, cp, ca: nd.conv2d_backward(a, b, stride, ca, kernel_out_dtype=b.dtype),
     )(x, w)
 
 
@@ -129,8 +130,7 @@ def conv3d(x, w, stride, padding=SAME):
     return build_piso(
         "conv3d",
         lambda a, b: nd.conv3d(a, b, stride, padding),
-        lambda a, b, cp, ca: (nd.conv3d_backward_input(a, b, stride, ca),
-                              nd.conv3d_backward_kernel(a, b, stride, ca, out_dtype=b.dtype)),
+        lambda a, b, cp, ca: nd.conv3d_backward(a, b, stride, ca, kernel_out_dtype=b.dtype),
     )(x, w)
 
 

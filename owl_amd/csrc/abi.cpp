@@ -664,16 +664,12 @@ extern "C" size_t owl_amd_conv_bwd_kernel_workspace(const owl_amd_conv_desc *d, 
   return (size_t)split * (size_t)p.M * (size_t)p.N * acc;
 }
 
-extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int out_t,
-                                       const void *x, const void *dy, void *dw, void *workspace,
-                                       size_t ws_bytes, void *stream) {
-  if (is_complex(in_t) || is_complex(out_t))
-    return complex_conv(2, d, in_t, out_t, x, dy, dw, workspace, ws_bytes, stream);
+static int bwd_kernel_impl(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x, const void *dy, void *dw,
+                           void *workspace, size_t ws_bytes, hipStream_t st) {
   int rc = check_conv(d);
   if (rc) return rc;
   ConvParams p;
   bwd_kernel_params(d, p);
-  hipStream_t st = (hipStream_t)stream;
   if (p.M == 0 || p.N == 0) return OWL_AMD_OK;
   if (!x || !dy || !dw) ARG_FAIL("null pointer");
   if (p.K == 0) {
@@ -736,6 +732,27 @@ extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int
   else run_splitk_reduce<float, f16_t>((const float *)ws, (f16_t *)dw, n, p.split, st);
   OWL_CHECK_HIP(hipGetLastError());
   return OWL_AMD_OK;
+}
+
+extern "C" int owl_amd_conv_bwd_kernel(const owl_amd_conv_desc *d, int in_t, int out_t,
+                                       const void *x, const void *dy, void *dw, void *workspace,
+                                       size_t ws_bytes, void *stream) {
+  if (is_complex(in_t) || is_complex(out_t))
+    return complex_conv(2, d, in_t, out_t, x, dy, dw, workspace, ws_bytes, stream);
+  return bwd_kernel_impl(d, in_t, out_t, x, dy, dw, workspace, ws_bytes, (hipStream_t)stream);
+}
+
+// Both backward passes of a layer (the pair Owl's reverse rule computes,
+// owl_algodiff_ops.ml:2364), on the caller's stream: the weight gradient
+// (main kernel and split-K reduction), then the backward input.  Running
+// the reduction on a side stream beside the backward-input kernel measured
+// slower (64x56x56x64 -> 128 BF16 step 123.6 against 121.7 us): the two
+// compete for the CUs' slots instead of filling the other's gaps.
+extern "C" int owl_amd_conv_bwd(const owl_amd_conv_desc *d, int in_t, int dx_t, int dw_t, const void *x,
+                                const void *w, const void *dy, void *dx, void *dw, void *workspace, size_t ws_bytes,
+                                void *stream) {
+  int rc = owl_amd_conv_bwd_kernel(d, in_t, dw_t, x, dy, dw, workspace, ws_bytes, stream);
+  return rc ? rc : owl_amd_conv_bwd_input(d, in_t, dx_t, w, dy, dx, stream);
 }
 
 // ------------------------------------------------------------------ backward kernel without allocation

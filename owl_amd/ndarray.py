@@ -315,6 +315,38 @@ def conv2d_backward_kernel(input, kernel, stride, output_grad, out_dtype=None, o
                      "spatial_backward_kernel", out_dtype, out=out)
 
 
+def _conv_bwd_both(name, nd, x, k, stride, rate, dy, dx_dtype=None, dw_dtype=None, workspace=None):
+    _verify(_is_torch(x) and _is_torch(k) and _is_torch(dy), f"{name}: the paired backward takes device tensors")
+    _same_kind(x, k, dy)
+    _check_conv_args(name, nd, x, k, stride, rate, dy)
+    x, k, dy = _contig(x), _contig(k), _contig(dy)
+    rate = list(rate) if rate is not None else [1] * nd
+    g = _bwd_geom(nd, list(x.shape), list(k.shape), stride, rate, list(dy.shape))
+    dx = _out_buf(None, x, list(x.shape), dx_dtype, name, (x, k, dy))
+    dw = _out_buf(None, k, list(k.shape), dw_dtype, name, (x, k, dy))
+    ws, nb = (None, 0) if workspace is None else (_ptr(workspace), workspace.numel())
+    d = g.desc()
+    _lib.check(_lib.lib().owl_amd_conv_bwd(ctypes.byref(d), _tdt(x), _TORCH_DT[dx.dtype], _TORCH_DT[dw.dtype],
+                                           _ptr(x), _ptr(k), _ptr(dy), _ptr(dx), _ptr(dw), ws, nb, _stream()))
+    return dx, dw
+
+
+def conv2d_backward(input, kernel, stride, output_grad, out_dtype=None, kernel_out_dtype=None, workspace=None):
+    """Both backward passes of conv2d: (conv2d_backward_input,
+    conv2d_backward_kernel), the pair Owl's reverse rule for conv2d computes
+    (owl_algodiff_ops.ml:2364), in one library call on the current stream.
+    Device tensors; out_dtype is dx's type, kernel_out_dtype dw's."""
+    return _conv_bwd_both("conv2d_backward", 2, input, kernel, stride, None, output_grad, out_dtype,
+                          kernel_out_dtype, workspace)
+
+
+def conv3d_backward(input, kernel, stride, output_grad, out_dtype=None, kernel_out_dtype=None, workspace=None):
+    """Both backward passes of conv3d (owl_algodiff_ops.ml's conv3d reverse
+    rule), as ``conv2d_backward``."""
+    return _conv_bwd_both("conv3d_backward", 3, input, kernel, stride, None, output_grad, out_dtype,
+                          kernel_out_dtype, workspace)
+
+
 def conv3d(input, kernel, stride, padding=SAME, out_dtype=None, out=None):
     """owl_dense_ndarray_generic.ml:2848 (dispatches to cuboid_im2col, owl_ndarray_conv.ml:826)."""
     return _conv_fwd("conv3d", 3, input, kernel, stride, None, padding, "cuboid_im2col", out_dtype, out=out)
