@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <condition_variable>
 #include <map>
@@ -1887,7 +1888,12 @@ DEFINE_UPSAMPLE_DROPINS(complex64, owl_amd_complex64)
 // the autotuner fills (or a size heuristic until it has run).
 namespace {
 
-typedef std::tuple<int, int, int, int, int, int> ClassKey;  // in_t, out_t, aligned, log2 M, log2 N, log2 K
+// in_t, out_t, aligned, log2 M, log2 N, log2 K, unit strides, taps
+typedef std::tuple<int, int, int, int, int, int, int, int> ClassKey;
+// cfg of a problem: its tile edge in the persistent launch (0 full, 1 half),
+// or its own launch of the single-problem kernels (owl_amd_conv_fwd picks
+// the halo, direct-A or generic kernel) after the persistent one
+constexpr int GROUP_OWN_LAUNCH = 2;
 std::map<ClassKey, int> g_tune_cache;
 
 int ilog2(int64_t v) {
@@ -1918,12 +1924,15 @@ static int group_edge(int in_t, int cfg) {
   return group_tile_edge<bf16_t>(cfg);
 }
 
+// Tile ranges of the persistent launch; problems with their own launch own
+// no tiles (the kernel's search over tile_begin skips them).
 static void group_layout(owl_amd_conv_group *g) {
   int64_t t = 0;
   for (int i = 0; i < g->n; ++i) {
     GroupEntry &e = g->tab[i];
-    const int edge = group_edge(g->in_t, e.cfg);
     e.tile_begin = t;
+    if (e.cfg == GROUP_OWN_LAUNCH) continue;
+    const int edge = group_edge(g->in_t, e.cfg);
     t += ((e.p.M + edge - 1) / edge) * (int64_t)((e.p.N + edge - 1) / edge);
   }
   g->total = t;
@@ -1938,6 +1947,24 @@ static int group_launch(owl_amd_conv_group *g, const GroupEntry *tab, int nprob,
   else run_group_fwd<float, float>(tab, nprob, total, grid, st);
   OWL_CHECK_HIP(hipGetLastError());
   return OWL_AMD_OK;
+}
+
+// The problems of indices idx (or all with their own launch when idx is
+// null), each as a single-problem forward on st.
+static int group_own_launches(owl_amd_conv_group *g, const std::vector<int> *idx, hipStream_t st) {
+  for (int i = 0; i < g->n; ++i) {
+    if (idx ? std::find(idx->begin(), idx->end(), i) == idx->end() : g->tab[i].cfg != GROUP_OWN_LAUNCH) continue;
+    const ConvParams &p = g->tab[i].p;
+    int rc = owl_amd_conv_fwd(&g->descs[i], g->in_t, g->out_t, p.a, p.b, p.c, st);
+    if (rc) return rc;
+  }
+  return OWL_AMD_OK;
+}
+
+// One run: the persistent launch, then the problems with their own launch.
+static int group_dispatch(owl_amd_conv_group *g, hipStream_t st) {
+  int rc = group_launch(g, g->dev_tab, g->n, g->total, st);
+  return rc ? rc : group_own_launches(g, nullptr, st);
 }
 
 static void group_set_ptrs(owl_amd_conv_group *g, const void *const *xs, const void *const *ws, void *const *ys) {
@@ -1981,7 +2008,8 @@ extern "C" int owl_amd_conv_group_create(owl_amd_conv_group **out, int n, const 
     e.p.ldc = e.p.Co;
     const int v = vec_of(in_t);
     const int al = e.p.Ci % v == 0 && e.p.Co % v == 0;
-    g->keys[i] = ClassKey(in_t, out_t, al, ilog2(e.p.M), ilog2(e.p.N), ilog2(e.p.K));
+    g->keys[i] = ClassKey(in_t, out_t, al, ilog2(e.p.M), ilog2(e.p.N), ilog2(e.p.K),
+                          e.p.sh == 1 && e.p.sw == 1 && e.p.sd == 1, e.p.kh * e.p.kw * e.p.kd);
     auto it = g_tune_cache.find(g->keys[i]);
     // heuristic until tuned: half-size tiles when the full tile would leave
     // most of the chip idle or mostly empty
@@ -2003,7 +2031,8 @@ extern "C" int owl_amd_conv_group_create(owl_amd_conv_group **out, int n, const 
 extern "C" int owl_amd_conv_group_info(const owl_amd_conv_group *g, int *tile_edges, int64_t *total_tiles) {
   if (!g) ARG_FAIL("null group");
   if (tile_edges)
-    for (int i = 0; i < g->n; ++i) tile_edges[i] = group_edge(g->in_t, g->tab[i].cfg);
+    for (int i = 0; i < g->n; ++i)
+      tile_edges[i] = g->tab[i].cfg == GROUP_OWN_LAUNCH ? 0 : group_edge(g->in_t, g->tab[i].cfg);
   if (total_tiles) *total_tiles = g->total;
   return OWL_AMD_OK;
 }
@@ -2018,14 +2047,15 @@ extern "C" int owl_amd_conv_group_run(owl_amd_conv_group *g, const void *const *
   // the table copy is ordered before the launch on the same stream; the host
   // vector is not touched again until the next run, which first waits here
   OWL_CHECK_HIP(hipMemcpyAsync(g->dev_tab, g->tab.data(), sizeof(GroupEntry) * g->n, hipMemcpyHostToDevice, st));
-  int rc = group_launch(g, g->dev_tab, g->n, g->total, st);
+  int rc = group_dispatch(g, st);
   if (rc) return rc;
   OWL_CHECK_HIP(hipStreamSynchronize(st));
   return OWL_AMD_OK;
 }
 
-// Time each shape class with both tile edges (separate grouped launches of
-// just that class's problems) and keep the faster in the process-wide cache.
+// Time each shape class three ways -- a grouped launch of just that class's
+// problems with either tile edge, and the class's problems as single-problem
+// launches -- and keep the fastest in the process-wide cache.
 extern "C" int owl_amd_conv_group_autotune(owl_amd_conv_group *g, const void *const *xs, const void *const *ws,
                                            void *const *ys, void *stream) {
   if (!g) ARG_FAIL("null group");
@@ -2042,7 +2072,23 @@ extern "C" int owl_amd_conv_group_autotune(owl_amd_conv_group *g, const void *co
   for (auto &kv : classes) {
     float best = 1e30f;
     int best_cfg = 0;
-    for (int cfg = 0; cfg < 2 && rc == OWL_AMD_OK; ++cfg) {
+    for (int cfg = 0; cfg < 3 && rc == OWL_AMD_OK; ++cfg) {
+      if (cfg == GROUP_OWN_LAUNCH) {
+        // the class's problems as single-problem launches
+        rc = group_own_launches(g, &kv.second, st);  // warm-up
+        if (rc) break;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < 3 && rc == OWL_AMD_OK; ++r) rc = group_own_launches(g, &kv.second, st);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rc == OWL_AMD_OK && ms < best) {
+          best = ms;
+          best_cfg = cfg;
+        }
+        continue;
+      }
       std::vector<GroupEntry> sub;
       int64_t t = 0;
       const int edge = group_edge(g->in_t, cfg);
@@ -2085,8 +2131,9 @@ extern "C" int owl_amd_conv_group_autotune(owl_amd_conv_group *g, const void *co
   return rc;
 }
 
-// Capture one grouped launch into a hipGraph (the problem table is uploaded
-// before capture, so the graph holds a single kernel node).
+// Capture one run into a hipGraph (the problem table is uploaded before
+// capture, so the graph holds the grouped kernel node followed by the
+// kernel nodes of the problems with their own launch).
 extern "C" int owl_amd_conv_group_capture(owl_amd_conv_group *g, const void *const *xs, const void *const *ws,
                                           void *const *ys, void *stream) {
   if (!g || !stream) ARG_FAIL("group capture needs a group and a non-default stream");
@@ -2102,7 +2149,7 @@ extern "C" int owl_amd_conv_group_capture(owl_amd_conv_group *g, const void *con
     g->graph = nullptr;
   }
   OWL_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  int rc = group_launch(g, g->dev_tab, g->n, g->total, st);
+  int rc = group_dispatch(g, st);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(st, &graph);
   if (rc) return rc;

@@ -2,8 +2,10 @@
 
 ``ConvGroup`` wraps the ``owl_amd_conv_group_*`` entry points: build a plan
 for many conv2d / conv3d problems of different shapes, optionally autotune
-the tile edge per shape class, then run all of them with one kernel launch,
-or capture that launch in a hipGraph and replay it.  Each problem's
+per shape class the tile edge or a launch of the single-problem kernels of
+its own (the halo kernel wins for large stride-1 layers), then run all of
+them with one persistent kernel launch plus those own launches, or capture
+that run in a hipGraph and replay it.  Each problem's
 geometry is derived exactly like ``owl_amd.ndarray.dilated_conv2d``.
 """
 import ctypes
@@ -65,6 +67,8 @@ class ConvGroup:
         _lib.check(_lib.lib().owl_amd_conv_group_replay(self._h, self._stream(stream)))
 
     def tile_edges(self):
+        """(per-problem tile edge in the persistent launch, 0 for a problem
+        with its own launch; tiles of the persistent launch)."""
         n = len(self.geoms)
         e = (ctypes.c_int * max(n, 1))()
         t = ctypes.c_int64()

@@ -83,12 +83,52 @@ def test_ragged_group_matches_separate_launches_and_oracle():
                 assert torch.equal(a, b), f"graph replay {rep} differs from the eager grouped launch"
             _ = torch.ones(1024, device=DEV).sum().item()
 
-    # autotuning changes tile edges, not results (same K order per element)
+    # autotuning picks a tile edge (same K order per element: the eager
+    # grouped result) or the problem's own launch (the separate launch's
+    # result), per shape class
     ys_t = group.empty_outputs()
     group.autotune(xs, ws, ys_t)
     edges, _ = group.tile_edges()
-    assert set(edges) <= {64, 128}
+    assert set(edges) <= {0, 64, 128}
     ys_t = group.run(xs, ws, ys_t)
     torch.cuda.synchronize()
-    for a, b in zip(ys_t, ys):
-        assert torch.equal(a, b)
+    for i, (a, e) in enumerate(zip(ys_t, edges)):
+        assert torch.equal(a, sep[i] if e == 0 else ys[i]), (i, e)
+
+    # and a capture of the tuned plan replays the same
+    ys_g2 = group.empty_outputs()
+    with torch.cuda.stream(stream):
+        group.capture(xs, ws, ys_g2, stream)
+        for rep in range(3):
+            for y in ys_g2:
+                y.zero_()
+            group.replay(stream)
+            stream.synchronize()
+            for a, b in zip(ys_g2, ys_t):
+                assert torch.equal(a, b), f"tuned graph replay {rep} differs from the tuned run"
+            _ = torch.ones(1024, device=DEV).sum().item()
+
+
+@pytest.mark.gpu
+def test_group_own_launch_classes():
+    """Large stride-1 3x3 layers beside small ones: whatever the tuner picks,
+    every problem equals its separate launch's result when it has its own
+    launch, and the eager grouped result otherwise."""
+    problems = [([4, 56, 56, 128], [3, 3, 128, 256], [1, 1], [1, 1], "SAME"),
+                ([2, 56, 56, 128], [3, 3, 128, 256], [1, 1], [1, 1], "SAME"),
+                ([3, 14, 14, 32], [1, 1, 32, 48], [2, 2], [1, 1], "VALID"),
+                ([1, 7, 7, 16], [3, 3, 16, 24], [1, 1], [2, 2], "SAME")]
+    xs, ws = make_inputs(problems, seed=45)
+    group = ConvGroup(problems, torch.bfloat16, torch.bfloat16)
+    ys = group.run(xs, ws)
+    sep = [nd.dilated_conv2d(x, w, p[2], p[3], p[4]) for x, w, p in zip(xs, ws, problems)]
+    ys_t = group.empty_outputs()
+    group.autotune(xs, ws, ys_t)
+    edges, total = group.tile_edges()
+    ys_t = group.run(xs, ws, ys_t)
+    torch.cuda.synchronize()
+    assert total >= 0
+    for i, (a, e) in enumerate(zip(ys_t, edges)):
+        assert torch.equal(a, sep[i] if e == 0 else ys[i]), (i, e)
+        ad, sd = a.double(), sep[i].double()
+        assert (torch.linalg.norm(ad - sd) / torch.linalg.norm(sd)).item() < 1e-2
