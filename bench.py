@@ -633,9 +633,9 @@ def bench_config3(steps, check=True):
 
 def bench_config4(steps):
     """64 seeded ragged conv2d problems (BF16, FP32 accumulate): one
-    persistent grouped launch (autotuned per shape class: tile edge, or a
-    launch of its own), its hipGraph replay, and the same 64 problems as
-    separate launches."""
+    persistent grouped launch (autotuned per shape class: tile edge and
+    split-K, or a launch of its own), its hipGraph replay, and the same 64
+    problems as separate launches."""
     import torch
     from owl_amd import ndarray as nd
     from owl_amd.group import ConvGroup, ragged_problems
@@ -677,9 +677,10 @@ def bench_config4(steps):
             "group_frac_of_bf16_peak": round(flop / (t_group * 1e-3) / PEAK["bf16"], 4),
             "tiles": tiles, "tile_edges": sorted(set(e for e in edges if e)),
             "own_launches": sum(1 for e in edges if e == 0),
+            "split_k_problems": sum(1 for sp in grp.plan()[1] if sp > 1),
             "what": "64 conv2d problems (seed 4): B 1-32, H=W 7-112, Ci/Co 16-512, k 1/3/5, s 1/2, rate 1/2, "
-                    "SAME/VALID; one persistent launch (plus the shape classes the tuner gives a launch of "
-                    "their own) vs the same calls one by one"}
+                    "SAME/VALID; one persistent launch (tile edge and split-K per shape class, plus the classes "
+                    "the tuner gives a launch of their own) vs the same calls one by one"}
 
 
 def bench_config5(steps):

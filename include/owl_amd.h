@@ -231,12 +231,21 @@ int owl_amd_conv_bwd_kernel_noalloc(const owl_amd_conv_desc *d, int in_t, int ou
 
 /* Ragged batches of forward convolutions in one persistent launch (no
  * counterpart in Owl, which loops over calls; this is the batched form of
- * owl_amd_conv_fwd).  create: validates the n descriptors and picks a tile
- * edge per shape class (from the autotuner's cache, else a size heuristic);
- * autotune: times both tile edges per shape class and caches the faster;
- * run: one launch over all problems, then waits; capture / replay: record
- * that launch in a hipGraph on a non-default stream and replay it.  in_t /
- * out_t: BF16 -> BF16 or F32, F16 -> F16, F32 -> F32. */
+ * owl_amd_conv_fwd).  Each problem has a plan: kind 0 / 1 = a full / half
+ * tile edge in the persistent launch, with 1..16 K splits (FP32 partials,
+ * summed in split order by the deterministic reduction after the launch),
+ * or kind 2 = a launch of the single-problem kernels of its own after it.
+ * create: validates the n descriptors and takes each plan from the
+ * autotuner's cache for its shape class, else a size heuristic (no splits);
+ * autotune: times every candidate plan per shape class and caches the
+ * fastest; plan / set_plan: read or impose the per-problem kinds and split
+ * counts (plan reports the effective split count, which can come out below
+ * the requested one); run: the launches over all problems, then waits;
+ * capture / replay: record a run in a hipGraph on a non-default stream and
+ * replay it (autotune and set_plan drop the graph: capture again).  Unsplit
+ * problems in the persistent launch equal the single-problem implicit-GEMM
+ * kernel bit for bit; own launches equal owl_amd_conv_fwd.  in_t / out_t:
+ * BF16 -> BF16 or F32, F16 -> F16, F32 -> F32. */
 typedef struct owl_amd_conv_group owl_amd_conv_group;
 int owl_amd_conv_group_create(owl_amd_conv_group **g, int n,
                               const owl_amd_conv_desc *descs, int in_t, int out_t);
@@ -249,6 +258,8 @@ int owl_amd_conv_group_capture(owl_amd_conv_group *g, const void *const *xs,
 int owl_amd_conv_group_replay(owl_amd_conv_group *g, void *stream);
 int owl_amd_conv_group_info(const owl_amd_conv_group *g, int *tile_edges,
                             int64_t *total_tiles);
+int owl_amd_conv_group_plan(const owl_amd_conv_group *g, int *kinds, int *splits);
+int owl_amd_conv_group_set_plan(owl_amd_conv_group *g, const int *kinds, const int *splits);
 void owl_amd_conv_group_destroy(owl_amd_conv_group *g);
 
 /* Pooling (owl_ndarray_pool_impl.h): F32 / F64 results bit-identical to

@@ -120,6 +120,8 @@ def lib():
             "owl_amd_conv_group_capture": ([vp, vp, vp, vp, vp], i32),
             "owl_amd_conv_group_replay": ([vp, vp], i32),
             "owl_amd_conv_group_info": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i64)], i32),
+            "owl_amd_conv_group_plan": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
+            "owl_amd_conv_group_set_plan": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
             "owl_amd_conv_group_destroy": ([vp], None),
         }
         for name, (args, res) in sig.items():

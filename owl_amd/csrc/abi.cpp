@@ -1884,17 +1884,28 @@ DEFINE_UPSAMPLE_DROPINS(complex64, owl_amd_complex64)
 
 // ------------------------------------------------------------------ ragged groups
 // A plan for running many forward convolutions (different shapes) as one
-// persistent launch.  Tile edge per problem comes from the shape-class cache
-// the autotuner fills (or a size heuristic until it has run).
+// persistent launch.  Each problem's plan -- tile edge, split-K factor, or a
+// launch of its own -- comes from the shape-class cache the autotuner fills
+// (or a size heuristic until it has run).
 namespace {
 
 // in_t, out_t, aligned, log2 M, log2 N, log2 K, unit strides, taps
 typedef std::tuple<int, int, int, int, int, int, int, int> ClassKey;
-// cfg of a problem: its tile edge in the persistent launch (0 full, 1 half),
-// or its own launch of the single-problem kernels (owl_amd_conv_fwd picks
-// the halo, direct-A or generic kernel) after the persistent one
+// plan kinds: the tile edge in the persistent launch (0 full, 1 half), or a
+// launch of the single-problem kernels of its own (owl_amd_conv_fwd picks the
+// halo, direct-A or generic kernel) after the persistent one
 constexpr int GROUP_OWN_LAUNCH = 2;
-std::map<ClassKey, int> g_tune_cache;
+constexpr int GROUP_MAX_SPLIT = 16;
+// split-K candidates the autotuner tries keep at least this many K-steps per
+// split, and at most this many bytes of FP32 partials per shape class
+constexpr int64_t kGroupMinStepsPerSplit = 16;
+constexpr size_t kGroupSplitCap = (size_t)256 << 20;
+
+struct GroupPlan {
+  int kind;   // 0 / 1 / GROUP_OWN_LAUNCH
+  int split;  // requested K splits in the persistent launch (1: none)
+};
+std::map<ClassKey, GroupPlan> g_tune_cache;
 
 int ilog2(int64_t v) {
   int r = 0;
@@ -1905,6 +1916,8 @@ int ilog2(int64_t v) {
   return r;
 }
 
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
 }  // namespace
 
 struct owl_amd_conv_group {
@@ -1912,7 +1925,11 @@ struct owl_amd_conv_group {
   std::vector<owl_amd_conv_desc> descs;
   std::vector<GroupEntry> tab;
   std::vector<ClassKey> keys;
+  std::vector<size_t> ws_off;     // byte offset of each split problem's partials in ws
+  std::vector<void *> ys;         // outputs of the last run / capture / autotune
   GroupEntry *dev_tab = nullptr;
+  void *ws = nullptr;             // FP32 split-K partials of the plan
+  size_t ws_bytes = 0;
   int64_t total = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1924,18 +1941,77 @@ static int group_edge(int in_t, int cfg) {
   return group_tile_edge<bf16_t>(cfg);
 }
 
-// Tile ranges of the persistent launch; problems with their own launch own
-// no tiles (the kernel's search over tile_begin skips them).
-static void group_layout(owl_amd_conv_group *g) {
+static int group_kstep(int in_t) {
+  if (in_t == OWL_AMD_F32) return group_k_step<float>();
+  if (in_t == OWL_AMD_F16) return group_k_step<f16_t>();
+  return group_k_step<bf16_t>();
+}
+
+static int64_t group_tiles_mn(int in_t, const GroupEntry &e) {
+  const int edge = group_edge(in_t, e.cfg);
+  return ((e.p.M + edge - 1) / edge) * (int64_t)((e.p.N + edge - 1) / edge);
+}
+
+// Set a problem's plan: tile edge, and the K splits as whole K-steps (the
+// effective count can come out below the requested one; 1 when the problem
+// is empty or has its own launch).
+static void group_apply(int in_t, GroupEntry &e, GroupPlan plan) {
+  e.cfg = plan.kind;
+  e.p.split = 1;
+  e.p.k_per_split = 0;
+  if (plan.kind == GROUP_OWN_LAUNCH || plan.split <= 1 || e.p.M == 0 || e.p.N == 0 || e.p.K == 0) return;
+  const int64_t bk = group_kstep(in_t);
+  const int64_t steps = (e.p.K + bk - 1) / bk;
+  const int64_t kps = ((steps + plan.split - 1) / plan.split) * bk;
+  const int64_t s = (e.p.K + kps - 1) / kps;
+  if (s < 2) return;
+  e.p.split = (int)s;
+  e.p.k_per_split = kps;
+}
+
+static size_t group_partial_bytes(const GroupEntry &e) {
+  return e.p.split > 1 ? (size_t)e.p.split * (size_t)e.p.M * (size_t)e.p.N * sizeof(float) : 0;
+}
+
+static void group_drop_graph(owl_amd_conv_group *g) {
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->exec = nullptr;
+  g->graph = nullptr;
+}
+
+// Tile ranges of the persistent launch (problems with their own launch own
+// no tiles: the kernel's search over tile_begin skips them) and the
+// partials workspace of the split problems, grown when the plan needs more.
+static int group_layout(owl_amd_conv_group *g) {
   int64_t t = 0;
+  size_t off = 0;
   for (int i = 0; i < g->n; ++i) {
     GroupEntry &e = g->tab[i];
     e.tile_begin = t;
+    e.tiles_mn = 0;
+    g->ws_off[i] = off;
     if (e.cfg == GROUP_OWN_LAUNCH) continue;
-    const int edge = group_edge(g->in_t, e.cfg);
-    t += ((e.p.M + edge - 1) / edge) * (int64_t)((e.p.N + edge - 1) / edge);
+    e.tiles_mn = group_tiles_mn(g->in_t, e);
+    t += e.tiles_mn * e.p.split;
+    off += align256(group_partial_bytes(e));
   }
   g->total = t;
+  if (off > g->ws_bytes) {
+    if (g->ws) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(g->ws);
+      g->ws = nullptr;
+      g->ws_bytes = 0;
+    }
+    hipError_t err = hipMalloc(&g->ws, off);
+    if (err != hipSuccess) {
+      owl_amd_set_error("hipMalloc group split-K partials (%zu bytes): %s", off, hipGetErrorString(err));
+      return OWL_AMD_ERR_NOMEM;
+    }
+    g->ws_bytes = off;
+  }
+  return OWL_AMD_OK;
 }
 
 static int group_launch(owl_amd_conv_group *g, const GroupEntry *tab, int nprob, int64_t total, hipStream_t st) {
@@ -1949,22 +2025,43 @@ static int group_launch(owl_amd_conv_group *g, const GroupEntry *tab, int nprob,
   return OWL_AMD_OK;
 }
 
-// The problems of indices idx (or all with their own launch when idx is
-// null), each as a single-problem forward on st.
-static int group_own_launches(owl_amd_conv_group *g, const std::vector<int> *idx, hipStream_t st) {
+// Sum a split problem's partials (at p.c) into its output, in split order.
+static int group_reduce(owl_amd_conv_group *g, const ConvParams &p, void *y, hipStream_t st) {
+  const float *part = (const float *)p.c;
+  const int64_t n = p.M * (int64_t)p.N;
+  if (g->out_t == OWL_AMD_F32) run_splitk_reduce<float, float>(part, (float *)y, n, p.split, st);
+  else if (g->out_t == OWL_AMD_BF16) run_splitk_reduce<float, bf16_t>(part, (bf16_t *)y, n, p.split, st);
+  else run_splitk_reduce<float, f16_t>(part, (f16_t *)y, n, p.split, st);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
+static bool in_set(const std::vector<int> *idx, int i) {
+  return !idx || std::find(idx->begin(), idx->end(), i) != idx->end();
+}
+
+// After a persistent launch: the split-K reductions of its split problems,
+// then the problems with their own launch, each as a single-problem forward
+// on st -- those of idx, or all when idx is null.
+static int group_finish(owl_amd_conv_group *g, const GroupEntry *tab, const std::vector<int> *idx, hipStream_t st) {
   for (int i = 0; i < g->n; ++i) {
-    if (idx ? std::find(idx->begin(), idx->end(), i) == idx->end() : g->tab[i].cfg != GROUP_OWN_LAUNCH) continue;
-    const ConvParams &p = g->tab[i].p;
-    int rc = owl_amd_conv_fwd(&g->descs[i], g->in_t, g->out_t, p.a, p.b, p.c, st);
+    if (!in_set(idx, i) || tab[i].cfg == GROUP_OWN_LAUNCH || tab[i].p.split <= 1) continue;
+    int rc = group_reduce(g, tab[i].p, g->ys[i], st);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < g->n; ++i) {
+    if (!in_set(idx, i) || tab[i].cfg != GROUP_OWN_LAUNCH) continue;
+    const ConvParams &p = tab[i].p;
+    int rc = owl_amd_conv_fwd(&g->descs[i], g->in_t, g->out_t, p.a, p.b, g->ys[i], st);
     if (rc) return rc;
   }
   return OWL_AMD_OK;
 }
 
-// One run: the persistent launch, then the problems with their own launch.
+// One run: the persistent launch, the reductions, the own launches.
 static int group_dispatch(owl_amd_conv_group *g, hipStream_t st) {
   int rc = group_launch(g, g->dev_tab, g->n, g->total, st);
-  return rc ? rc : group_own_launches(g, nullptr, st);
+  return rc ? rc : group_finish(g, g->tab.data(), nullptr, st);
 }
 
 static void group_set_ptrs(owl_amd_conv_group *g, const void *const *xs, const void *const *ws, void *const *ys) {
@@ -1972,10 +2069,19 @@ static void group_set_ptrs(owl_amd_conv_group *g, const void *const *xs, const v
     GroupEntry &e = g->tab[i];
     e.p.a = xs[i];
     e.p.b = ws[i];
-    e.p.c = ys[i];
+    g->ys[i] = ys[i];
+    e.p.c = e.p.split > 1 ? (void *)((char *)g->ws + g->ws_off[i]) : ys[i];
     const int v = vec_of(g->in_t);
     e.aligned = e.p.Ci % v == 0 && e.p.Co % v == 0 && aligned16(xs[i], ws[i], nullptr);
   }
+}
+
+static int group_check_ptrs(const owl_amd_conv_group *g, const void *const *xs, const void *const *ws,
+                            void *const *ys) {
+  if (g->n > 0 && (!xs || !ws || !ys)) ARG_FAIL("bad group arguments");
+  for (int i = 0; i < g->n; ++i)
+    if (!xs[i] || !ws[i] || !ys[i]) ARG_FAIL("null pointer for problem %d", i);
+  return OWL_AMD_OK;
 }
 
 extern "C" int owl_amd_conv_group_create(owl_amd_conv_group **out, int n, const owl_amd_conv_desc *descs,
@@ -1998,6 +2104,8 @@ extern "C" int owl_amd_conv_group_create(owl_amd_conv_group **out, int n, const 
   g->descs.assign(descs, descs + n);
   g->tab.resize(n);
   g->keys.resize(n);
+  g->ws_off.assign(n, 0);
+  g->ys.assign(n, nullptr);
   for (int i = 0; i < n; ++i) {
     GroupEntry &e = g->tab[i];
     memset(&e, 0, sizeof(e));
@@ -2010,17 +2118,30 @@ extern "C" int owl_amd_conv_group_create(owl_amd_conv_group **out, int n, const 
     const int al = e.p.Ci % v == 0 && e.p.Co % v == 0;
     g->keys[i] = ClassKey(in_t, out_t, al, ilog2(e.p.M), ilog2(e.p.N), ilog2(e.p.K),
                           e.p.sh == 1 && e.p.sw == 1 && e.p.sd == 1, e.p.kh * e.p.kw * e.p.kd);
-    auto it = g_tune_cache.find(g->keys[i]);
-    // heuristic until tuned: half-size tiles when the full tile would leave
-    // most of the chip idle or mostly empty
-    const int full = group_edge(in_t, 0);
-    const int64_t full_tiles = ((e.p.M + full - 1) / full) * (int64_t)((e.p.N + full - 1) / full);
-    e.cfg = it != g_tune_cache.end() ? it->second : (full_tiles < 256 || e.p.N <= full / 2 ? 1 : 0);
+    GroupPlan plan;
+    {
+      std::lock_guard<std::recursive_mutex> lk(g_mu);
+      auto it = g_tune_cache.find(g->keys[i]);
+      if (it != g_tune_cache.end()) {
+        plan = it->second;
+      } else {
+        // heuristic until tuned: half-size tiles when the full tile would
+        // leave most of the chip idle or mostly empty; no splits
+        const int full = group_edge(in_t, 0);
+        const int64_t full_tiles = ((e.p.M + full - 1) / full) * (int64_t)((e.p.N + full - 1) / full);
+        plan = GroupPlan{full_tiles < 256 || e.p.N <= full / 2 ? 1 : 0, 1};
+      }
+    }
+    group_apply(in_t, e, plan);
   }
-  group_layout(g);
+  int rc = group_layout(g);
+  if (rc) {
+    owl_amd_conv_group_destroy(g);
+    return rc;
+  }
   hipError_t err = hipMalloc(&g->dev_tab, sizeof(GroupEntry) * (n > 0 ? n : 1));
   if (err != hipSuccess) {
-    delete g;
+    owl_amd_conv_group_destroy(g);
     owl_amd_set_error("hipMalloc group table: %s", hipGetErrorString(err));
     return OWL_AMD_ERR_NOMEM;
   }
@@ -2037,122 +2158,175 @@ extern "C" int owl_amd_conv_group_info(const owl_amd_conv_group *g, int *tile_ed
   return OWL_AMD_OK;
 }
 
+extern "C" int owl_amd_conv_group_plan(const owl_amd_conv_group *g, int *kinds, int *splits) {
+  if (!g) ARG_FAIL("null group");
+  for (int i = 0; i < g->n; ++i) {
+    if (kinds) kinds[i] = g->tab[i].cfg;
+    if (splits) splits[i] = g->tab[i].p.split;
+  }
+  return OWL_AMD_OK;
+}
+
+extern "C" int owl_amd_conv_group_set_plan(owl_amd_conv_group *g, const int *kinds, const int *splits) {
+  if (!g || (g->n > 0 && (!kinds || !splits))) ARG_FAIL("bad group plan arguments");
+  for (int i = 0; i < g->n; ++i) {
+    if (kinds[i] < 0 || kinds[i] > GROUP_OWN_LAUNCH) ARG_FAIL("problem %d: plan kind %d not 0, 1 or 2", i, kinds[i]);
+    if (splits[i] < 1 || splits[i] > GROUP_MAX_SPLIT)
+      ARG_FAIL("problem %d: split %d outside 1..%d", i, splits[i], GROUP_MAX_SPLIT);
+    if (kinds[i] == GROUP_OWN_LAUNCH && splits[i] != 1) ARG_FAIL("problem %d: an own launch takes split 1", i);
+  }
+  for (int i = 0; i < g->n; ++i) group_apply(g->in_t, g->tab[i], GroupPlan{kinds[i], splits[i]});
+  group_drop_graph(g);
+  return group_layout(g);
+}
+
 extern "C" int owl_amd_conv_group_run(owl_amd_conv_group *g, const void *const *xs, const void *const *ws,
                                       void *const *ys, void *stream) {
-  if (!g || (g->n > 0 && (!xs || !ws || !ys))) ARG_FAIL("bad group run arguments");
-  for (int i = 0; i < g->n; ++i)
-    if (!xs[i] || !ws[i] || !ys[i]) ARG_FAIL("null pointer for problem %d", i);
+  if (!g) ARG_FAIL("bad group run arguments");
+  int rc = group_check_ptrs(g, xs, ws, ys);
+  if (rc) return rc;
   group_set_ptrs(g, xs, ws, ys);
   hipStream_t st = (hipStream_t)stream;
   // the table copy is ordered before the launch on the same stream; the host
   // vector is not touched again until the next run, which first waits here
   OWL_CHECK_HIP(hipMemcpyAsync(g->dev_tab, g->tab.data(), sizeof(GroupEntry) * g->n, hipMemcpyHostToDevice, st));
-  int rc = group_dispatch(g, st);
+  rc = group_dispatch(g, st);
   if (rc) return rc;
   OWL_CHECK_HIP(hipStreamSynchronize(st));
   return OWL_AMD_OK;
 }
 
-// Time each shape class three ways -- a grouped launch of just that class's
-// problems with either tile edge, and the class's problems as single-problem
-// launches -- and keep the fastest in the process-wide cache.
+// Time each shape class under every candidate plan -- a grouped launch of
+// just that class's problems with either tile edge, unsplit or split-K in
+// two to sixteen splits (when every problem of the class keeps >= 16 K-steps
+// per split and its partials stay within 256 MiB), and the class's problems
+// as single-problem launches -- and keep the fastest in the process-wide
+// cache.  A captured graph is dropped (capture again after tuning).
 extern "C" int owl_amd_conv_group_autotune(owl_amd_conv_group *g, const void *const *xs, const void *const *ws,
                                            void *const *ys, void *stream) {
   if (!g) ARG_FAIL("null group");
+  int rc = group_check_ptrs(g, xs, ws, ys);
+  if (rc) return rc;
+  group_drop_graph(g);
   group_set_ptrs(g, xs, ws, ys);
   hipStream_t st = (hipStream_t)stream;
   std::map<ClassKey, std::vector<int>> classes;
   for (int i = 0; i < g->n; ++i) classes[g->keys[i]].push_back(i);
+  const int64_t bk = group_kstep(g->in_t);
+  // candidate plans per class, and the partials the largest one needs
+  std::map<ClassKey, std::vector<GroupPlan>> cands;
+  size_t tmp_ws_bytes = 0;
+  for (auto &kv : classes) {
+    std::vector<GroupPlan> &c = cands[kv.first];
+    c.push_back(GroupPlan{0, 1});
+    c.push_back(GroupPlan{1, 1});
+    int64_t min_steps = INT64_MAX;
+    size_t mn = 0;
+    for (int i : kv.second) {
+      min_steps = std::min(min_steps, (g->tab[i].p.K + bk - 1) / bk);
+      mn += (size_t)g->tab[i].p.M * (size_t)g->tab[i].p.N;
+    }
+    for (int s = 2; s <= GROUP_MAX_SPLIT; s *= 2) {
+      const size_t bytes = (size_t)s * mn * sizeof(float) + kv.second.size() * 256;
+      if (min_steps / s < kGroupMinStepsPerSplit || bytes > kGroupSplitCap) break;
+      c.push_back(GroupPlan{0, s});
+      c.push_back(GroupPlan{1, s});
+      tmp_ws_bytes = std::max(tmp_ws_bytes, bytes);
+    }
+    c.push_back(GroupPlan{GROUP_OWN_LAUNCH, 1});
+  }
   GroupEntry *tmp = nullptr;
+  void *tmp_ws = nullptr;
   OWL_CHECK_HIP(hipMalloc(&tmp, sizeof(GroupEntry) * (g->n > 0 ? g->n : 1)));
+  if (tmp_ws_bytes && hipMalloc(&tmp_ws, tmp_ws_bytes) != hipSuccess) {
+    (void)hipFree(tmp);
+    owl_amd_set_error("hipMalloc autotune split-K partials (%zu bytes)", tmp_ws_bytes);
+    return OWL_AMD_ERR_NOMEM;
+  }
   hipEvent_t e0, e1;
   OWL_CHECK_HIP(hipEventCreate(&e0));
   OWL_CHECK_HIP(hipEventCreate(&e1));
-  int rc = OWL_AMD_OK;
+  std::vector<GroupEntry> sub_tab(g->tab);  // the class's entries at their problem index
   for (auto &kv : classes) {
     float best = 1e30f;
-    int best_cfg = 0;
-    for (int cfg = 0; cfg < 3 && rc == OWL_AMD_OK; ++cfg) {
-      if (cfg == GROUP_OWN_LAUNCH) {
-        // the class's problems as single-problem launches
-        rc = group_own_launches(g, &kv.second, st);  // warm-up
-        if (rc) break;
-        (void)hipEventRecord(e0, st);
-        for (int r = 0; r < 3 && rc == OWL_AMD_OK; ++r) rc = group_own_launches(g, &kv.second, st);
-        (void)hipEventRecord(e1, st);
-        (void)hipEventSynchronize(e1);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (rc == OWL_AMD_OK && ms < best) {
-          best = ms;
-          best_cfg = cfg;
-        }
-        continue;
-      }
+    GroupPlan best_plan{0, 1};
+    for (const GroupPlan &plan : cands[kv.first]) {
+      // a table of just this class's problems under the candidate plan
       std::vector<GroupEntry> sub;
       int64_t t = 0;
-      const int edge = group_edge(g->in_t, cfg);
+      size_t off = 0;
       for (int i : kv.second) {
         GroupEntry e = g->tab[i];
-        e.cfg = cfg;
+        group_apply(g->in_t, e, plan);
         e.tile_begin = t;
-        t += ((e.p.M + edge - 1) / edge) * (int64_t)((e.p.N + edge - 1) / edge);
+        e.tiles_mn = plan.kind == GROUP_OWN_LAUNCH ? 0 : group_tiles_mn(g->in_t, e);
+        t += e.tiles_mn * e.p.split;
+        e.p.c = g->ys[i];
+        if (e.p.split > 1) {
+          e.p.c = (char *)tmp_ws + off;
+          off += align256(group_partial_bytes(e));
+        }
         sub.push_back(e);
+        sub_tab[i] = e;
       }
-      if (hipMemcpy(tmp, sub.data(), sizeof(GroupEntry) * sub.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      if (t > 0 && hipMemcpy(tmp, sub.data(), sizeof(GroupEntry) * sub.size(), hipMemcpyHostToDevice) != hipSuccess) {
         rc = OWL_AMD_ERR_HIP;
         break;
       }
-      rc = group_launch(g, tmp, (int)sub.size(), t, st);  // warm-up
+      auto once = [&]() {
+        int r = group_launch(g, tmp, (int)sub.size(), t, st);
+        return r ? r : group_finish(g, sub_tab.data(), &kv.second, st);
+      };
+      rc = once();  // warm-up
       if (rc) break;
       (void)hipEventRecord(e0, st);
-      for (int r = 0; r < 3 && rc == OWL_AMD_OK; ++r) rc = group_launch(g, tmp, (int)sub.size(), t, st);
+      for (int r = 0; r < 3 && rc == OWL_AMD_OK; ++r) rc = once();
       (void)hipEventRecord(e1, st);
       (void)hipEventSynchronize(e1);
+      if (rc) break;
       float ms = 0;
       (void)hipEventElapsedTime(&ms, e0, e1);
       if (ms < best) {
         best = ms;
-        best_cfg = cfg;
+        best_plan = plan;
       }
     }
     if (rc) break;
     {
       std::lock_guard<std::recursive_mutex> lk(g_mu);
-      g_tune_cache[kv.first] = best_cfg;
+      g_tune_cache[kv.first] = best_plan;
     }
-    for (int i : kv.second) g->tab[i].cfg = best_cfg;
+    for (int i : kv.second) group_apply(g->in_t, g->tab[i], best_plan);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(st);
   (void)hipFree(tmp);
-  group_layout(g);
-  return rc;
+  if (tmp_ws) (void)hipFree(tmp_ws);
+  int lrc = group_layout(g);
+  return rc ? rc : lrc;
 }
 
 // Capture one run into a hipGraph (the problem table is uploaded before
-// capture, so the graph holds the grouped kernel node followed by the
-// kernel nodes of the problems with their own launch).
+// capture, so the graph holds the grouped kernel node, then the split-K
+// reductions and the kernel nodes of the problems with their own launch).
 extern "C" int owl_amd_conv_group_capture(owl_amd_conv_group *g, const void *const *xs, const void *const *ws,
                                           void *const *ys, void *stream) {
   if (!g || !stream) ARG_FAIL("group capture needs a group and a non-default stream");
+  int rc = group_check_ptrs(g, xs, ws, ys);
+  if (rc) return rc;
   group_set_ptrs(g, xs, ws, ys);
   OWL_CHECK_HIP(hipMemcpy(g->dev_tab, g->tab.data(), sizeof(GroupEntry) * g->n, hipMemcpyHostToDevice));
   hipStream_t st = (hipStream_t)stream;
-  if (g->exec) {
-    (void)hipGraphExecDestroy(g->exec);
-    g->exec = nullptr;
-  }
-  if (g->graph) {
-    (void)hipGraphDestroy(g->graph);
-    g->graph = nullptr;
-  }
+  group_drop_graph(g);
   OWL_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  int rc = group_dispatch(g, st);
+  rc = group_dispatch(g, st);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(st, &graph);
-  if (rc) return rc;
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
   if (e != hipSuccess) {
     owl_amd_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
     return OWL_AMD_ERR_HIP;
@@ -2170,11 +2344,9 @@ extern "C" int owl_amd_conv_group_replay(owl_amd_conv_group *g, void *stream) {
 
 extern "C" void owl_amd_conv_group_destroy(owl_amd_conv_group *g) {
   if (!g) return;
-  if (g->exec) (void)hipGraphExecDestroy(g->exec);
-  if (g->graph) (void)hipGraphDestroy(g->graph);
-  if (g->dev_tab) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(g->dev_tab);
-  }
+  group_drop_graph(g);
+  if (g->dev_tab || g->ws) (void)hipDeviceSynchronize();
+  if (g->dev_tab) (void)hipFree(g->dev_tab);
+  if (g->ws) (void)hipFree(g->ws);
   delete g;
 }

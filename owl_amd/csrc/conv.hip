@@ -216,8 +216,11 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
   const int64_t m0 = (tile / n_tiles) * BM;
   const int n0 = (int)(tile % n_tiles) * BN;
 
+  // split-K: the backward kernel's pixel splits, or a grouped forward's
+  // (tap, channel) splits
+  const bool splitk = PASS == 2 || (PASS == 0 && p.split > 1);
   int64_t kbeg = 0, kend = Kfull;
-  if (PASS == 2) {
+  if (splitk) {
     kbeg = (int64_t)split_idx * p.k_per_split;
     kend = kbeg + p.k_per_split;
     if (kend > Kfull) kend = Kfull;
@@ -569,7 +572,7 @@ __device__ __forceinline__ void igemm_tile(const ConvParams &p, int64_t tile, in
       for (int r = 0; r < MF::NACC; ++r) {
         int64_t m = m0 + wm * TL::WM + i * MF::TM + acc_row<MF::TM>(lane, r);
         if (m >= M) continue;
-        if (PASS == 2 && p.split > 1) {
+        if (splitk && p.split > 1) {
           typedef typename MF::acc_elem TA;
           TA *ws = (TA *)p.c + (int64_t)split_idx * M * N;
           ws[m * N + n] = (TA)acc[i][j][r];
@@ -646,10 +649,13 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const TA *__restrict
 // ---------------------------------------------------------------------------
 // Ragged batches: many forward convolutions of different shapes in one
 // persistent launch.  The host builds a table of problems (each with its
-// own tile edge, chosen per shape class) and the prefix sum of their tile
-// counts; workgroups stride over the concatenated tile list, find their
-// problem by binary search and run the same tile body as the single-problem
-// kernel, so a grouped result equals the separate launch's bit for bit.
+// own tile edge and split-K factor, chosen per shape class) and the prefix
+// sum of their tile counts; workgroups stride over the concatenated tile
+// list, find their problem by binary search and run the same tile body as
+// the single-problem implicit-GEMM kernel, so an unsplit grouped result
+// equals that kernel's bit for bit.  A split problem's tiles write FP32
+// partials (one slice per split of its (tap, channel) range), which the
+// deterministic split-K reduction then sums in split order.
 // ---------------------------------------------------------------------------
 template <typename TI, typename TO>
 __global__ __launch_bounds__(NT) void conv_group_fwd_kernel(const GroupEntry *__restrict__ tab, int nprob,
@@ -665,14 +671,21 @@ __global__ __launch_bounds__(NT) void conv_group_fwd_kernel(const GroupEntry *__
       else hi = mid - 1;
     }
     const ConvParams p = tab[lo].p;
-    const int64_t local = t - tab[lo].tile_begin;
+    int64_t local = t - tab[lo].tile_begin;
+    // split-major within a problem: split s owns tiles [s tiles_mn, (s+1) tiles_mn)
+    int sidx = 0;
+    if (p.split > 1) {
+      const int64_t tmn = tab[lo].tiles_mn;
+      sidx = (int)(local / tmn);
+      local -= sidx * tmn;
+    }
     const int cfg = tab[lo].cfg, al = tab[lo].aligned;
     if (cfg == 0) {
-      if (al) igemm_tile<0, TI, TO, true, Mfma<TI>::BM>(p, local, 0, lds);
-      else igemm_tile<0, TI, TO, false, Mfma<TI>::BM>(p, local, 0, lds);
+      if (al) igemm_tile<0, TI, TO, true, Mfma<TI>::BM>(p, local, sidx, lds);
+      else igemm_tile<0, TI, TO, false, Mfma<TI>::BM>(p, local, sidx, lds);
     } else {
-      if (al) igemm_tile<0, TI, TO, true, Mfma<TI>::BM / 2>(p, local, 0, lds);
-      else igemm_tile<0, TI, TO, false, Mfma<TI>::BM / 2>(p, local, 0, lds);
+      if (al) igemm_tile<0, TI, TO, true, Mfma<TI>::BM / 2>(p, local, sidx, lds);
+      else igemm_tile<0, TI, TO, false, Mfma<TI>::BM / 2>(p, local, sidx, lds);
     }
     __syncthreads();
   }
@@ -685,6 +698,7 @@ void run_group_fwd(const GroupEntry *dev_tab, int nprob, int64_t total, int grid
 }
 
 template <typename TI> int group_tile_edge(int cfg) { return cfg == 0 ? Mfma<TI>::BM : Mfma<TI>::BM / 2; }
+template <typename TI> int group_k_step() { return Mfma<TI>::BK; }
 
 template void run_group_fwd<bf16_t, bf16_t>(const GroupEntry *, int, int64_t, int, hipStream_t);
 template void run_group_fwd<bf16_t, float>(const GroupEntry *, int, int64_t, int, hipStream_t);
@@ -693,6 +707,9 @@ template void run_group_fwd<float, float>(const GroupEntry *, int, int64_t, int,
 template int group_tile_edge<bf16_t>(int);
 template int group_tile_edge<f16_t>(int);
 template int group_tile_edge<float>(int);
+template int group_k_step<bf16_t>();
+template int group_k_step<f16_t>();
+template int group_k_step<float>();
 
 // ---------------------------------------------------------------------------
 // Launchers

@@ -41,14 +41,16 @@ void run_igemm(const ConvParams &p, bool aligned, hipStream_t st);
 
 // One problem of a grouped (ragged) forward launch.
 struct GroupEntry {
-  ConvParams p;
+  ConvParams p;        // p.split > 1: p.c is the problem's FP32 partials
   int64_t tile_begin;  // first tile of this problem in the concatenated list
+  int64_t tiles_mn;    // output tiles of one split
   int cfg;             // 0: full tile edge, 1: half (for small problems)
   int aligned;
 };
 template <typename TI, typename TO>
 void run_group_fwd(const GroupEntry *dev_tab, int nprob, int64_t total, int grid, hipStream_t st);
 template <typename TI> int group_tile_edge(int cfg);
+template <typename TI> int group_k_step();  // K per step of the grouped tile body
 template <typename TA, typename TO>
 void run_splitk_reduce(const TA *ws, TO *out, int64_t n, int splits, hipStream_t st);
 template <typename T>

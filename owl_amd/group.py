@@ -2,10 +2,11 @@
 
 ``ConvGroup`` wraps the ``owl_amd_conv_group_*`` entry points: build a plan
 for many conv2d / conv3d problems of different shapes, optionally autotune
-per shape class the tile edge or a launch of the single-problem kernels of
-its own (the halo kernel wins for large stride-1 layers), then run all of
-them with one persistent kernel launch plus those own launches, or capture
-that run in a hipGraph and replay it.  Each problem's
+per shape class a plan -- the tile edge and split-K factor in the
+persistent launch, or a launch of the single-problem kernels of its own
+(the halo kernel wins for large stride-1 layers) -- then run all of them
+with one persistent kernel launch plus the split-K reductions and those own
+launches, or capture that run in a hipGraph and replay it.  Each problem's
 geometry is derived exactly like ``owl_amd.ndarray.dilated_conv2d``.
 """
 import ctypes
@@ -65,6 +66,22 @@ class ConvGroup:
 
     def replay(self, stream):
         _lib.check(_lib.lib().owl_amd_conv_group_replay(self._h, self._stream(stream)))
+
+    def plan(self):
+        """(per-problem kind: 0 full tile, 1 half tile, 2 own launch;
+        effective split-K count)."""
+        n = len(self.geoms)
+        k, s = (ctypes.c_int * max(n, 1))(), (ctypes.c_int * max(n, 1))()
+        _lib.check(_lib.lib().owl_amd_conv_group_plan(self._h, k, s))
+        return list(k)[:n], list(s)[:n]
+
+    def set_plan(self, kinds, splits):
+        """Impose per-problem kinds and requested split counts (1..16; 1
+        with kind 2); drops a captured graph."""
+        n = len(self.geoms)
+        k = (ctypes.c_int * max(n, 1))(*kinds)
+        s = (ctypes.c_int * max(n, 1))(*splits)
+        _lib.check(_lib.lib().owl_amd_conv_group_set_plan(self._h, k, s))
 
     def tile_edges(self):
         """(per-problem tile edge in the persistent launch, 0 for a problem

@@ -39,6 +39,12 @@ def test_dropin_table_matches_header():
         assert name in decl
 
 
+def test_group_plan_calls_check_arguments():
+    L = _lib.lib()
+    assert L.owl_amd_conv_group_set_plan(None, None, None) != 0
+    assert L.owl_amd_conv_group_plan(None, None, None) != 0
+
+
 def test_library_loads_and_binds():
     L = _lib.lib()
     for name in _lib.DROPINS:
