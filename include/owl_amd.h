@@ -81,6 +81,19 @@ int owl_amd_init(int device);
 const char *owl_amd_last_error(void);
 /* Number of device buffers and bytes cached by the drop-in layer. */
 int owl_amd_cache_stats(int64_t *n_buffers, int64_t *bytes);
+/* The drop-in layer's hipGraph cache (no counterpart in Owl): while on, a
+ * staging context runs a call kind / shape / dtype directly the first time
+ * it sees it, captures the device work of the second such call (for calls
+ * whose arrays fit the pinned chunks: the input DMAs, kernels and output
+ * DMAs; otherwise the kernels) and replays the graph for every later call
+ * on the same device buffers; host-side copies stay outside the graph.
+ * Off by default (replays measured no faster than enqueueing on MI355X);
+ * OWL_AMD_DROPIN_GRAPHS=1 or owl_amd_dropin_graphs(1) turns it on, (0) off;
+ * any other mode only queries.  Returns the current setting.  Bypassed
+ * while kernel timing is on.  graph_stats: graphs held now; captures /
+ * replays: totals since load. */
+int owl_amd_dropin_graphs(int mode);
+int owl_amd_dropin_graph_stats(int64_t *graphs, int64_t *captures, int64_t *replays);
 void owl_amd_cache_release(void);
 
 /* Kernel timing (diagnostics; no counterpart in the reference).  While

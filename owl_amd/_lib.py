@@ -91,6 +91,8 @@ def lib():
             "owl_amd_init": ([i32], i32),
             "owl_amd_last_error": ([], ctypes.c_char_p),
             "owl_amd_cache_stats": ([ctypes.POINTER(i64), ctypes.POINTER(i64)], i32),
+            "owl_amd_dropin_graphs": ([i32], i32),
+            "owl_amd_dropin_graph_stats": ([ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)], i32),
             "owl_amd_cache_release": ([], None),
             "owl_amd_complex_supported": ([], i32),
             "owl_amd_kernel_timing": ([i32], i32),
