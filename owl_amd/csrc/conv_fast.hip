@@ -57,6 +57,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, ui
 constexpr uint32_t OOB = 0x80000000u;  // >= every num_records used here (tensors < 2 GiB)
 
 typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u32 __attribute__((ext_vector_type(2)));
 // Cache policy of the 16-bit kernels' result stores: non-temporal (nt).  A
 // layer's output is written once and read by the next pass long after; the
 // streaming hint keeps the end-of-tile write bursts from allocating in L2
@@ -1437,7 +1438,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i], fb[j], acc[i][j]);
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fb[j], fa[i], acc[i][j]);
     } else if constexpr (FM == 4) {
       // 128 accumulators leave no room for both K halves' fragments: read
       // and consume one half at a time
@@ -1451,7 +1452,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i], fb[j], acc[i][j]);
+          for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fb[j], fa[i], acc[i][j]);
         __builtin_amdgcn_sched_barrier(0);
       }
     } else if constexpr (F8) {
@@ -1473,7 +1474,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
         for (int j = 0; j < FN; ++j)
           // cbsz = blgp = 0: both operands OCP e4m3; block scales 2^0
-          acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0, 127, 0, 127);
+          acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fb[j], fa[i], acc[i][j], 0, 0, 0, 127, 0, 127);
     } else {
       uint4 fa[FM][2], fb[FN][2];
 #pragma unroll
@@ -1492,7 +1493,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fa[i][s2], fb[j][s2], acc[i][j]);
+          for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(fb[j][s2], fa[i][s2], acc[i][j]);
     }
     // unconditional, so every step issues and waits for the same loads;
     // after the last step this fills a buffer nobody reads again
@@ -1528,30 +1529,39 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   if (ks < nk) step(ks, rb1);
 
   // ---- epilogue: output pixel (b, i0 + p / Wo, p % Wo), channel n ----
-  // Accumulator r of fragment (i, j) holds pixel prow + rofs(r) and channel
-  // ncol: a 32x32 result gives lane l column l % 32 of rows
-  // 8 (r / 4) + 4 (l / 32) + r % 4, a 16x16 result column l % 16 of rows
-  // 4 (l / 16) + r.
+  // The MFMAs take the kernel tile as their first operand, so each result
+  // block is channels x pixels: a 32x32 result gives lane l column (pixel)
+  // l % 32 of rows (channels) 8 (r / 4) + 4 (l / 32) + r % 4, a 16x16 result
+  // column l % 16 of rows 4 (l / 16) + r.  Accumulator r of fragment (i, j)
+  // thus holds tile pixel pix(i) = wm FM MS + i MS + frow and tile channel
+  // chn(j) + rofs(r), chn(j) = wn0 + j MS + 4 fkg: each run of four
+  // accumulators is four consecutive channels of one pixel, written as one
+  // 8-byte (16-bit) or 16-byte (FP32) piece of its NHWC row.
   const uint32_t o_bytes = (uint32_t)((int64_t)g.B * g.Ho * g.Wo * g.N * sizeof(TO));
   const int obase = (b * g.Ho + i0) * g.Wo;  // first output pixel of the tile
   auto rofs = [](int r) __attribute__((always_inline)) -> int { return MS == 32 ? 8 * (r >> 2) + (r & 3) : r; };
+  auto pix = [&](int i) __attribute__((always_inline)) -> int { return wm * FM * MS + i * MS + frow; };
+  auto chn = [&](int j) __attribute__((always_inline)) -> int { return wn0 + j * MS + 4 * fkg; };
   // fused bias and activation (forward only); relu is fmax(v, 0): NaN -> 0
   const bool epi = PASS == 0 && (g.bias || g.act);
-  float bj[FN];
-#pragma unroll
-  for (int j = 0; j < FN; ++j) {
-    const int n = n0 + wn0 + j * MS + frow;
-    bj[j] = (epi && g.bias && n < g.N) ? g.bias[n] : 0.0f;
-  }
   const float fsc = F8 ? g.sa[0] * g.sb[0] : 1.0f;
-  auto val = [&](int i, int j, int r, bool act) __attribute__((always_inline)) -> float {
-    float v = acc[i][j][r];
-    if (F8) v *= fsc;
-    if (epi) {
-      if (g.bias) v += bj[j];
-      if (act && g.act == OWL_AMD_ACT_RELU) v = fmaxf(v, 0.0f);
+  // the bias of run q (accumulators 4q .. 4q + 3) of fragment column j
+  auto bias4 = [&](int j, int q, float (&bv)[4]) __attribute__((always_inline)) {
+    const int n = n0 + chn(j) + rofs(4 * q);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) bv[e] = (epi && g.bias && n < g.N) ? g.bias[n + e] : 0.0f;
+  };
+  // run q of fragment (i, j) as four outputs of type TO
+  auto run4 = [&](int i, int j, int q, const float (&bv)[4], bool add_bias, bool relu,
+                  TO (&o)[4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float v = acc[i][j][4 * q + e];
+      if (F8) v *= fsc;
+      if (add_bias) v += bv[e];
+      if (relu) v = fmaxf(v, 0.0f);
+      o[e] = (TO)v;
     }
-    return v;
   };
   // staged output row: BN values, plus 16 B of padding when that still fits
   constexpr int ORS_PAD = BN * (int)sizeof(TO) + 16;
@@ -1572,17 +1582,16 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       // tested per value they cost 3.5 us on a 64x56x56x64 -> 128 layer.
       auto stage = [&](auto HB, auto HA) __attribute__((always_inline)) {
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) {
-            char *rowp = smem + (wm * FM * MS + i * MS + 4 * fkg) * ORS + (wn0 + j * MS + frow) * (int)sizeof(TO);
+          for (int q = 0; q < NR / 4; ++q) {
+            float bv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if constexpr (decltype(HB)::value) bias4(j, q, bv);
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
-              float v = acc[i][j][r];
-              if (F8) v *= fsc;
-              if constexpr (decltype(HB)::value) v += bj[j];
-              if constexpr (decltype(HA)::value) v = fmaxf(v, 0.0f);
-              *(TO *)(rowp + rofs(r) * ORS) = (TO)v;
+            for (int i = 0; i < FM; ++i) {
+              TO o[4];
+              run4(i, j, q, bv, decltype(HB)::value, decltype(HA)::value, o);
+              __builtin_memcpy(smem + pix(i) * ORS + (chn(j) + rofs(4 * q)) * (int)sizeof(TO), o, sizeof(o));
             }
           }
       };
@@ -1666,23 +1675,30 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
         }
       }
     } else if (dst) {
+      const bool relu = epi && act && g.act == OWL_AMD_ACT_RELU;
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+      for (int j = 0; j < FN; ++j)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          const int n = n0 + wn0 + j * MS + frow;
-          const int prow = wm * FM * MS + i * MS + 4 * fkg;
+        for (int q = 0; q < NR / 4; ++q) {
+          float bv[4];
+          bias4(j, q, bv);
+          const int n = n0 + chn(j) + rofs(4 * q);
 #pragma unroll
-          for (int r = 0; r < NR; ++r) {
-            const int pp = prow + rofs(r);
+          for (int i = 0; i < FM; ++i) {
+            const int pp = pix(i);
+            // N % 8 == 0: a run of four channels is wholly inside or outside
             const bool ok = pp < P && n < g.N;
             const uint32_t vo = ok ? (uint32_t)((obase + pp) * g.N + n) * (uint32_t)sizeof(TO) : OOB;
-            const float v = val(i, j, r, act);
+            TO o[4];
+            run4(i, j, q, bv, epi && g.bias, relu, o);
             if constexpr (sizeof(TO) == 2) {
-              const TO h = (TO)v;
-              __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), orr, (int)vo, 0, ST_NT);
+              v2u32 w2;
+              __builtin_memcpy(&w2, o, 8);
+              __builtin_amdgcn_raw_buffer_store_b64(w2, orr, (int)vo, 0, ST_NT);
             } else {
-              __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), orr, (int)vo, 0, ST_NT);
+              v4u32 w4;
+              __builtin_memcpy(&w4, o, 16);
+              __builtin_amdgcn_raw_buffer_store_b128(w4, orr, (int)vo, 0, ST_NT);
             }
           }
         }
