@@ -1182,10 +1182,19 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   constexpr bool SWZ = MS == 16 && KROWS;
   constexpr int BK = 32, BN = WGN * FN * MS, PR = SWZ ? BK : BK + 8;
   constexpr int LDR = PR;
-  // Two halo buffers of HBUF pixels: while one channel block's taps run,
-  // the next block's halo is loaded into registers (HQ 16-byte chunks per
-  // thread) and written to the other buffer.  A halo above HBUF pixels uses
-  // both buffers as one and is restaged between channel blocks.
+  // PF: two halo buffers of HBUF pixels; while one channel block's taps
+  // run, the next block's halo is loaded into registers (HQ 16-byte chunks
+  // per thread) at the block's first step and written to the other buffer
+  // at its last.  A halo above HBUF pixels uses both buffers as one: it is
+  // still prefetched into registers when they hold it, and written after
+  // the block's last barrier.  Without PF the halo is one buffer, restaged
+  // between channel blocks behind a barrier (HQ chunks per thread in flight
+  // at once).  The registers PF holds across a block cost the forward more
+  // than its early loads save (64x56x56x64 -> 128 layer, alternating
+  // processes: 34.7 against 37.1 us), and the 224-pixel backward input
+  // about as much (128-channel layers); the 448-pixel backward-input tiles
+  // (64 channels) gain from it (36.6 against 37.5 us).
+  constexpr bool PF = KROWS && WGN == 1;
   constexpr int HQ = CAP > 256 ? 10 : 6, HBUF = 352;
   // one LDS block: the two halo buffers, then the two kernel tiles; after
   // the K loop it holds the output tile for the 16-byte store epilogue
@@ -1244,10 +1253,10 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   // halo staging: chunk q -> pixel q / 4, 16-byte channel chunk q % 4;
   // OOB marks padding, which the buffer loads fill with zeros.
   const int hq_total = HR * HC * 4;
-  const bool dbl = HR * HC <= HBUF;
+  const bool dbl = PF && HR * HC <= HBUF;
   // a single-buffer halo can still be prefetched into registers when it fits
   // HQ chunks per thread; it is then written after the block's last barrier
-  const bool pf = hq_total <= HQ * 256;
+  const bool pf = PF && hq_total <= HQ * 256;
   auto chunk_src = [&](int q) __attribute__((always_inline)) -> uint32_t {
     const int pix = q >> 2, ch = q & 3;
     const int r = pix / HC, c = pix - (pix / HC) * HC;
