@@ -19,6 +19,21 @@ template <typename A> __device__ __forceinline__ A relu_factor(A z) {
   return (s + (A)1) / (A)2;
 }
 
+// g * factor, the IEEE product, for a result stored as T.  With FP16 results
+// the compiler selects the FP32 product and its rounding to FP16 as one
+// mixed-precision FMA with a +0 addend (v_fma_mixlo_f16 ..., 0), which gives
+// +0 where the product is -0 (a negative gradient times the factor 0, as
+// Owl's dy * (signum z + 1) / 2 keeps it); so for FP16 the product's sign
+// bit is set from the operands' explicitly, which the compiler cannot fold.
+template <typename T, typename A> __device__ __forceinline__ A adjoint_product(A g, A factor) {
+  A p = g * factor;
+  if constexpr (__is_same(T, f16_t)) {
+    const unsigned sign = (__builtin_bit_cast(unsigned, g) ^ __builtin_bit_cast(unsigned, factor)) & 0x80000000u;
+    p = __builtin_bit_cast(A, (__builtin_bit_cast(unsigned, p) & 0x7fffffffu) | sign);
+  }
+  return p;
+}
+
 // Block: 256 threads = PL pixel lanes x CW channel vectors of VEC channels.
 // Grid: (pixel chunks of PCH pixels, channel-vector blocks of CW).
 template <typename T, typename A, int VEC>
@@ -45,7 +60,7 @@ __global__ __launch_bounds__(256) void bias_act_bwd_kernel(int act, int64_t npix
 #pragma unroll
       for (int v = 0; v < VEC; ++v) {
         A d = (A)g[v];
-        if (act == OWL_AMD_ACT_RELU) d = d * relu_factor<A>((A)z[v]);
+        if (act == OWL_AMD_ACT_RELU) d = adjoint_product<T, A>(d, relu_factor<A>((A)z[v]));
         g[v] = (T)d;
         s[v] += (A)g[v];  // the stored (rounded) gradient is what the bias sums
       }
@@ -174,7 +189,7 @@ __global__ __launch_bounds__(256) void maxpool_relu_bwd_kernel(int64_t nwin, int
 #pragma unroll
         for (int v = 0; v < VEC; ++v) {
           const A g = arg[v] == k ? (A)(T)((A)0 + (A)d[v]) : (A)0;
-          o[v] = (T)(g * relu_factor<A>((A)zv[k][v]));
+          o[v] = (T)adjoint_product<T, A>(g, relu_factor<A>((A)zv[k][v]));
           s[v] += (A)o[v];
         }
         __builtin_memcpy(dz + cell[k] * C + c0, o, sizeof(o));

@@ -130,7 +130,7 @@ def _owl_relu_adjoint(pre, dy):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("shape", [[4, 7, 9, 64], [3, 5, 13], [2, 6, 6, 3], [1, 1000, 128]])
-@pytest.mark.parametrize("dt", ["f32", "f64", "bf16"])
+@pytest.mark.parametrize("dt", ["f32", "f64", "bf16", "f16"])
 def test_bias_act_backward(shape, dt):
     td = TD[dt]
     rng = np.random.default_rng(sum(shape))
@@ -142,10 +142,14 @@ def test_bias_act_backward(shape, dt):
     pt, gt = dev(pre, td), dev(dy, td)
     dz, db = nd.bias_act_backward(pt, gt, act="relu")
     ref = _owl_relu_adjoint(host(pt), host(gt))
+    if dt == "f16":
+        ref = ref.astype(np.float16).astype(np.float64)   # dy / 2 can land in FP16's subnormals
     got = host(dz)
     assert np.array_equal(np.isnan(got), np.isnan(ref))
     fin = ~np.isnan(ref)
     assert np.array_equal(got[fin], ref[fin])            # exact: the factor is 0, 1/2 or 1
+    # and the sign of zero: -0 for a negative gradient times the factor 0
+    assert np.array_equal(np.signbit(got[fin]), np.signbit(ref[fin]))
     C = shape[-1]
     ref_db = np.nansum(got.reshape(-1, C), axis=0)
     nan_cols = np.isnan(got.reshape(-1, C)).any(axis=0)
