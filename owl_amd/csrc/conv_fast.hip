@@ -439,11 +439,11 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
   const int M = (int)p.M, N = p.N;
   const int n_tiles = (N + BN - 1) / BN;
   const int mn_tiles = ((M + BM - 1) / BM) * n_tiles;
-  // 1-D grid of mn_tiles x split in launch order: the tiles of one split
-  // (which read the same pixels) spread over the XCDs.  Grouping them on one
-  // XCD cut the L2 fetch 5x on a 64x56x56x64 -> 128 layer but ran slower
-  // (the Infinity Cache already serves the re-reads).
-  const int lid = blockIdx.x;
+  // 1-D grid of mn_tiles x split: the tiles of one split read the same
+  // pixels, so consecutive tile indices go to one XCD (xcd_swizzle) and
+  // share its L2 (64x56x56x64 -> 128 BF16 layer 39.4 -> 37.5 us against
+  // the tiles of a split spread over the XCDs in launch order)
+  const int lid = xcd_swizzle(blockIdx.x, gridDim.x);
   const int tile = lid % mn_tiles, split_idx = lid / mn_tiles;
   const int m0 = (tile / n_tiles) * BM, n0 = (tile % n_tiles) * BN;
   // this group's pixels: the whole split (KG 1) or one half of it (KG 2,
@@ -768,7 +768,8 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_halo_ker
   const int n_tiles = (N + BN - 1) / BN;
   const int cblocks = p.Ci / CB;
   const int mn_tiles = p.kh * cblocks * n_tiles;
-  const int lid = blockIdx.x;
+  // the tiles of one split share their pixels: one XCD's L2 for them
+  const int lid = xcd_swizzle(blockIdx.x, gridDim.x);
   const int tile = lid % mn_tiles, split_idx = lid / mn_tiles;
   const int mt = tile / n_tiles, n0 = (tile % n_tiles) * BN;
   const int ta = mt / cblocks, cb = mt - (mt / cblocks) * cblocks;  // kernel row a, channel block
@@ -1208,7 +1209,10 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   constexpr int K_BYTES = HALO_BYTES + 2 * B_BYTES;
   constexpr int STG_BYTES = CAP * (BN * (int)sizeof(TO) + 16);
   constexpr int LDS_BYTES = sizeof(TO) == 2 && STG_BYTES > K_BYTES ? STG_BYTES : K_BYTES;
-  __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES];
+  // plus a 16-byte slot per lane past the end that takes the halo chunks
+  // beyond the halo (q >= hq_total), so that every lane writes its chunks
+  // unconditionally (one slot per lane: lanes writing one address conflict)
+  __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES + 64 * 16];
   TI *halo = (TI *)smem;
   char *const Bs0 = smem + HALO_BYTES;  // kernel tile buffer i at Bs0 + i * B_BYTES
 
@@ -1288,11 +1292,25 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       r += hs_r + cy;
     }
   };
+  // Every lane writes all HQ chunks, those past the halo into the dump slot:
+  // a write guarded per lane is an exec-masked branch, after which the
+  // compiler cannot tell that the chunk's load has been waited for, and it
+  // then waited for every load in flight (s_waitcnt vmcnt(0)) before the
+  // next step's fragment reads reused those registers, draining the
+  // kernel-tile ring at every step.
   auto halo_commit = [&](TI *hb) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < HQ; ++u) {
       const int q = tid + u * 256;
-      if (q < hq_total) *(uint4 *)&hb[(q >> 2) * PR + ((q & 3) ^ swz(q >> 2)) * 8] = hreg[u];
+      if constexpr (PF) {
+        // the prefetching form keeps the guarded write: the halo registers
+        // live across the block there, and the unguarded form spilled more
+        // (60 against 24 bytes per lane) and ran the backward input slower
+        if (q < hq_total) *(uint4 *)&hb[(q >> 2) * PR + ((q & 3) ^ swz(q >> 2)) * 8] = hreg[u];
+      } else {
+        char *dst = q < hq_total ? (char *)&hb[(q >> 2) * PR + ((q & 3) ^ swz(q >> 2)) * 8] : smem + LDS_BYTES + lane * 16;
+        *(uint4 *)dst = hreg[u];
+      }
     }
   };
   auto stage_halo = [&](TI *hb, int c0) __attribute__((always_inline)) {
