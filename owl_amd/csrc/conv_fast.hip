@@ -1183,22 +1183,19 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   constexpr bool SWZ = MS == 16 && KROWS;
   constexpr int BK = 32, BN = WGN * FN * MS, PR = SWZ ? BK : BK + 8;
   constexpr int LDR = PR;
-  // PF: two halo buffers of HBUF pixels; while one channel block's taps
-  // run, the next block's halo is loaded into registers (HQ 16-byte chunks
-  // per thread) at the block's first step and written to the other buffer
-  // at its last.  A halo above HBUF pixels uses both buffers as one: it is
-  // still prefetched into registers when they hold it, and written after
-  // the block's last barrier.  Without PF the halo is one buffer, restaged
-  // between channel blocks behind a barrier (HQ chunks per thread in flight
-  // at once).  The registers PF holds across a block cost the forward more
-  // than its early loads save (64x56x56x64 -> 128 layer, alternating
-  // processes: 34.7 against 37.1 us), and the 224-pixel backward input
-  // about as much (128-channel layers); the 448-pixel backward-input tiles
-  // (64 channels) gain from it (36.6 against 37.5 us).
-  constexpr bool PF = KROWS && WGN == 1;
+  // The halo is one buffer of up to 2 HBUF pixels, restaged between channel
+  // blocks behind a barrier with HQ 16-byte chunks per thread in flight at
+  // once.  Loading the next block's halo into registers during a block (and
+  // double-buffering it in LDS) measured slower everywhere once the
+  // restaged form's chunk writes were made unconditional (see halo_commit):
+  // the registers it holds across a block cost the fragment pipelining more
+  // than the early loads save (64x56x56x64 -> 128 BF16 backward input 34.1
+  // against 31.4 us, FP8 forward 28.0 against 26.0 us), and spreading the
+  // next block's halo over the block's steps, one chunk per thread and
+  // step, ran the forward at 39.9 against 35.2 us.
   constexpr int HQ = CAP > 256 ? 10 : 6, HBUF = 352;
-  // one LDS block: the two halo buffers, then the two kernel tiles; after
-  // the K loop it holds the output tile for the 16-byte store epilogue
+  // one LDS block: the halo, then the two kernel tiles; after the K loop it
+  // holds the output tile for the 16-byte store epilogue
   constexpr int HALO_BYTES = 2 * HBUF * PR * (int)sizeof(TI);
   // natural-layout kernel tile: BK rows of BN values, padded by 64 bytes so
   // the four rows one transposing read phase touches cover all 64 banks
@@ -1257,10 +1254,6 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   // halo staging: chunk q -> pixel q / 4, 16-byte channel chunk q % 4;
   // OOB marks padding, which the buffer loads fill with zeros.
   const int hq_total = HR * HC * 4;
-  const bool dbl = PF && HR * HC <= HBUF;
-  // a single-buffer halo can still be prefetched into registers when it fits
-  // HQ chunks per thread; it is then written after the block's last barrier
-  const bool pf = PF && hq_total <= HQ * 256;
   auto chunk_src = [&](int q) __attribute__((always_inline)) -> uint32_t {
     const int pix = q >> 2, ch = q & 3;
     const int r = pix / HC, c = pix - (pix / HC) * HC;
@@ -1302,15 +1295,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
     for (int u = 0; u < HQ; ++u) {
       const int q = tid + u * 256;
-      if constexpr (PF) {
-        // the prefetching form keeps the guarded write: the halo registers
-        // live across the block there, and the unguarded form spilled more
-        // (60 against 24 bytes per lane) and ran the backward input slower
-        if (q < hq_total) *(uint4 *)&hb[(q >> 2) * PR + ((q & 3) ^ swz(q >> 2)) * 8] = hreg[u];
-      } else {
-        char *dst = q < hq_total ? (char *)&hb[(q >> 2) * PR + ((q & 3) ^ swz(q >> 2)) * 8] : smem + LDS_BYTES + lane * 16;
-        *(uint4 *)dst = hreg[u];
-      }
+      char *dst = q < hq_total ? (char *)&hb[(q >> 2) * PR + ((q & 3) ^ swz(q >> 2)) * 8] : smem + LDS_BYTES + lane * 16;
+      *(uint4 *)dst = hreg[u];
     }
   };
   auto stage_halo = [&](TI *hb, int c0) __attribute__((always_inline)) {
@@ -1442,9 +1428,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       ++ncb;
     }
     const bool more = ks + 1 < nk;
-    if (pf && tap == 0 && cb + 1 < cblocks) halo_issue((cb + 1) * BK);
     // A from the halo at this tap's offset, B from LDS
-    const TI *hcur = halo + (dbl ? (cb & 1) * HBUF * PR : 0);
+    const TI *hcur = halo;
     const int tsh = ta * g.rh * HC + tc * g.rw;  // the tap's pixel shift in the halo
     const int toff = tsh * PR;
     const char *bs = Bs0 + bcur * B_BYTES;
@@ -1527,17 +1512,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     store_b(rnext, bcur ^ 1);
     next_load(rnext);
     if (more && ncb != cb) {
-      if (dbl) {
-        // the other buffer was last read before the previous block's final
-        // barrier; the barrier below publishes the new halo
-        halo_commit(halo + (ncb & 1) * HBUF * PR);
-      } else if (pf) {
-        __syncthreads();  // every wave is done with this channel block's halo
-        halo_commit(halo);
-      } else {
-        __syncthreads();
-        stage_halo(halo, ncb * BK);
-      }
+      __syncthreads();  // every wave is done with this channel block's halo
+      stage_halo(halo, ncb * BK);
     }
     __syncthreads();
     tap = ntap;
@@ -1752,7 +1728,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 static int halo_cap(int N, int ms) { return N <= 64 ? (ms == 32 ? 512 : 448) : (ms == 32 ? 256 : 224); }
 
 // Whole output rows per tile for a capacity (0 when one row does not fit),
-// and whether their input halo fits the LDS (two 352-pixel buffers).
+// and whether their input halo fits the LDS (704 pixels).
 static int halo_rows(const HaloGeom &g, int cap) {
   const int tr = cap / g.Wo;
   return tr > g.Ho ? g.Ho : tr;
