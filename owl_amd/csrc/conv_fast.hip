@@ -104,7 +104,9 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
   const uint32_t a_bytes = PASS == 0 ? (uint32_t)((int64_t)p.B * p.H * p.W * p.D * p.Ci * ES)
                                      : (uint32_t)((int64_t)p.B * p.Ho * p.Wo * p.Do * p.Co * ES);
   const __amdgpu_buffer_rsrc_t arsrc = make_rsrc(p.a, a_bytes);
-  const TI *__restrict__ Bg = (const TI *)p.b;
+  // the kernel tile through buffer loads: columns past N and taps past the
+  // last one read zeros from out-of-range offsets, with no per-lane branch
+  const __amdgpu_buffer_rsrc_t brsrc = make_rsrc(p.b, (uint32_t)((int64_t)T * p.Ci * p.Co * ES));
 
   // ---- per-lane A rows (one per M fragment) ----
   // row_off: element offset of the row's window origin (may be negative for
@@ -205,18 +207,21 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
 
   auto load_b = [&](int slot, int tap_, int c0_) __attribute__((always_inline)) {
     TI *rbs = rb[slot];
+    const bool tok = tap_ < T;
     if (PASS == 0) {
       const int n = n0 + f_ng * 4;
-      const bool nok = n < N;
-      const TI *src = Bg + (tap_ * p.Ci + c0_ + f_kg * BKV) * p.Co + n;
+      const bool nok = tok && n < N;
+      const uint32_t src = (uint32_t)((tap_ * p.Ci + c0_ + f_kg * BKV) * p.Co + n) * ES;
 #pragma unroll
       for (int kv = 0; kv < BKV; ++kv) {
         TI tmp[4];
-        if (nok) {
-          __builtin_memcpy(tmp, src + kv * p.Co, 4 * sizeof(TI));
+        if constexpr (sizeof(TI) == 4) {
+          const uint4 v = bload(brsrc, nok ? src + (uint32_t)(kv * p.Co * ES) : OOB);
+          __builtin_memcpy(tmp, &v, 16);
         } else {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) tmp[v] = (TI)0.0f;
+          const uint2 v = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(
+                                                        brsrc, (int)(nok ? src + (uint32_t)(kv * p.Co * ES) : OOB), 0, 0));
+          __builtin_memcpy(tmp, &v, 8);
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) rbs[v * BKV + kv] = tmp[v];
@@ -224,9 +229,8 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
     } else {
 #pragma unroll
       for (int i = 0; i < BNV; ++i) {
-        int n = n0 + b_row + i * B_ROWS_PER_PASS;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (n < N) v = *(const uint4 *)(Bg + (tap_ * p.Ci + n) * p.Co + c0_ + b_kv);
+        const int n = n0 + b_row + i * B_ROWS_PER_PASS;
+        const uint4 v = bload(brsrc, tok && n < N ? (uint32_t)((tap_ * p.Ci + n) * p.Co + c0_ + b_kv) * ES : OOB);
         __builtin_memcpy(&rbs[i * VEC], &v, 16);
       }
     }
@@ -264,33 +268,30 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
 
   // prologue: A and B for steps 0 .. PD-1 into ring slots 0 .. PD-1; the B
   // tile of step 0 goes to LDS now, step s+1's at the end of iteration s.
+  // Loads past the last step are issued all the same (A from the last tap's
+  // rows, B from out-of-range offsets), so every step issues the same loads
+  // and the compiler's waits keep PD steps of them in flight: with the loads
+  // and the kernel-tile store conditional, it waited for every load in
+  // flight at each step.
   set_tap();
 #pragma unroll
   for (int u = 0; u < PD; ++u) {
-    if (u < nk) {
-      load_a(u);
-      advance_a();
-      load_b(u, tap, c0);
-      advance_b();
-    }
+    load_a(u);
+    advance_a();
+    load_b(u, tap, c0);
+    advance_b();
   }
   store_b(0, 0);
   __syncthreads();
 
   const int fb_off = (wn * FN * 32 + frow) * LDR + fkg * LK;
-  for (int kb = 0; kb < nk; kb += NS) {
-#pragma unroll
-    for (int u = 0; u < NS; ++u) {
-      const int ks = kb + u;
-      if (ks < nk) {
+  // step ks from ring slot u = ks % NS (a constant once unrolled)
+  auto step = [&](int u, int ks) __attribute__((always_inline)) {
         const int bcur = ks & 1;
-        const bool more = ks + 1 < nk;
-        if (ks + PD < nk) {
-          load_a((u + PD) % NS);
-          advance_a();
-          load_b((u + PD) % NS, tap, c0);
-          advance_b();
-        }
+        load_a((u + PD) % NS);
+        advance_a();
+        load_b((u + PD) % NS, tap, c0);
+        advance_b();
         const TI *bs = Bs[bcur] + fb_off;
         uint4 fb[FN][2];
 #pragma unroll
@@ -326,11 +327,18 @@ __global__ __launch_bounds__(256, 2) void conv_fast_kernel(ConvParams p) {
               for (int j = 0; j < FN; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[i][s2], bf[j][s2], acc[i][j], 0, 0, 0);
         }
-        if (more) store_b(bcur ^ 1, (u + 1) % NS);
+        // after the last step this fills a buffer nobody reads again
+        store_b(bcur ^ 1, (u + 1) % NS);
         __syncthreads();
-      }
-    }
+  };
+  int kb = 0;
+  for (; kb + NS <= nk; kb += NS) {
+#pragma unroll
+    for (int u = 0; u < NS; ++u) step(u, kb + u);
   }
+#pragma unroll
+  for (int u = 0; u < NS - 1; ++u)
+    if (kb + u < nk) step(u, kb + u);
 
   // ---- epilogue: lane holds column (lane % 32) of each fragment ----
   // Raw buffer stores: the uniform row offset goes in soffset, the lane's
@@ -629,7 +637,8 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
     store(rn, cur ^ 1);
     load(rn);
     // scheduling: the LDS reads as one group ahead of the MFMAs (an
-    // MFMA-by-MFMA interleave with the loads and writes ran slower)
+    // MFMA-by-MFMA interleave with the loads and writes ran slower, and so
+    // did the second quarter's reads spread between the first quarter's MFMAs)
     __builtin_amdgcn_sched_group_barrier(0x100, 2 * NQ * (FMW + 2), 0);
     __builtin_amdgcn_sched_group_barrier(0x008, 2 * NQ * FMW, 0);
     __syncthreads();
