@@ -424,7 +424,9 @@ int tile_m(int t) {
 // depends only on the shape and the input type, so the workspace query
 // agrees with the call.
 void plan_splitk(const ConvParams &p, int in_t, int *split, int64_t *kps) {
-  const bool w16 = in_t == OWL_AMD_BF16 || in_t == OWL_AMD_F16;
+  // the weight-gradient kernel's tiles (16-bit and FP32 storage), else the
+  // implicit GEMM's
+  const bool w16 = in_t == OWL_AMD_BF16 || in_t == OWL_AMD_F16 || in_t == OWL_AMD_F32;
   const int bm = w16 ? wgrad_tile_m(p.M) : tile_m(in_t), bn = w16 ? 128 : tile_m(in_t), bk = tile_k(in_t);
   int64_t tiles = ((p.M + bm - 1) / bm) * (int64_t)((p.N + bn - 1) / bn);
   static const int env_target = env_int("OWL_AMD_SPLIT_TARGET", 0);  // workgroups to aim for (tuning)
@@ -684,17 +686,22 @@ static int bwd_kernel_impl(const owl_amd_conv_desc *d, int in_t, int out_t, cons
   p.a = x; p.b = dy; p.ldc = p.Co;
   const int v = vec_of(in_t);
   bool aligned = dtype_size(in_t) && p.Ci % v == 0 && p.Co % v == 0 && aligned16(x, dy, nullptr);
-  // 16-bit storage: the transposing-LDS kernel of conv_fast.hip (32-bit byte
-  // offsets, so operands below 2 GiB)
-  const bool tr = fast_path_enabled() && aligned && (in_t == OWL_AMD_BF16 || in_t == OWL_AMD_F16) &&
-                  (int64_t)p.B * p.H * p.W * p.D * p.Ci * 2 < (1LL << 31) && p.K * (int64_t)p.Co * 2 < (1LL << 31) &&
+  // 16-bit and FP32 storage: the weight-gradient kernel of conv_fast.hip
+  // (32-bit byte offsets, so operands below 2 GiB; FP32 in, FP32 out)
+  const int64_t es = (int64_t)dtype_size(in_t);
+  const bool tr = fast_path_enabled() && aligned &&
+                  (in_t == OWL_AMD_BF16 || in_t == OWL_AMD_F16 || (in_t == OWL_AMD_F32 && out_t == OWL_AMD_F32)) &&
+                  (int64_t)p.B * p.H * p.W * p.D * p.Ci * es < (1LL << 31) && p.K * (int64_t)p.Co * es < (1LL << 31) &&
                   p.M * (int64_t)p.N * 4 < (1LL << 31) && p.K < (1LL << 31) &&
                   (out_t == in_t || out_t == OWL_AMD_F32);
+  const char *wg_name = in_t != OWL_AMD_F32 && wgrad_halo_fits(p) ? "conv_wgrad_halo_kernel" : "conv_wgrad_kernel";
   if (p.split <= 1) {
     p.c = dw;
     if (tr) {
-      KTimer kt(OWL_AMD_KT_BWD_KERNEL, wgrad_halo_fits(p) ? "conv_wgrad_halo_kernel" : "conv_wgrad_kernel", st);
-      if (in_t == OWL_AMD_BF16) {
+      KTimer kt(OWL_AMD_KT_BWD_KERNEL, wg_name, st);
+      if (in_t == OWL_AMD_F32) {
+        run_wgrad<float, float>(p, st);
+      } else if (in_t == OWL_AMD_BF16) {
         if (out_t == OWL_AMD_BF16) run_wgrad<bf16_t, bf16_t>(p, st); else run_wgrad<bf16_t, float>(p, st);
       } else {
         if (out_t == OWL_AMD_F16) run_wgrad<f16_t, f16_t>(p, st); else run_wgrad<f16_t, float>(p, st);
@@ -719,8 +726,9 @@ static int bwd_kernel_impl(const owl_amd_conv_desc *d, int in_t, int out_t, cons
   }
   p.c = ws;
   if (tr) {
-    KTimer kt(OWL_AMD_KT_BWD_KERNEL, wgrad_halo_fits(p) ? "conv_wgrad_halo_kernel" : "conv_wgrad_kernel", st);
-    if (in_t == OWL_AMD_BF16) run_wgrad<bf16_t, float>(p, st);
+    KTimer kt(OWL_AMD_KT_BWD_KERNEL, wg_name, st);
+    if (in_t == OWL_AMD_F32) run_wgrad<float, float>(p, st);
+    else if (in_t == OWL_AMD_BF16) run_wgrad<bf16_t, float>(p, st);
     else run_wgrad<f16_t, float>(p, st);
     OWL_CHECK_HIP(hipGetLastError());
   } else {

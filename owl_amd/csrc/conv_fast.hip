@@ -424,7 +424,13 @@ template <> __device__ __forceinline__ f32x16 mfma16<f16_t>(const v4s16 &a0, con
 }
 
 // FMW: 32-row fragments per wave along M; the tile is BM = 64 FMW rows
-// (2 x 2 waves of 32 FMW x 64).  D1: a 2-D problem (D = Do = kd = 1, no
+// (2 x 2 waves of 32 FMW x 64).  TI float: the FP32 weight gradient.  A
+// 32x32x2 FP32 MFMA takes one k value per lane and operand, which a plain
+// 4-byte LDS read of the natural [pixel][channel] rows supplies (lanes 0-31
+// one pixel's 32 channels, lanes 32-63 the next pixel's), so nothing is
+// transposed; a K-step is 16 pixels (the same bytes per row as 32 16-bit
+// pixels) and rows are padded to 128 bytes past a multiple of 256, which
+// puts the two half-waves' reads on disjoint banks.  D1: a 2-D problem (D = Do = kd = 1, no
 // depth pad), whose depth digit is always 0: the loader skips its bounds
 // check and carry.  KG: groups of four waves per workgroup.  With KG = 2 the
 // two groups sum the two halves of the split's pixels into the same tile,
@@ -434,12 +440,18 @@ template <> __device__ __forceinline__ f32x16 mfma16<f16_t>(const v4s16 &a0, con
 template <typename TI, typename TO, int FMW, bool D1, int KG>
 __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(ConvParams p) {
   // BK 64 (80 KB of LDS) ran slower than BK 32: one workgroup per CU
-  constexpr int BM = 64 * FMW, BN = 128, BK = 32, KP = BK / 32;  // KP pixels per loader thread
-  constexpr int QA = BM / 64;      // A chunks per loader thread and pixel
-  constexpr int RSA = BM * 2 + 64;  // LDS row strides in bytes (16-bit values + 64 bytes pad)
-  constexpr int RSB = BN * 2 + 64;
+  constexpr bool F32 = sizeof(TI) == 4;
+  constexpr int ES = (int)sizeof(TI), VEC = 16 / ES;  // bytes per value, values per 16-byte chunk
+  constexpr int BM = 64 * FMW, BN = 128, BK = F32 ? 16 : 32, KP = 1;  // KP pixels per loader thread
+  constexpr int G = 256 / BK;       // loader threads per pixel
+  constexpr int QA = BM / (VEC * G);  // A chunks per loader thread and pixel
+  // LDS row strides in bytes: 16-bit rows + 64 bytes pad; FP32 rows + 128
+  constexpr int RSA = BM * ES + (F32 ? 128 : 64);
+  constexpr int RSB = BN * ES + (F32 ? 128 : 64);
   constexpr int STAGE = BK * (RSA + RSB);  // A rows, then B rows
-  __shared__ __attribute__((aligned(16))) char lds_all[KG][2][STAGE];  // 40 KB (FMW 2), 48 KB (FMW 3) per group
+  static_assert(QA * VEC * G == BM && 2 * VEC * G == BN, "loader covers the tile");
+  // 40 KB (FMW 2), 48 KB (FMW 3) per group
+  __shared__ __attribute__((aligned(16))) char lds_all[KG][2][STAGE];
   // tid / wave: within the group
   const int tid = threadIdx.x & 255, grp = threadIdx.x >> 8, lane = tid & 63, wave = tid >> 6;
   char(*lds)[STAGE] = lds_all[grp];
@@ -466,24 +478,24 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
   const int64_t len0 = half < send - sbeg ? half : send - sbeg;
   const int nk = (int)((len0 + BK - 1) / BK);
 
-  const uint32_t x_bytes = (uint32_t)((int64_t)p.B * p.H * p.W * p.D * p.Ci * 2);
-  const uint32_t g_bytes = (uint32_t)((int64_t)p.K * p.Co * 2);
+  const uint32_t x_bytes = (uint32_t)((int64_t)p.B * p.H * p.W * p.D * p.Ci * ES);
+  const uint32_t g_bytes = (uint32_t)((int64_t)p.K * p.Co * ES);
   const __amdgpu_buffer_rsrc_t xr = make_rsrc(p.a, x_bytes);
   const __amdgpu_buffer_rsrc_t gr = make_rsrc(p.b, g_bytes);
 
   // ---- loader assignment: per K-step the A tile (BK pixels x BM rows m)
-  // is 8 QA 16-byte chunks per pixel and the B tile (BK pixels x 128
-  // columns n) 16.  Thread t loads pixel kp = t / 8 of the step, and of it
-  // the A chunks c8 + 8 q (q < QA) and the B chunks c8, c8 + 8.
-  const int c8 = tid & 7, kp = tid >> 3;
-  // A chunk q covers rows m0 + 8 (c8 + 8 q) .. + 7, all in one tap (Ci % 8 == 0):
+  // is G QA 16-byte chunks per pixel and the B tile (BK pixels x 128
+  // columns n) 2 G.  Thread t loads pixel kp = t / G of the step, and of it
+  // the A chunks c8 + G q (q < QA) and the B chunks c8, c8 + G.
+  const int c8 = tid % G, kp = tid / G;
+  // A chunk q covers rows m0 + VEC (c8 + G q) .. + VEC - 1, all in one tap (Ci % VEC == 0):
   // tap window offset (tdh, tdw, tdd) and the chunk's element offset from
   // the pixel's window origin
   int tdh[QA], tdw[QA], tdd[QA], toff[QA];
   bool aok[QA];
 #pragma unroll
   for (int q = 0; q < QA; ++q) {
-    const int am = m0 + 8 * (c8 + 8 * q);
+    const int am = m0 + VEC * (c8 + G * q);
     aok[q] = am < M;
     const int tap = aok[q] ? am / p.Ci : 0;
     const int ci = aok[q] ? am - tap * p.Ci : 0;
@@ -495,7 +507,7 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
     tdd[q] = te * p.rd - p.pd;
     toff[q] = ((tdh[q] * p.W + tdw[q]) * p.D + tdd[q]) * p.Ci + ci;
   }
-  const int bn = n0 + 8 * c8;
+  const int bn = n0 + VEC * c8;
   const bool bok0 = bn < N, bok1 = bn + 64 < N;
 
   // Pixel positions, advanced by BK per K-step without division: the step is
@@ -507,7 +519,7 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
   int pk[KP], pld[KP], pjw[KP], pih[KP], pbase[KP], gbase[KP];
 #pragma unroll
   for (int h = 0; h < KP; ++h) {
-    int t = (int)kbeg + kp + 32 * h;
+    int t = (int)kbeg + kp + BK * h;
     pk[h] = t;
     const int l = t % p.Do; t /= p.Do;
     const int j = t % p.Wo; t /= p.Wo;
@@ -544,12 +556,12 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
         const int hh = pih[h] + tdh[q], ww = pjw[h] + tdw[q], dd = pld[h] + tdd[q];
         const bool ok = kok & aok[q] & ((unsigned)hh < (unsigned)p.H) & ((unsigned)ww < (unsigned)p.W) &
                         (D1 || ((unsigned)dd < (unsigned)p.D));
-        const uint32_t off = (uint32_t)(pbase[h] + toff[q]) * 2u;
+        const uint32_t off = (uint32_t)(pbase[h] + toff[q]) * (uint32_t)ES;
         rg[h][q] = bload(xr, ok ? off : OOB);
       }
-      const uint32_t goff = (uint32_t)gbase[h] * 2u;
+      const uint32_t goff = (uint32_t)gbase[h] * (uint32_t)ES;
       rg[h][QA] = bload(gr, (kok & bok0) ? goff : OOB);
-      rg[h][QA + 1] = bload(gr, (kok & bok1) ? goff + 128u : OOB);
+      rg[h][QA + 1] = bload(gr, (kok & bok1) ? goff + 64u * ES : OOB);
       // advance to the pixel BK further on
       pk[h] += BK;
       gbase[h] += gstep;
@@ -580,11 +592,11 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
   auto store = [&](const Stage &rg, int st) __attribute__((always_inline)) {
 #pragma unroll
     for (int h = 0; h < KP; ++h) {
-      const int row = kp + 32 * h;
+      const int row = kp + BK * h;
 #pragma unroll
-      for (int q = 0; q < QA; ++q) *(uint4 *)(&lds[st][row * RSA + (c8 + 8 * q) * 16]) = rg[h][q];
+      for (int q = 0; q < QA; ++q) *(uint4 *)(&lds[st][row * RSA + (c8 + G * q) * 16]) = rg[h][q];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) *(uint4 *)(&lds[st][BK * RSA + row * RSB + (c8 + 8 * q) * 16]) = rg[h][QA + q];
+      for (int q = 0; q < 2; ++q) *(uint4 *)(&lds[st][BK * RSA + row * RSB + (c8 + G * q) * 16]) = rg[h][QA + q];
     }
   };
 
@@ -609,6 +621,35 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
   auto kstep = [&](int ks, Stage &rn) __attribute__((always_inline)) {
     const int cur = ks & 1;
     const char *Ls = lds[cur];
+    if constexpr (F32) {
+      // pixel pair kk: lanes 0-31 read pixel 2 kk, lanes 32-63 pixel 2 kk + 1,
+      // each lane its fragment column lane % 32 -- the 32x32x2 MFMA's k = lane / 32
+      constexpr int KK = BK / 2;
+      float a[KK][FMW], b[KK][2];
+      const char *la = Ls + (lane >> 5) * RSA + 4 * (wm * 32 * FMW + (lane & 31));
+      const char *lb = Ls + BK * RSA + (lane >> 5) * RSB + 4 * (wn * 64 + (lane & 31));
+#pragma unroll
+      for (int kk = 0; kk < KK; ++kk) {
+#pragma unroll
+        for (int f = 0; f < FMW; ++f) a[kk][f] = *(const float *)(la + 2 * kk * RSA + 128 * f);
+#pragma unroll
+        for (int f = 0; f < 2; ++f) b[kk][f] = *(const float *)(lb + 2 * kk * RSB + 128 * f);
+      }
+#pragma unroll
+      for (int kk = 0; kk < KK; ++kk)
+#pragma unroll
+        for (int i = 0; i < FMW; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[kk][i], b[kk][j], acc[i][j], 0, 0, 0);
+      // after the last step this fills a stage nobody reads again
+      store(rn, cur ^ 1);
+      load(rn);
+      __builtin_amdgcn_sched_group_barrier(0x100, KK * (FMW + 2), 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, KK * 2 * FMW, 0);
+      __syncthreads();
+      return;
+    }
     // every 16-pixel quarter's fragments in their own registers, so all the
     // transposed reads can be in flight before the first MFMA
     constexpr int NQ = BK / 16;
@@ -1049,10 +1090,12 @@ void run_wgrad(const ConvParams &p, hipStream_t st) {
   const int bm = wgrad_tile_m(p.M);
   const int kg = p.split > 1 ? wgrad_kgroups() : 1;
   dim3 grid((unsigned)(((p.M + bm - 1) / bm) * ((p.N + 127) / 128) * p.split)), blk(256 * kg);
-  if (wgrad_halo_fits(p)) {
-    if (kg == 2) launch_k((conv_wgrad_halo_kernel<TI, TO, 2>), grid, blk, st, p);
-    else launch_k((conv_wgrad_halo_kernel<TI, TO, 1>), grid, blk, st, p);
-    return;
+  if constexpr (sizeof(TI) == 2) {
+    if (wgrad_halo_fits(p)) {
+      if (kg == 2) launch_k((conv_wgrad_halo_kernel<TI, TO, 2>), grid, blk, st, p);
+      else launch_k((conv_wgrad_halo_kernel<TI, TO, 1>), grid, blk, st, p);
+      return;
+    }
   }
   const bool d1 = p.D == 1 && p.Do == 1 && p.kd == 1 && p.pd == 0;
   if (kg == 2) {
@@ -1075,6 +1118,7 @@ template void run_wgrad<bf16_t, bf16_t>(const ConvParams &, hipStream_t);
 template void run_wgrad<bf16_t, float>(const ConvParams &, hipStream_t);
 template void run_wgrad<f16_t, f16_t>(const ConvParams &, hipStream_t);
 template void run_wgrad<f16_t, float>(const ConvParams &, hipStream_t);
+template void run_wgrad<float, float>(const ConvParams &, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Stride-1 2-D convolution with an LDS-staged input halo.

@@ -3,7 +3,7 @@ partials in the library's file-scope device scratch instead of an allocated
 workspace.  Through the C ABI: two calls in a row (the second smaller) give
 the same bits as owl_amd_conv_bwd_kernel with a caller workspace on the same
 inputs (same kernels, same split plan), in BF16 and FP32; the call creates no
-library buffer; a problem whose partials exceed the scratch is rejected; and
+library buffer; every split plan's partials fit the scratch; and
 a HIP graph that captures the call replays it with the same bits."""
 import ctypes
 
@@ -74,14 +74,20 @@ def test_noalloc_equals_workspace_entry_twice(dt):
         assert b.value == 0, b.value      # no library buffer was created
 
 
-@pytest.mark.gpu
-def test_noalloc_rejects_partials_above_the_scratch():
-    x, dy, geo = _problem([16, 56, 56, 256], [3, 3, 256, 256], [1, 1], torch.float32, 80)
-    need = _need(geo, torch.float32)
-    assert need > LIMIT, need
-    dw = torch.empty([3, 3, 256, 256], dtype=torch.float32, device=DEV)
-    assert _noalloc(geo, x, dy, dw) == _lib.ERR_ARG
-    assert b"scratch" in _lib.lib().owl_amd_last_error()
+def test_split_plans_fit_the_scratch():
+    """The split plan bounds the partials by one round of workgroups' tiles
+    (at most 256 FP32 tiles of 192 x 128 for the weight-gradient kernel, 512
+    FP64 tiles of 64 x 64 for the implicit GEMM), so every real problem's
+    partials fit the scratch and the entry point's size check (an ERR_ARG
+    naming the scratch) is a guard no plan reaches.  Host-side query only."""
+    import itertools
+    worst = {}
+    for dt in (torch.float32, torch.float64, torch.bfloat16, torch.float16):
+        for b, h, c, co, k, s in itertools.product([1, 8, 64, 256], [7, 28, 56, 112], [16, 64, 256, 512],
+                                                   [16, 128, 512], [1, 3, 5], [1, 2]):
+            geo = nd.conv_problem([b, h, h, c], [k, k, c, co], [s, s], None, SAME)
+            worst[dt] = max(worst.get(dt, 0), _need(geo, dt))
+    assert all(0 < v <= LIMIT for v in worst.values()), worst
 
 
 @pytest.mark.gpu
