@@ -235,7 +235,8 @@ int owl_amd_conv_bwd(const owl_amd_conv_desc *d, int in_t, int dx_t, int dw_t, c
  * holds until the process ends; the call itself allocates nothing (no
  * hipMalloc / hipFree, no library buffer).  A problem whose partials
  * (owl_amd_conv_bwd_kernel_workspace) exceed the scratch is rejected with
- * OWL_AMD_ERR_ARG.  Real kinds only (F32, F64, BF16, F16).  Every call on one
+ * OWL_AMD_ERR_ARG (the split plans keep every problem's partials at or
+ * below 25 MiB, so this is a guard).  Real kinds only (F32, F64, BF16, F16).  Every call on one
  * GPU shares that GPU's scratch: calls that overlap on one GPU (different
  * streams or host threads) race. */
 #define OWL_AMD_NOALLOC_SCRATCH_BYTES (32 * 1024 * 1024)
