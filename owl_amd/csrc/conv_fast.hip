@@ -1147,6 +1147,14 @@ struct HaloGeom {
   int B, TR;           // batch, output rows per tile
   int ms;              // MFMA fragment edge: 32 or 16
   int wCi, wCo;        // the kernel buffer's [kh][kw][wCi][wCo] sizes
+  // 3-D layers (stride 1): each "image" above is one (w, d) plane of an
+  // NDHWC tensor -- for fixed (b, h) the W x D pixels are contiguous -- so B
+  // counts B x HOA output planes and the kernel's h taps are KA outer
+  // blocks around the channel blocks: block a restages the halo from input
+  // plane i + a ra - pa of the same image (zeros outside [0, HA)) and uses
+  // kernel taps a kh kw + (c, e).  2-D: KA = HA = HOA = ra = 1, pa = 0.
+  int KA, ra, pa, HA, HOA;
+  int IB;              // input images (planes) in the input buffer
   // forward only: fused bias / activation and the pre-activation output
   const float *bias;
   void *pre;
@@ -1218,7 +1226,8 @@ template <> struct HaloMfma<f16_t, 16> {
 // 2 x 2 fragments (tiles of 256 pixels x 64 channels): the scaled FP8 MFMA
 // keeps its result apart from its accumulator input, and 2 x 4 fragments
 // then spilled about 400 bytes per lane.
-template <int PASS, typename TI, typename TO, int MS, int FM, int FN, int WGN, bool KROWS, bool F8 = false>
+template <int PASS, typename TI, typename TO, int MS, int FM, int FN, int WGN, bool KROWS, bool F8 = false,
+          bool D3 = false>
 __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI *__restrict__ in,
                                                            const TI *__restrict__ wt, TO *__restrict__ out) {
   static_assert(!F8 || (PASS == 0 && MS == 32 && KROWS && FM == 2 && FN == 2), "FP8: forward, 32-edge, k-rows kernel tile");
@@ -1283,7 +1292,11 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
   const int nt = bid % n_tiles;
   const int rt = (bid / n_tiles) % row_tiles;
-  const int b = bid / (n_tiles * row_tiles);
+  const int b = bid / (n_tiles * row_tiles);  // output image (plane) of the tile
+  // D3: the plane's image and index (2-D instances compile the outer-block
+  // logic out: KA = 1, the halo always from image b)
+  const int bimg = D3 ? b / g.HOA : b, bpl = D3 ? b - (b / g.HOA) * g.HOA : 0;
+  const int KA = D3 ? g.KA : 1;
   const int n0 = nt * BN, i0 = rt * g.TR;
   const int TRv = min(g.TR, g.Ho - i0);     // output rows in this tile
   const int P = TRv * g.Wo;                 // output pixels in this tile
@@ -1302,11 +1315,21 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     hoff[f] = SWZ ? ti * HC + tj : (ti * HC + tj) * PR + fkg * (MS == 32 ? 16 : 8);
   }
 
-  const uint32_t in_bytes = (uint32_t)((int64_t)g.B * g.Hi * g.Wi * g.C * 2);
+  const uint32_t in_bytes = (uint32_t)((int64_t)g.IB * g.Hi * g.Wi * g.C * 2);
   const __amdgpu_buffer_rsrc_t inr = make_rsrc(in, in_bytes);
-  const int T = g.kh * g.kw;
-  const __amdgpu_buffer_rsrc_t wr = make_rsrc(wt, (uint32_t)(T * g.N * g.C * 2));
+  const int T = g.kh * g.kw;  // taps of one outer block
+  const __amdgpu_buffer_rsrc_t wr = make_rsrc(wt, (uint32_t)(KA * T * g.N * g.C * 2));
   const int cblocks = g.C / BK;
+  // the input image of the halo for outer block a, and whether it exists
+  int himg = b;
+  bool hval = true;
+  auto set_plane = [&](int a) __attribute__((always_inline)) {
+    if constexpr (D3) {
+      const int ip = bpl + a * g.ra - g.pa;
+      hval = (unsigned)ip < (unsigned)g.HA;
+      himg = bimg * g.HA + (hval ? ip : 0);
+    }
+  };
   // 16-byte slot XOR of pixel / kernel row p (0 for 80-byte rows)
   auto swz = [](int p) __attribute__((always_inline)) -> int { return SWZ ? (p >> 1) & 3 : 0; };
 
@@ -1317,8 +1340,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     const int pix = q >> 2, ch = q & 3;
     const int r = pix / HC, c = pix - (pix / HC) * HC;
     const int hi = hr0 + r, wi = hc0 + c;
-    if (q < hq_total && (unsigned)hi < (unsigned)g.Hi && (unsigned)wi < (unsigned)g.Wi)
-      return (uint32_t)(((b * g.Hi + hi) * g.Wi + wi) * g.C + ch * 8) * 2u;
+    if ((!D3 || hval) && q < hq_total && (unsigned)hi < (unsigned)g.Hi && (unsigned)wi < (unsigned)g.Wi)
+      return (uint32_t)((((D3 ? himg : b) * g.Hi + hi) * g.Wi + wi) * g.C + ch * 8) * 2u;
     return OOB;
   };
   // This thread's chunks q = tid + 256 u sit 64 pixels apart: the pixel's
@@ -1335,8 +1358,9 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
     for (int u = 0; u < HQ; ++u) {
       const int hi = hr0 + r, wi = hc0 + c;
-      const bool ok = tid + u * 256 < hq_total && (unsigned)hi < (unsigned)g.Hi && (unsigned)wi < (unsigned)g.Wi;
-      const uint32_t so = (uint32_t)(((b * g.Hi + hi) * g.Wi + wi) * g.C + c0 + hch * 8) * 2u;
+      const bool ok = (!D3 || hval) && tid + u * 256 < hq_total && (unsigned)hi < (unsigned)g.Hi &&
+                      (unsigned)wi < (unsigned)g.Wi;
+      const uint32_t so = (uint32_t)((((D3 ? himg : b) * g.Hi + hi) * g.Wi + wi) * g.C + c0 + hch * 8) * 2u;
       hreg[u] = bload(inr, ok ? so : OOB);
       c += hs_c;
       const int cy = c >= HC;
@@ -1390,7 +1414,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
       }
     } else {
       // rows of k: [tap][n][k] with N rows of C values per tap
-      const int ft = PASS == 1 ? T - 1 - tap : tap;  // backward input flips the taps
+      const int ft = PASS == 1 ? KA * T - 1 - tap : tap;  // backward input flips the taps
 #pragma unroll
       for (int i = 0; i < BNV; ++i) {
         const int n = n0 + b_row + i * 64;
@@ -1452,26 +1476,40 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
       for (int r = 0; r < NR; ++r) acc[i][j][r] = 0.0f;
 
-  const int nk = cblocks * T;
-  // load cursor: (tap, channel block) of the next B tile to fetch.  Past the
-  // last step it fetches tile (0, 0) again, so every step issues the same
-  // loads and the waits on them stay exact.
-  int ltap = 0, lcb = 0;
+  const int nk = KA * cblocks * T;
+  // load cursor: (outer block, channel block, tap) of the next B tile to
+  // fetch.  Past the last step it fetches tile (0, 0, 0) again, so every
+  // step issues the same loads and the waits on them stay exact.
+  int ltap = 0, lcb = 0, la = 0;
   auto next_load = [&](TI(&rb)[BELEMS]) __attribute__((always_inline)) {
-    const bool in = lcb < cblocks;
-    load_b(rb, in ? ltap : 0, in ? lcb * BK : 0);
-    if (++ltap == T) {
-      ltap = 0;
-      ++lcb;
+    if constexpr (D3) {
+      const bool in = la < KA;
+      load_b(rb, in ? la * T + ltap : 0, in ? lcb * BK : 0);
+      if (++ltap == T) {
+        ltap = 0;
+        if (++lcb == cblocks) {
+          lcb = 0;
+          ++la;
+        }
+      }
+    } else {
+      const bool in = lcb < cblocks;
+      load_b(rb, in ? ltap : 0, in ? lcb * BK : 0);
+      if (++ltap == T) {
+        ltap = 0;
+        ++lcb;
+      }
     }
   };
   next_load(rb0);  // step 0
   next_load(rb1);  // step 1
+  set_plane(0);
   stage_halo(halo, 0);
   store_b(rb0, 0);
   next_load(rb0);  // step 2
   __syncthreads();
-  int cb = 0, tap = 0, ta = 0, tc = 0;  // current step: channel block, tap = (ta, tc)
+  // current step: block cb (outer block sa, channel block sc), tap = (ta, tc)
+  int cb = 0, tap = 0, ta = 0, tc = 0, sa = 0, sc = 0;
   // one K-step; rnext holds B(ks + 1) and is refilled with B(ks + 3)
   auto step = [&](int ks, TI(&rnext)[BELEMS]) __attribute__((always_inline)) {
     const int bcur = ks & 1;
@@ -1571,8 +1609,14 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     store_b(rnext, bcur ^ 1);
     next_load(rnext);
     if (more && ncb != cb) {
+      if constexpr (D3) {
+        if (++sc == cblocks) {
+          sc = 0;
+          set_plane(++sa);
+        }
+      }
       __syncthreads();  // every wave is done with this channel block's halo
-      stage_halo(halo, ncb * BK);
+      stage_halo(halo, (D3 ? sc : ncb) * BK);
     }
     __syncthreads();
     tap = ntap;
@@ -1806,19 +1850,42 @@ static bool halo_rows_fit(const HaloGeom &g, int tr) {
 template <int PASS>
 static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom &g, int force_ms = 0) {
   if (in_bytes != 2) return false;
-  if (p.D != 1 || p.kd != 1 || p.Do != 1) return false;
   if (p.sh != 1 || p.sw != 1) return false;
-  if (PASS == 0) {
-    g.Hi = p.H; g.Wi = p.W; g.C = p.Ci; g.Ho = p.Ho; g.Wo = p.Wo; g.N = p.Co;
-    g.ph = p.ph; g.pw = p.pw;
+  const bool d3 = p.D != 1 || p.kd != 1 || p.Do != 1;
+  if (!d3) {
+    if (PASS == 0) {
+      g.Hi = p.H; g.Wi = p.W; g.C = p.Ci; g.Ho = p.Ho; g.Wo = p.Wo; g.N = p.Co;
+      g.ph = p.ph; g.pw = p.pw;
+    } else {
+      g.Hi = p.Ho; g.Wi = p.Wo; g.C = p.Co; g.Ho = p.H; g.Wo = p.W; g.N = p.Ci;
+      g.ph = (p.kh - 1) * p.rh - p.ph;
+      g.pw = (p.kw - 1) * p.rw - p.pw;
+      if (g.ph < 0 || g.pw < 0) return false;
+    }
+    g.kh = p.kh; g.kw = p.kw; g.rh = p.rh; g.rw = p.rw;
+    g.KA = 1; g.ra = 1; g.pa = 0; g.HA = 1; g.HOA = 1;
+    g.B = p.B; g.IB = p.B;
   } else {
-    g.Hi = p.Ho; g.Wi = p.Wo; g.C = p.Co; g.Ho = p.H; g.Wo = p.W; g.N = p.Ci;
-    g.ph = (p.kh - 1) * p.rh - p.ph;
-    g.pw = (p.kw - 1) * p.rw - p.pw;
-    if (g.ph < 0 || g.pw < 0) return false;
+    // 3-D, unit strides: (w, d) planes, the h taps as outer blocks
+    if (p.sd != 1 || p.pool) return false;
+    if (PASS == 0) {
+      g.Hi = p.W; g.Wi = p.D; g.C = p.Ci; g.Ho = p.Wo; g.Wo = p.Do; g.N = p.Co;
+      g.ph = p.pw; g.pw = p.pd; g.pa = p.ph;
+      g.HA = p.H; g.HOA = p.Ho;
+    } else {
+      g.Hi = p.Wo; g.Wi = p.Do; g.C = p.Co; g.Ho = p.W; g.Wo = p.D; g.N = p.Ci;
+      g.ph = (p.kw - 1) * p.rw - p.pw;
+      g.pw = (p.kd - 1) * p.rd - p.pd;
+      g.pa = (p.kh - 1) * p.rh - p.ph;
+      if (g.ph < 0 || g.pw < 0 || g.pa < 0) return false;
+      g.HA = p.Ho; g.HOA = p.H;
+    }
+    g.kh = p.kw; g.kw = p.kd; g.rh = p.rw; g.rw = p.rd;
+    g.KA = p.kh; g.ra = p.rh;
+    g.B = p.B * g.HOA; g.IB = p.B * g.HA;
+    if ((int64_t)p.B * g.HOA >= (1LL << 31) || (int64_t)p.B * g.HA >= (1LL << 31)) return false;
   }
-  g.kh = p.kh; g.kw = p.kw; g.rh = p.rh; g.rw = p.rw;
-  g.B = p.B; g.wCi = p.Ci; g.wCo = p.Co;
+  g.wCi = p.Ci; g.wCo = p.Co;
   g.bias = PASS == 0 ? (const float *)p.bias : nullptr;
   g.pre = PASS == 0 ? p.pre : nullptr;
   g.act = PASS == 0 ? p.act : OWL_AMD_ACT_NONE;
@@ -1827,8 +1894,8 @@ static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom
   // the fused pooling reads the staged 16-bit tile
   if (g.pool && out_bytes != 2) return false;
   if (g.C % 32 != 0 || g.N % 8 != 0 || g.Wo > 256 || g.Ho < 1) return false;
-  if ((int64_t)g.B * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * out_bytes >= (1LL << 31) ||
-      (int64_t)g.kh * g.kw * g.C * g.N * 2 >= (1LL << 31))
+  if ((int64_t)g.IB * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * out_bytes >= (1LL << 31) ||
+      (int64_t)g.KA * g.kh * g.kw * g.C * g.N * 2 >= (1LL << 31))
     return false;  // 32-bit buffer offsets
   const char *fe = getenv("OWL_AMD_HALO_MS");
   const int forced = force_ms ? force_ms : fe ? atoi(fe) : 0;
@@ -1871,6 +1938,17 @@ bool run_halo(const ConvParams &p, hipStream_t st) {
   constexpr bool KROWS = PASS == 1;
   dim3 grid((unsigned)blocks), blk(256);
   KTimer kt(PASS, "conv_halo_kernel", st);
+  const bool d3 = p.D != 1 || p.kd != 1 || p.Do != 1;
+  if (d3) {
+    if (g.ms == 16) {
+      if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
+      else launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 2, KROWS, false, true>), grid, blk, st, g, in, w, out);
+    } else {
+      if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 32, 4, 2, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
+      else launch_k((conv_halo_kernel<PASS, TI, TO, 32, 2, 4, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
+    }
+    return true;
+  }
   if (g.ms == 16) {
     if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 1, KROWS>), grid, blk, st, g, in, w, out);
     else launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 2, KROWS>), grid, blk, st, g, in, w, out);
@@ -1884,6 +1962,7 @@ bool run_halo(const ConvParams &p, hipStream_t st) {
 // 16-bit units), 32-edge tiles of 256 pixels x 64 channels.
 static bool halo_geom_fp8(const ConvParams &p, int out_bytes, HaloGeom &g) {
   if (p.Ci % 64 != 0 || p.Co % 8 != 0) return false;
+  if (p.D != 1 || p.kd != 1 || p.Do != 1) return false;  // 2-D layers only
   ConvParams q = p;
   q.Ci = p.Ci / 2;
   q.bias = nullptr;

@@ -1,11 +1,9 @@
-"""Stride-1 2-D problems that take the LDS-halo kernel (16-bit storage,
-channels a multiple of 32, output channels a multiple of 8, rows up to 256
-pixels), checked against the FP64 oracle on the same rounded inputs.
+"""Stride-1 2-D and 3-D problems that take the LDS-halo kernel (16-bit
+storage, channels a multiple of 32, output channels a multiple of 8, rows up
+to 256 pixels), checked against the FP64 oracle on the same rounded inputs.
 
-The shapes cover both halo modes: halos of at most 352 pixels are double
-buffered (the next channel block loads while the current one runs); larger
-ones (112-pixel rows, 5x5 kernels) restage a single buffer between channel
-blocks.  Also 1x1 kernels (one tap per block), dilation, VALID padding and
+The 2-D shapes cover halos up to 456 pixels (112-pixel rows, 5x5 kernels),
+restaged between channel blocks.  Also 1x1 kernels (one tap per block), dilation, VALID padding and
 output channels that do not fill a tile (the forward reads the weights in
 place with transposing LDS loads; columns past the last channel read as
 zeros).  FP32 output: rel-L2 < 1e-5 (the MFMA accumulation is the only
@@ -15,7 +13,13 @@ Each case runs with the fragment edge the library picks and with each edge
 forced (OWL_AMD_HALO_MS): 32-edge tiles (256 / 512 pixels, 32x32x16 MFMAs)
 and 16-edge tiles (224 / 448 pixels, 16x16x32 MFMAs, 2 x 2 waves for
 128-wide channel slices), including short last row tiles, a partial second
-channel slice and the backward input with 128 input channels."""
+channel slice and the backward input with 128 input channels.
+
+3-D layers run the same kernel over (w, d) planes, with the kernel's h taps
+as outer blocks that restage the halo from another input plane (zeros past
+the image): SAME and VALID, dilation, even and one-wide kernel dims, a
+partial second channel slice; the library's kernel timing confirms that the
+halo kernel ran."""
 import os
 
 import numpy as np
@@ -98,4 +102,57 @@ def test_halo_vs_oracle(case, dtype, halo_edge):
     assert rel(y, ry) < 1e-2 and rel(dx, rdx) < 1e-2, (rel(y, ry), rel(dx, rdx))
     # the same call twice gives the same bits (no state left between calls)
     y2, dx2 = _run(x, w, g, rate, pad, td, None)
+    assert np.array_equal(y, y2) and np.array_equal(dx, dx2)
+
+
+# 3-D: (x shape, kernel shape, rate, padding); both channel counts multiples of 32,
+# so the forward and the backward input both take the halo kernel
+CASES3 = [
+    ([2, 6, 20, 18, 32], [3, 3, 3, 32, 64], 1, SAME),     # the conv3d stack's second layer, smaller
+    ([1, 5, 12, 14, 64], [3, 3, 3, 64, 32], 1, SAME),     # two channel blocks per outer tap
+    ([2, 7, 9, 30, 32], [3, 3, 3, 32, 64], 2, VALID),     # dilated, VALID
+    ([1, 4, 8, 40, 32], [2, 3, 5, 32, 32], 1, SAME),      # even kh, uneven kw / kd
+    ([2, 8, 16, 16, 32], [3, 3, 3, 32, 160], 1, SAME),    # two 128-wide slices, the second partial
+    ([1, 3, 5, 6, 32], [3, 1, 1, 32, 32], 1, SAME),       # only outer taps
+]
+
+
+def _run3(x, w, g, rate, pad, td, out_dtype):
+    from owl_amd import _lib
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV).to(td)
+    xt, wt, gt = dev(x), dev(w), dev(g)
+    s, r = [1, 1, 1], [rate, rate, rate]
+    torch.cuda.synchronize()
+    _lib.kernel_timing(True)
+    y = nd.dilated_conv3d(xt, wt, s, r, pad, out_dtype=out_dtype)
+    dx = nd.dilated_conv3d_backward_input(xt, wt, s, r, gt, out_dtype=out_dtype)
+    torch.cuda.synchronize()
+    names = (_lib.kernel_time(_lib.KT_FWD)[2], _lib.kernel_time(_lib.KT_BWD_INPUT)[2])
+    _lib.kernel_timing(False)
+    return y.double().cpu().numpy(), dx.double().cpu().numpy(), names
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(CASES3)))
+@pytest.mark.parametrize("dtype", ["bf16", "f16"])
+def test_halo3d_vs_oracle(case, dtype, halo_edge):
+    xs, ks, rate, pad = CASES3[case]
+    print(f"halo 3-D case {case} {dtype} edge {halo_edge} x{xs} w{ks}", flush=True)
+    td = torch.bfloat16 if dtype == "bf16" else torch.float16
+    rng = np.random.default_rng(300 + case)
+    x = rng.standard_normal(xs)
+    w = rng.standard_normal(ks) / np.sqrt(ks[0] * ks[1] * ks[2] * ks[3])
+    ys = on.dilated_conv2d(x, w, [1, 1, 1], [rate] * 3, pad).shape
+    g = rng.standard_normal(ys)
+    rnd = lambda a: torch.from_numpy(a).to(td).to(torch.float64).numpy()
+    xr, wr, gr = rnd(x), rnd(w), rnd(g)
+    ry = on.dilated_conv2d(xr, wr, [1, 1, 1], [rate] * 3, pad)
+    rdx = on.dilated_conv2d_backward_input(xr, wr, [1, 1, 1], [rate] * 3, gr)
+    y, dx, names = _run3(x, w, g, rate, pad, td, torch.float32)
+    assert names == ("conv_halo_kernel", "conv_halo_kernel"), names
+    assert rel(y, ry) < 1e-5, rel(y, ry)
+    assert rel(dx, rdx) < 1e-5, rel(dx, rdx)
+    y, dx, _ = _run3(x, w, g, rate, pad, td, None)
+    assert rel(y, ry) < 1e-2 and rel(dx, rdx) < 1e-2, (rel(y, ry), rel(dx, rdx))
+    y2, dx2, _ = _run3(x, w, g, rate, pad, td, None)
     assert np.array_equal(y, y2) and np.array_equal(dx, dx2)
