@@ -437,19 +437,23 @@ template <> __device__ __forceinline__ f32x16 mfma16<f16_t>(const v4s16 &a0, con
 // each with its own LDS stages, and add their sums through LDS at the end
 // (first half + second half): half as many FP32 partials for the reduce, at
 // one 512-thread workgroup per CU instead of two of 256.
-template <typename TI, typename TO, int FMW, bool D1, int KG>
+// FNW: 32-column fragments per wave along N (2: tiles 128 output channels
+// wide; 1: 64 wide, for layers with at most 64, which would leave half of a
+// 128-wide tile's MFMA work on padding).
+template <typename TI, typename TO, int FMW, bool D1, int KG, int FNW = 2>
 __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(ConvParams p) {
   // BK 64 (80 KB of LDS) ran slower than BK 32: one workgroup per CU
   constexpr bool F32 = sizeof(TI) == 4;
   constexpr int ES = (int)sizeof(TI), VEC = 16 / ES;  // bytes per value, values per 16-byte chunk
-  constexpr int BM = 64 * FMW, BN = 128, BK = F32 ? 16 : 32, KP = 1;  // KP pixels per loader thread
+  constexpr int BM = 64 * FMW, BN = 64 * FNW, BK = F32 ? 16 : 32, KP = 1;  // KP pixels per loader thread
   constexpr int G = 256 / BK;       // loader threads per pixel
   constexpr int QA = BM / (VEC * G);  // A chunks per loader thread and pixel
+  constexpr int QB = BN / (VEC * G);  // B chunks per loader thread and pixel
   // LDS row strides in bytes: 16-bit rows + 64 bytes pad; FP32 rows + 128
   constexpr int RSA = BM * ES + (F32 ? 128 : 64);
   constexpr int RSB = BN * ES + (F32 ? 128 : 64);
   constexpr int STAGE = BK * (RSA + RSB);  // A rows, then B rows
-  static_assert(QA * VEC * G == BM && 2 * VEC * G == BN, "loader covers the tile");
+  static_assert(QA * VEC * G == BM && QB * VEC * G == BN && QB == FNW, "loader covers the tile");
   // 40 KB (FMW 2), 48 KB (FMW 3) per group
   __shared__ __attribute__((aligned(16))) char lds_all[KG][2][STAGE];
   // tid / wave: within the group
@@ -508,7 +512,7 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
     toff[q] = ((tdh[q] * p.W + tdw[q]) * p.D + tdd[q]) * p.Ci + ci;
   }
   const int bn = n0 + VEC * c8;
-  const bool bok0 = bn < N, bok1 = bn + 64 < N;
+  const bool bok0 = bn < N, bok1 = FNW == 2 && bn + 64 < N;
 
   // Pixel positions, advanced by BK per K-step without division: the step is
   // split once into digits of the radix (Do, Wo, Ho); the scaled digits
@@ -544,7 +548,7 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
   const int gstep = BK * p.Co;
 
   // register ring: loads run two K-steps ahead of the LDS stage they fill
-  typedef uint4 Stage[KP][QA + 2];  // per pixel: A chunks 0 .. QA-1, then B chunks 0, 1
+  typedef uint4 Stage[KP][QA + QB];  // per pixel: A chunks 0 .. QA-1, then B chunks 0 .. QB-1
   Stage rs0, rs1;
   auto load = [&](Stage &rg) __attribute__((always_inline)) {
 #pragma unroll
@@ -561,7 +565,7 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
       }
       const uint32_t goff = (uint32_t)gbase[h] * (uint32_t)ES;
       rg[h][QA] = bload(gr, (kok & bok0) ? goff : OOB);
-      rg[h][QA + 1] = bload(gr, (kok & bok1) ? goff + 64u * ES : OOB);
+      if constexpr (QB == 2) rg[h][QA + 1] = bload(gr, (kok & bok1) ? goff + 64u * ES : OOB);
       // advance to the pixel BK further on
       pk[h] += BK;
       gbase[h] += gstep;
@@ -596,15 +600,15 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
 #pragma unroll
       for (int q = 0; q < QA; ++q) *(uint4 *)(&lds[st][row * RSA + (c8 + G * q) * 16]) = rg[h][q];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) *(uint4 *)(&lds[st][BK * RSA + row * RSB + (c8 + G * q) * 16]) = rg[h][QA + q];
+      for (int q = 0; q < QB; ++q) *(uint4 *)(&lds[st][BK * RSA + row * RSB + (c8 + G * q) * 16]) = rg[h][QA + q];
     }
   };
 
-  f32x16 acc[FMW][2];
+  f32x16 acc[FMW][FNW];
 #pragma unroll
   for (int i = 0; i < FMW; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < FNW; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
@@ -625,35 +629,35 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
       // pixel pair kk: lanes 0-31 read pixel 2 kk, lanes 32-63 pixel 2 kk + 1,
       // each lane its fragment column lane % 32 -- the 32x32x2 MFMA's k = lane / 32
       constexpr int KK = BK / 2;
-      float a[KK][FMW], b[KK][2];
+      float a[KK][FMW], b[KK][FNW];
       const char *la = Ls + (lane >> 5) * RSA + 4 * (wm * 32 * FMW + (lane & 31));
-      const char *lb = Ls + BK * RSA + (lane >> 5) * RSB + 4 * (wn * 64 + (lane & 31));
+      const char *lb = Ls + BK * RSA + (lane >> 5) * RSB + 4 * (wn * 32 * FNW + (lane & 31));
 #pragma unroll
       for (int kk = 0; kk < KK; ++kk) {
 #pragma unroll
         for (int f = 0; f < FMW; ++f) a[kk][f] = *(const float *)(la + 2 * kk * RSA + 128 * f);
 #pragma unroll
-        for (int f = 0; f < 2; ++f) b[kk][f] = *(const float *)(lb + 2 * kk * RSB + 128 * f);
+        for (int f = 0; f < FNW; ++f) b[kk][f] = *(const float *)(lb + 2 * kk * RSB + 128 * f);
       }
 #pragma unroll
       for (int kk = 0; kk < KK; ++kk)
 #pragma unroll
         for (int i = 0; i < FMW; ++i)
 #pragma unroll
-          for (int j = 0; j < 2; ++j)
+          for (int j = 0; j < FNW; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[kk][i], b[kk][j], acc[i][j], 0, 0, 0);
       // after the last step this fills a stage nobody reads again
       store(rn, cur ^ 1);
       load(rn);
-      __builtin_amdgcn_sched_group_barrier(0x100, KK * (FMW + 2), 0);
-      __builtin_amdgcn_sched_group_barrier(0x008, KK * 2 * FMW, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, KK * (FMW + FNW), 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, KK * FNW * FMW, 0);
       __syncthreads();
       return;
     }
     // every 16-pixel quarter's fragments in their own registers, so all the
     // transposed reads can be in flight before the first MFMA
     constexpr int NQ = BK / 16;
-    v4s16 a[NQ][FMW][2], b[NQ][2][2];  // [quarter][fragment][half]
+    v4s16 a[NQ][FMW][2], b[NQ][FNW][2];  // [quarter][fragment][half]
 #pragma unroll
     for (int s = 0; s < NQ; ++s)
 #pragma unroll
@@ -663,16 +667,16 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
           a[s][f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
               (lds_v4s16 *)(Ls + tr_a + (16 * s + 4 * hf) * RSA + 2 * (wm * 32 * FMW + f * 32)));
 #pragma unroll
-        for (int f = 0; f < 2; ++f)
+        for (int f = 0; f < FNW; ++f)
           b[s][f][hf] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-              (lds_v4s16 *)(Ls + tr_b + (16 * s + 4 * hf) * RSB + 2 * (wn * 64 + f * 32)));
+              (lds_v4s16 *)(Ls + tr_b + (16 * s + 4 * hf) * RSB + 2 * (wn * 32 * FNW + f * 32)));
       }
 #pragma unroll
     for (int s = 0; s < NQ; ++s)
 #pragma unroll
       for (int i = 0; i < FMW; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < FNW; ++j)
           acc[i][j] = mfma16<TI>(a[s][i][0], a[s][i][1], b[s][j][0], b[s][j][1], acc[i][j]);
     // after the last step this fills a stage nobody reads again
     store(rn, cur ^ 1);
@@ -680,8 +684,8 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
     // scheduling: the LDS reads as one group ahead of the MFMAs (an
     // MFMA-by-MFMA interleave with the loads and writes ran slower, and so
     // did the second quarter's reads spread between the first quarter's MFMAs)
-    __builtin_amdgcn_sched_group_barrier(0x100, 2 * NQ * (FMW + 2), 0);
-    __builtin_amdgcn_sched_group_barrier(0x008, 2 * NQ * FMW, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 2 * NQ * (FMW + FNW), 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, NQ * FMW * FNW, 0);
     __syncthreads();
   };
 
@@ -705,39 +709,57 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
     if (ks < nk) kstep(ks, rs1);
   }
 
-  // ---- KG 2: add the halves.  Group 0 keeps the j = 0 fragments and group
-  // 1 the j = 1 ones; each hands the other half over through LDS (the
-  // stages are free after the loop's last barrier), and every sum is
-  // (first half) + (second half).
+  // ---- KG 2: add the halves.  With two fragment columns group 0 keeps the
+  // j = 0 fragments and group 1 the j = 1 ones, each handing the other half
+  // over through LDS; with one, group 1 hands over everything and group 0
+  // stores the tile.  Every sum is (first half) + (second half).
   if constexpr (KG == 2) {
     __syncthreads();
-    float *X = (float *)&lds_all[0][0][0];  // two regions of FMW * 16 * 256 floats
+    // regions of FMW * 16 * 256 floats: one per group (two fragment
+    // columns), or one that group 1 writes and group 0 reads (one column)
+    float *X = (float *)&lds_all[0][0][0];
     constexpr int REG = FMW * 16 * 256;
-    static_assert(2 * REG * 4 <= KG * 2 * STAGE, "exchange fits the stages");
-    float *mine = X + grp * REG;
-    const float *theirs = X + (1 - grp) * REG;
-    if (grp == 0) {
+    static_assert(FNW * REG * 4 <= KG * 2 * STAGE, "exchange fits the stages");
+    float *mine = X + (FNW == 2 ? grp : 0) * REG;
+    const float *theirs = X + (FNW == 2 ? 1 - grp : 0) * REG;
+    if constexpr (FNW == 2) {
+      if (grp == 0) {
 #pragma unroll
-      for (int i = 0; i < FMW; ++i)
+        for (int i = 0; i < FMW; ++i)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) mine[(i * 16 + r) * 256 + tid] = acc[i][1][r];
+          for (int r = 0; r < 16; ++r) mine[(i * 16 + r) * 256 + tid] = acc[i][1][r];
+      } else {
+#pragma unroll
+        for (int i = 0; i < FMW; ++i)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) mine[(i * 16 + r) * 256 + tid] = acc[i][0][r];
+      }
+      __syncthreads();
+      if (grp == 0) {
+#pragma unroll
+        for (int i = 0; i < FMW; ++i)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) acc[i][0][r] = acc[i][0][r] + theirs[(i * 16 + r) * 256 + tid];
+      } else {
+#pragma unroll
+        for (int i = 0; i < FMW; ++i)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) acc[i][1][r] = theirs[(i * 16 + r) * 256 + tid] + acc[i][1][r];
+      }
     } else {
+      if (grp == 1) {
 #pragma unroll
-      for (int i = 0; i < FMW; ++i)
+        for (int i = 0; i < FMW; ++i)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) mine[(i * 16 + r) * 256 + tid] = acc[i][0][r];
-    }
-    __syncthreads();
-    if (grp == 0) {
+          for (int r = 0; r < 16; ++r) mine[(i * 16 + r) * 256 + tid] = acc[i][0][r];
+      }
+      __syncthreads();
+      if (grp == 0) {
 #pragma unroll
-      for (int i = 0; i < FMW; ++i)
+        for (int i = 0; i < FMW; ++i)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[i][0][r] = acc[i][0][r] + theirs[(i * 16 + r) * 256 + tid];
-    } else {
-#pragma unroll
-      for (int i = 0; i < FMW; ++i)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[i][1][r] = theirs[(i * 16 + r) * 256 + tid] + acc[i][1][r];
+          for (int r = 0; r < 16; ++r) acc[i][0][r] = acc[i][0][r] + theirs[(i * 16 + r) * 256 + tid];
+      }
     }
   }
 
@@ -750,9 +772,9 @@ __global__ __launch_bounds__(256 * KG, KG == 1 ? 2 : 1) void conv_wgrad_kernel(C
 #pragma unroll
   for (int i = 0; i < FMW; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      if (KG == 2 && j != grp) continue;  // the other group stores it
-      const int n = n0 + wn * 64 + j * 32 + frow;
+    for (int j = 0; j < FNW; ++j) {
+      if (KG == 2 && (FNW == 2 ? j != grp : grp != 0)) continue;  // the other group stores it
+      const int n = n0 + wn * 32 * FNW + j * 32 + frow;
       const int mrow = m0 + wm * 32 * FMW + i * 32 + 4 * fkg;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -1098,6 +1120,26 @@ void run_wgrad(const ConvParams &p, hipStream_t st) {
     }
   }
   const bool d1 = p.D == 1 && p.Do == 1 && p.kd == 1 && p.pd == 0;
+  if (p.N <= 64) {
+    // one 64-wide tile covers the output channels (the grid above counts
+    // ceil(N / 128) = 1 column of tiles either way)
+    if (kg == 2) {
+      if (bm == 192) {
+        if (d1) launch_k((conv_wgrad_kernel<TI, TO, 3, true, 2, 1>), grid, blk, st, p);
+        else launch_k((conv_wgrad_kernel<TI, TO, 3, false, 2, 1>), grid, blk, st, p);
+      } else {
+        if (d1) launch_k((conv_wgrad_kernel<TI, TO, 2, true, 2, 1>), grid, blk, st, p);
+        else launch_k((conv_wgrad_kernel<TI, TO, 2, false, 2, 1>), grid, blk, st, p);
+      }
+    } else if (bm == 192) {
+      if (d1) launch_k((conv_wgrad_kernel<TI, TO, 3, true, 1, 1>), grid, blk, st, p);
+      else launch_k((conv_wgrad_kernel<TI, TO, 3, false, 1, 1>), grid, blk, st, p);
+    } else {
+      if (d1) launch_k((conv_wgrad_kernel<TI, TO, 2, true, 1, 1>), grid, blk, st, p);
+      else launch_k((conv_wgrad_kernel<TI, TO, 2, false, 1, 1>), grid, blk, st, p);
+    }
+    return;
+  }
   if (kg == 2) {
     if (bm == 192) {
       if (d1) launch_k((conv_wgrad_kernel<TI, TO, 3, true, 2>), grid, blk, st, p);
