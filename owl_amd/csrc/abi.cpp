@@ -369,7 +369,6 @@ bool try_fast(const ConvParams &p, int in_t, int out_t, bool aligned, hipStream_
   if (!fast_path_enabled() || !aligned) return false;
   if (in_t != OWL_AMD_F32 && in_t != OWL_AMD_BF16 && in_t != OWL_AMD_F16) return false;
   const int C = PASS == 0 ? p.Ci : p.Co;
-  if (C % fast_bk((int)dtype_size(in_t)) != 0) return false;
   if (PASS == 1 && (p.sh != 1 || p.sw != 1 || p.sd != 1)) return false;
   // 32-bit byte offsets: operands and the GEMM rows must stay below 2 GiB
   const int64_t es = (int64_t)dtype_size(in_t);
@@ -379,7 +378,7 @@ bool try_fast(const ConvParams &p, int in_t, int out_t, bool aligned, hipStream_
   if (a_elems * es >= (1LL << 31) || (int64_t)p.kh * p.kw * p.kd * p.Ci * p.Co * es >= (1LL << 31) ||
       c_elems * (int64_t)dtype_size(out_t) >= (1LL << 31))
     return false;
-  // stride-1 2-D problems with 16-bit storage: the halo kernel
+  // stride-1 problems with 16-bit storage: the halo kernel (2-D, or 3-D as planes)
   if (halo_enabled() && halo_fits(p, PASS, (int)dtype_size(in_t), (int)dtype_size(out_t))) {
     bool done = false;
     if (in_t == OWL_AMD_BF16 && out_t == OWL_AMD_BF16) done = run_halo<PASS, bf16_t, bf16_t>(p, st);
@@ -388,6 +387,8 @@ bool try_fast(const ConvParams &p, int in_t, int out_t, bool aligned, hipStream_
     else if (in_t == OWL_AMD_F16 && out_t == OWL_AMD_F32) done = run_halo<PASS, f16_t, float>(p, st);
     if (done) return true;
   }
+  // the direct-A kernel takes whole K-steps of channels
+  if (C % fast_bk((int)dtype_size(in_t)) != 0) return false;
   const int shape = fast_shape_for(p.N);
   KTimer kt(PASS, "conv_fast_kernel", st);
   if (in_t == OWL_AMD_F32 && out_t == OWL_AMD_F32) run_fast<PASS, float, float>(p, shape, fast_pd(), st);

@@ -1268,8 +1268,12 @@ template <> struct HaloMfma<f16_t, 16> {
 // 2 x 2 fragments (tiles of 256 pixels x 64 channels): the scaled FP8 MFMA
 // keeps its result apart from its accumulator input, and 2 x 4 fragments
 // then spilled about 400 bytes per lane.
+// GEN: the general instance -- 3-D layers as (w, d) planes with outer h
+// blocks (see HaloGeom), and channel counts that are multiples of 8 but not
+// of 32, whose last channel block loads the chunks and kernel rows past the
+// last channel as zeros.  The other instances compile that logic out.
 template <int PASS, typename TI, typename TO, int MS, int FM, int FN, int WGN, bool KROWS, bool F8 = false,
-          bool D3 = false>
+          bool GEN = false>
 __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI *__restrict__ in,
                                                            const TI *__restrict__ wt, TO *__restrict__ out) {
   static_assert(!F8 || (PASS == 0 && MS == 32 && KROWS && FM == 2 && FN == 2), "FP8: forward, 32-edge, k-rows kernel tile");
@@ -1335,10 +1339,10 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   const int nt = bid % n_tiles;
   const int rt = (bid / n_tiles) % row_tiles;
   const int b = bid / (n_tiles * row_tiles);  // output image (plane) of the tile
-  // D3: the plane's image and index (2-D instances compile the outer-block
-  // logic out: KA = 1, the halo always from image b)
-  const int bimg = D3 ? b / g.HOA : b, bpl = D3 ? b - (b / g.HOA) * g.HOA : 0;
-  const int KA = D3 ? g.KA : 1;
+  // GEN: the plane's image and index (the other instances compile the
+  // outer-block logic out: KA = 1, the halo always from image b)
+  const int bimg = GEN ? b / g.HOA : b, bpl = GEN ? b - (b / g.HOA) * g.HOA : 0;
+  const int KA = GEN ? g.KA : 1;
   const int n0 = nt * BN, i0 = rt * g.TR;
   const int TRv = min(g.TR, g.Ho - i0);     // output rows in this tile
   const int P = TRv * g.Wo;                 // output pixels in this tile
@@ -1361,12 +1365,12 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   const __amdgpu_buffer_rsrc_t inr = make_rsrc(in, in_bytes);
   const int T = g.kh * g.kw;  // taps of one outer block
   const __amdgpu_buffer_rsrc_t wr = make_rsrc(wt, (uint32_t)(KA * T * g.N * g.C * 2));
-  const int cblocks = g.C / BK;
+  const int cblocks = GEN ? (g.C + BK - 1) / BK : g.C / BK;
   // the input image of the halo for outer block a, and whether it exists
   int himg = b;
   bool hval = true;
   auto set_plane = [&](int a) __attribute__((always_inline)) {
-    if constexpr (D3) {
+    if constexpr (GEN) {
       const int ip = bpl + a * g.ra - g.pa;
       hval = (unsigned)ip < (unsigned)g.HA;
       himg = bimg * g.HA + (hval ? ip : 0);
@@ -1382,8 +1386,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     const int pix = q >> 2, ch = q & 3;
     const int r = pix / HC, c = pix - (pix / HC) * HC;
     const int hi = hr0 + r, wi = hc0 + c;
-    if ((!D3 || hval) && q < hq_total && (unsigned)hi < (unsigned)g.Hi && (unsigned)wi < (unsigned)g.Wi)
-      return (uint32_t)((((D3 ? himg : b) * g.Hi + hi) * g.Wi + wi) * g.C + ch * 8) * 2u;
+    if ((!GEN || hval) && q < hq_total && (unsigned)hi < (unsigned)g.Hi && (unsigned)wi < (unsigned)g.Wi)
+      return (uint32_t)((((GEN ? himg : b) * g.Hi + hi) * g.Wi + wi) * g.C + ch * 8) * 2u;
     return OOB;
   };
   // This thread's chunks q = tid + 256 u sit 64 pixels apart: the pixel's
@@ -1400,9 +1404,9 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
     for (int u = 0; u < HQ; ++u) {
       const int hi = hr0 + r, wi = hc0 + c;
-      const bool ok = (!D3 || hval) && tid + u * 256 < hq_total && (unsigned)hi < (unsigned)g.Hi &&
-                      (unsigned)wi < (unsigned)g.Wi;
-      const uint32_t so = (uint32_t)((((D3 ? himg : b) * g.Hi + hi) * g.Wi + wi) * g.C + c0 + hch * 8) * 2u;
+      const bool ok = (!GEN || (hval && c0 + hch * 8 < g.C)) && tid + u * 256 < hq_total &&
+                      (unsigned)hi < (unsigned)g.Hi && (unsigned)wi < (unsigned)g.Wi;
+      const uint32_t so = (uint32_t)((((GEN ? himg : b) * g.Hi + hi) * g.Wi + wi) * g.C + c0 + hch * 8) * 2u;
       hreg[u] = bload(inr, ok ? so : OOB);
       c += hs_c;
       const int cy = c >= HC;
@@ -1429,7 +1433,9 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     halo_commit(hb);
     for (int q = HQ * 256 + tid; q < hq_total; q += 256) {
       const uint32_t so = chunk_src(q);
-      *(uint4 *)&hb[(q >> 2) * PR + ((q & 3) ^ swz(q >> 2)) * 8] = bload(inr, so == OOB ? OOB : so + (uint32_t)c0 * 2u);
+      const bool cok = !GEN || c0 + (q & 3) * 8 < g.C;
+      *(uint4 *)&hb[(q >> 2) * PR + ((q & 3) ^ swz(q >> 2)) * 8] =
+          bload(inr, so == OOB || !cok ? OOB : so + (uint32_t)c0 * 2u);
     }
   };
 
@@ -1450,7 +1456,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
       for (int i = 0; i < BNV; ++i) {
         const int k = c0 + nb_row + i * RPP;
-        const uint32_t wo = n < g.N ? (uint32_t)((tap * g.wCi + k) * g.wCo + n) * 2u : OOB;
+        const uint32_t wo = n < g.N && (!GEN || k < g.C) ? (uint32_t)((tap * g.wCi + k) * g.wCo + n) * 2u : OOB;
         const uint4 v = bload(wr, wo);
         __builtin_memcpy(&rb[i * 8], &v, 16);
       }
@@ -1460,7 +1466,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 #pragma unroll
       for (int i = 0; i < BNV; ++i) {
         const int n = n0 + b_row + i * 64;
-        const uint32_t wo = n < g.N ? (uint32_t)((ft * g.N + n) * g.C + c0 + b_kv) * 2u : OOB;
+        const uint32_t wo =
+            n < g.N && (!GEN || c0 + b_kv < g.C) ? (uint32_t)((ft * g.N + n) * g.C + c0 + b_kv) * 2u : OOB;
         const uint4 v = bload(wr, wo);
         __builtin_memcpy(&rb[i * 8], &v, 16);
       }
@@ -1524,7 +1531,7 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   // step issues the same loads and the waits on them stay exact.
   int ltap = 0, lcb = 0, la = 0;
   auto next_load = [&](TI(&rb)[BELEMS]) __attribute__((always_inline)) {
-    if constexpr (D3) {
+    if constexpr (GEN) {
       const bool in = la < KA;
       load_b(rb, in ? la * T + ltap : 0, in ? lcb * BK : 0);
       if (++ltap == T) {
@@ -1651,14 +1658,14 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
     store_b(rnext, bcur ^ 1);
     next_load(rnext);
     if (more && ncb != cb) {
-      if constexpr (D3) {
+      if constexpr (GEN) {
         if (++sc == cblocks) {
           sc = 0;
           set_plane(++sa);
         }
       }
       __syncthreads();  // every wave is done with this channel block's halo
-      stage_halo(halo, (D3 ? sc : ncb) * BK);
+      stage_halo(halo, (GEN ? sc : ncb) * BK);
     }
     __syncthreads();
     tap = ntap;
@@ -1935,7 +1942,8 @@ static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom
   g.sa = g.sb = nullptr;
   // the fused pooling reads the staged 16-bit tile
   if (g.pool && out_bytes != 2) return false;
-  if (g.C % 32 != 0 || g.N % 8 != 0 || g.Wo > 256 || g.Ho < 1) return false;
+  // channel counts that are not multiples of 32 run the general instance
+  if (g.C % 8 != 0 || g.N % 8 != 0 || g.Wo > 256 || g.Ho < 1) return false;
   if ((int64_t)g.IB * g.Hi * g.Wi * g.C * 2 >= (1LL << 31) || (int64_t)g.B * g.Ho * g.Wo * g.N * out_bytes >= (1LL << 31) ||
       (int64_t)g.KA * g.kh * g.kw * g.C * g.N * 2 >= (1LL << 31))
     return false;  // 32-bit buffer offsets
@@ -1980,8 +1988,8 @@ bool run_halo(const ConvParams &p, hipStream_t st) {
   constexpr bool KROWS = PASS == 1;
   dim3 grid((unsigned)blocks), blk(256);
   KTimer kt(PASS, "conv_halo_kernel", st);
-  const bool d3 = p.D != 1 || p.kd != 1 || p.Do != 1;
-  if (d3) {
+  const bool gen = p.D != 1 || p.kd != 1 || p.Do != 1 || g.C % 32 != 0;
+  if (gen) {
     if (g.ms == 16) {
       if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
       else launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 2, KROWS, false, true>), grid, blk, st, g, in, w, out);

@@ -1,6 +1,8 @@
 """Stride-1 2-D and 3-D problems that take the LDS-halo kernel (16-bit
-storage, channels a multiple of 32, output channels a multiple of 8, rows up
-to 256 pixels), checked against the FP64 oracle on the same rounded inputs.
+storage, channels a multiple of 8 -- those not a multiple of 32 on the
+general instance, whose last channel block reads zeros past the last
+channel --, output channels a multiple of 8, rows up to 256 pixels),
+checked against the FP64 oracle on the same rounded inputs.
 
 The 2-D shapes cover halos up to 456 pixels (112-pixel rows, 5x5 kernels),
 restaged between channel blocks.  Also 1x1 kernels (one tap per block), dilation, VALID padding and
@@ -46,6 +48,8 @@ CASES = [
     ([3, 13, 56, 64], [3, 3, 64, 128], 1, SAME),     # row tiles 4,4,4,1 (forward), 8,5 (backward input)
     ([2, 11, 30, 64], [3, 3, 64, 200], 1, SAME),     # two 128-wide slices, the second 72 wide
     ([4, 14, 14, 128], [3, 3, 128, 256], 1, SAME),   # backward input with 128 input channels
+    ([2, 10, 30, 16], [3, 3, 16, 32], 1, SAME),      # 16 channels: one zero-padded block (general instance)
+    ([1, 8, 20, 40], [3, 3, 40, 24], 1, SAME),       # 40 / 24 channels: a partial last block
 ]
 
 
@@ -105,7 +109,7 @@ def test_halo_vs_oracle(case, dtype, halo_edge):
     assert np.array_equal(y, y2) and np.array_equal(dx, dx2)
 
 
-# 3-D: (x shape, kernel shape, rate, padding); both channel counts multiples of 32,
+# 3-D: (x shape, kernel shape, rate, padding); both channel counts multiples of 8,
 # so the forward and the backward input both take the halo kernel
 CASES3 = [
     ([2, 6, 20, 18, 32], [3, 3, 3, 32, 64], 1, SAME),     # the conv3d stack's second layer, smaller
@@ -114,6 +118,8 @@ CASES3 = [
     ([1, 4, 8, 40, 32], [2, 3, 5, 32, 32], 1, SAME),      # even kh, uneven kw / kd
     ([2, 8, 16, 16, 32], [3, 3, 3, 32, 160], 1, SAME),    # two 128-wide slices, the second partial
     ([1, 3, 5, 6, 32], [3, 1, 1, 32, 32], 1, SAME),       # only outer taps
+    ([2, 6, 12, 10, 16], [3, 3, 3, 16, 32], 1, SAME),     # 16 input channels (the conv3d stack's first layer)
+    ([1, 5, 8, 9, 24], [3, 3, 3, 24, 40], 1, SAME),       # 24 / 40 channels: partial blocks
 ]
 
 
