@@ -32,9 +32,9 @@ SPINUP_S = 0.25  # untimed work before each timed region, after the W warmup ste
 # for a plain copy of known size (x2.0 and x1.0); the runs and their command
 # are in profiles/conv2d_vgg_layer_bf16_hbm_traffic.md
 TRAFFIC = {
-    ("conv_halo_kernel", "fwd"): 13238.6 * 2048 + 50187.6 * 1024,
-    ("conv_halo_kernel", "bwd_input"): 25831.6 * 2048 + 25088.0 * 1024,
-    ("conv_wgrad_kernel", "bwd_kernel"): 41098.6 * 2048 + 24516.5 * 1024,
+    ("conv_halo_kernel", "fwd"): 13242.9 * 2048 + 50186.2 * 1024,
+    ("conv_halo_kernel", "bwd_input"): 25808.6 * 2048 + 25088.1 * 1024,
+    ("conv_wgrad_kernel", "bwd_kernel"): 41098.6 * 2048 + 24521.4 * 1024,
 }
 
 
