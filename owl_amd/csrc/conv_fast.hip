@@ -1296,6 +1296,10 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   // slower (44.0 against 42.0 us), so only the KROWS instance swizzles.
   constexpr bool SWZ = MS == 16 && KROWS;
   constexpr int BK = 32, BN = WGN * FN * MS, PR = SWZ ? BK : BK + 8;
+  // kernel-tile columns staged per step: at least 64, so every thread moves
+  // whole 16-byte chunks; a 32-wide tile stages columns past N as zeros
+  // (layers with at most 32 output channels) and its MFMAs read the first 32
+  constexpr int BNL = BN < 64 ? 64 : BN;
   constexpr int LDR = PR;
   // The halo is one buffer of up to 2 HBUF pixels, restaged between channel
   // blocks behind a barrier with HQ 16-byte chunks per thread in flight at
@@ -1313,8 +1317,8 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   constexpr int HALO_BYTES = 2 * HBUF * PR * (int)sizeof(TI);
   // natural-layout kernel tile: BK rows of BN values, padded by 64 bytes so
   // the four rows one transposing read phase touches cover all 64 banks
-  constexpr int RSB = BN * (int)sizeof(TI) + 64;
-  constexpr int B_BYTES = KROWS ? BN * LDR * (int)sizeof(TI) : BK * RSB;
+  constexpr int RSB = BNL * (int)sizeof(TI) + 64;
+  constexpr int B_BYTES = KROWS ? BNL * LDR * (int)sizeof(TI) : BK * RSB;
   // 16-bit outputs are staged for the 16-byte store epilogue in the same
   // block, which is sized for whichever use is larger
   constexpr int K_BYTES = HALO_BYTES + 2 * B_BYTES;
@@ -1444,9 +1448,9 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   // Both use unconditional buffer loads (rows or columns past N read as
   // zeros), which keeps the number of loads in flight static, so the waits
   // stay precise across the loop.
-  constexpr int BELEMS = BK * BN / 256;
+  constexpr int BELEMS = BK * BNL / 256;
   constexpr int BNV = BELEMS / 8;              // 16-byte chunks per thread
-  constexpr int CPRB = BN / 8, RPP = 256 / CPRB;  // natural layout: chunks per row, rows per pass
+  constexpr int CPRB = BNL / 8, RPP = 256 / CPRB;  // natural layout: chunks per row, rows per pass
   const int nb_ch = tid % CPRB, nb_row = tid / CPRB;
   const int b_kv = (tid % 4) * 8, b_row = tid / 4;
   TI rb0[BELEMS], rb1[BELEMS];  // register ring: B tiles two steps ahead
@@ -1878,6 +1882,10 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
 // fragments (512 pixels) or of 7 x 4 16-edge ones (448 pixels).  Either way
 // 112 or 128 accumulators per lane.
 static int halo_cap(int N, int ms) { return N <= 64 ? (ms == 32 ? 512 : 448) : (ms == 32 ? 256 : 224); }
+// Channel slice of a halo tile: 32 for layers with at most 32 output
+// channels (waves of 7 x 2 16-edge or 4 x 1 32-edge fragments), 64 for at
+// most 64, else 128.
+static int halo_bn(int N) { return N <= 32 ? 32 : N <= 64 ? 64 : 128; }
 
 // Whole output rows per tile for a capacity (0 when one row does not fit),
 // and whether their input halo fits the LDS (704 pixels).
@@ -1980,7 +1988,7 @@ template <int PASS, typename TI, typename TO>
 bool run_halo(const ConvParams &p, hipStream_t st) {
   HaloGeom g;
   if (!halo_geom<PASS>(p, (int)sizeof(TI), (int)sizeof(TO), g)) return false;
-  const int BN = g.N <= 64 ? 64 : 128;
+  const int BN = halo_bn(g.N);
   const int64_t blocks = (int64_t)g.B * ((g.Ho + g.TR - 1) / g.TR) * ((g.N + BN - 1) / BN);
   const TI *in = (const TI *)p.a;
   const TI *w = (const TI *)p.b;
@@ -1991,19 +1999,23 @@ bool run_halo(const ConvParams &p, hipStream_t st) {
   const bool gen = p.D != 1 || p.kd != 1 || p.Do != 1 || g.C % 32 != 0;
   if (gen) {
     if (g.ms == 16) {
-      if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
+      if (BN == 32) launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 2, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
+      else if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
       else launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 2, KROWS, false, true>), grid, blk, st, g, in, w, out);
     } else {
-      if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 32, 4, 2, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
+      if (BN == 32) launch_k((conv_halo_kernel<PASS, TI, TO, 32, 4, 1, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
+      else if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 32, 4, 2, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
       else launch_k((conv_halo_kernel<PASS, TI, TO, 32, 2, 4, 1, KROWS, false, true>), grid, blk, st, g, in, w, out);
     }
     return true;
   }
   if (g.ms == 16) {
-    if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 1, KROWS>), grid, blk, st, g, in, w, out);
+    if (BN == 32) launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 2, 1, KROWS>), grid, blk, st, g, in, w, out);
+    else if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 1, KROWS>), grid, blk, st, g, in, w, out);
     else launch_k((conv_halo_kernel<PASS, TI, TO, 16, 7, 4, 2, KROWS>), grid, blk, st, g, in, w, out);
   } else {
-    if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 32, 4, 2, 1, KROWS>), grid, blk, st, g, in, w, out);
+    if (BN == 32) launch_k((conv_halo_kernel<PASS, TI, TO, 32, 4, 1, 1, KROWS>), grid, blk, st, g, in, w, out);
+    else if (BN == 64) launch_k((conv_halo_kernel<PASS, TI, TO, 32, 4, 2, 1, KROWS>), grid, blk, st, g, in, w, out);
     else launch_k((conv_halo_kernel<PASS, TI, TO, 32, 2, 4, 1, KROWS>), grid, blk, st, g, in, w, out);
   }
   return true;
