@@ -1188,6 +1188,7 @@ struct HaloGeom {
   int ph, pw;          // leading pads (of this forward view)
   int B, TR;           // batch, output rows per tile
   int ms;              // MFMA fragment edge: 32 or 16
+  int bn;              // output channels per tile (channel slice): 32, 64 or 128
   int wCi, wCo;        // the kernel buffer's [kh][kw][wCi][wCo] sizes
   // 3-D layers (stride 1): each "image" above is one (w, d) plane of an
   // NDHWC tensor -- for fixed (b, h) the W x D pixels are contiguous -- so B
@@ -1876,15 +1877,15 @@ __global__ __launch_bounds__(256, 2) void conv_halo_kernel(HaloGeom g, const TI 
   }
 }
 
-// Pixel capacity of a halo tile: 128-wide channel slices (N > 64) run 4
+// Pixel capacity of a halo tile of channel slice bn: 128-wide slices run 4
 // waves of 2 x 4 32-edge fragments (256 pixels) or 2 x 2 waves of 7 x 4
-// 16-edge ones (224 pixels); 64-wide slices 4 waves of 4 x 2 32-edge
-// fragments (512 pixels) or of 7 x 4 16-edge ones (448 pixels).  Either way
-// 112 or 128 accumulators per lane.
-static int halo_cap(int N, int ms) { return N <= 64 ? (ms == 32 ? 512 : 448) : (ms == 32 ? 256 : 224); }
-// Channel slice of a halo tile: 32 for layers with at most 32 output
-// channels (waves of 7 x 2 16-edge or 4 x 1 32-edge fragments), 64 for at
-// most 64, else 128.
+// 16-edge ones (224 pixels); 64- and 32-wide slices 4 waves along the
+// pixels, of 4 x 2 (4 x 1) 32-edge fragments (512 pixels) or of 7 x 4
+// (7 x 2) 16-edge ones (448 pixels).  At most 112 or 128 accumulators per
+// lane.
+static int halo_cap(int bn, int ms) { return bn <= 64 ? (ms == 32 ? 512 : 448) : (ms == 32 ? 256 : 224); }
+// Default channel slice for N output channels: 32 for at most 32 (instead
+// of a 64-wide tile half on padding), 64 for at most 64, else 128.
 static int halo_bn(int N) { return N <= 32 ? 32 : N <= 64 ? 64 : 128; }
 
 // Whole output rows per tile for a capacity (0 when one row does not fit),
@@ -1959,18 +1960,34 @@ static bool halo_geom(const ConvParams &p, int in_bytes, int out_bytes, HaloGeom
   const int forced = force_ms ? force_ms : fe ? atoi(fe) : 0;
   g.ms = 0;
   double best = -1.0;
-  for (int ms : {32, 16}) {
-    const int cap = halo_cap(g.N, ms);
-    int tr = halo_rows(g, cap);
-    // fused pooling: tiles start at even rows (whole 2x2 windows)
-    if (g.pool && tr < g.Ho) tr &= ~1;
-    if (!halo_rows_fit(g, tr)) continue;
-    const double use = (double)g.Ho * g.Wo / ((double)((g.Ho + tr - 1) / tr) * cap);
-    const bool take = forced == 16 || forced == 32 ? ms == forced || g.ms == 0 : use > best + 1e-9;
-    if (take) {
-      g.ms = ms;
-      g.TR = tr;
-      best = use;
+  // Candidates (slice, edge), scored by the share of the tiles' MFMA work
+  // on real pixels and channels; earlier candidates win ties.  The
+  // backward input also tries 64-wide slices for N > 64, first: where their
+  // 448-pixel tiles fill the rows as well as the 224-pixel ones they ran 2-3%
+  // faster (BF16 backward input, 28- and 56-pixel rows, N 128 and 256; same
+  // bits); on the forward they ran 0-7% slower.
+  const int bn0 = halo_bn(g.N);
+  const int nbn = PASS == 1 && g.N > 64 ? 2 : 1;
+  const int bns[2] = {nbn == 2 ? 64 : bn0, bn0};
+  for (int bi = 0; bi < nbn; ++bi) {
+    const int bn = bns[bi];
+    const double ch_use = (double)g.N / ((double)((g.N + bn - 1) / bn) * bn);
+    for (int ms : {32, 16}) {
+      const int cap = halo_cap(bn, ms);
+      int tr = halo_rows(g, cap);
+      // fused pooling: tiles start at even rows (whole 2x2 windows)
+      if (g.pool && tr < g.Ho) tr &= ~1;
+      if (!halo_rows_fit(g, tr)) continue;
+      const double use = ch_use * (double)g.Ho * g.Wo / ((double)((g.Ho + tr - 1) / tr) * cap);
+      // a forced edge (tests, tuning) ranks first, then the default slice;
+      // an edge that does not fit falls back to one that does
+      const double score = forced == 16 || forced == 32 ? 2.0 * (ms == forced) + (bn == bn0) : use;
+      if (score > best + 1e-9) {
+        g.ms = ms;
+        g.bn = bn;
+        g.TR = tr;
+        best = score;
+      }
     }
   }
   return g.ms != 0;
@@ -1988,7 +2005,7 @@ template <int PASS, typename TI, typename TO>
 bool run_halo(const ConvParams &p, hipStream_t st) {
   HaloGeom g;
   if (!halo_geom<PASS>(p, (int)sizeof(TI), (int)sizeof(TO), g)) return false;
-  const int BN = halo_bn(g.N);
+  const int BN = g.bn;
   const int64_t blocks = (int64_t)g.B * ((g.Ho + g.TR - 1) / g.TR) * ((g.N + BN - 1) / BN);
   const TI *in = (const TI *)p.a;
   const TI *w = (const TI *)p.b;

@@ -15,7 +15,11 @@ Each case runs with the fragment edge the library picks and with each edge
 forced (OWL_AMD_HALO_MS): 32-edge tiles (256 / 512 pixels, 32x32x16 MFMAs)
 and 16-edge tiles (224 / 448 pixels, 16x16x32 MFMAs, 2 x 2 waves for
 128-wide channel slices), including short last row tiles, a partial second
-channel slice and the backward input with 128 input channels.
+channel slice and the backward input with 128 input channels.  With the
+edge left to the library, the backward input of layers with more than 64
+input channels takes 64-wide slices where their 448-pixel tiles fill the
+rows as well as 224-pixel ones do (the last two 2-D cases before the
+3-D ones, and the last 3-D case); a forced edge keeps the default slice.
 
 3-D layers run the same kernel over (w, d) planes, with the kernel's h taps
 as outer blocks that restage the halo from another input plane (zeros past
@@ -50,6 +54,8 @@ CASES = [
     ([4, 14, 14, 128], [3, 3, 128, 256], 1, SAME),   # backward input with 128 input channels
     ([2, 10, 30, 16], [3, 3, 16, 32], 1, SAME),      # 16 channels: one zero-padded block (general instance)
     ([1, 8, 20, 40], [3, 3, 40, 24], 1, SAME),       # 40 / 24 channels: a partial last block
+    ([2, 16, 56, 128], [3, 3, 128, 64], 1, SAME),    # backward input (N = 128) on two 64-wide slices
+    ([1, 32, 28, 96], [3, 3, 96, 32], 1, SAME),      # backward input (N = 96) on 64-wide slices, the second half used
 ]
 
 
@@ -120,6 +126,7 @@ CASES3 = [
     ([1, 3, 5, 6, 32], [3, 1, 1, 32, 32], 1, SAME),       # only outer taps
     ([2, 6, 12, 10, 16], [3, 3, 3, 16, 32], 1, SAME),     # 16 input channels (the conv3d stack's first layer)
     ([1, 5, 8, 9, 24], [3, 3, 3, 24, 40], 1, SAME),       # 24 / 40 channels: partial blocks
+    ([1, 4, 16, 28, 96], [3, 3, 3, 96, 32], 1, SAME),     # backward input (N = 96) on 64-wide slices
 ]
 
 
