@@ -428,13 +428,14 @@ void plan_splitk(const ConvParams &p, int in_t, int *split, int64_t *kps) {
   // the weight-gradient kernel's tiles (16-bit and FP32 storage), else the
   // implicit GEMM's
   const bool w16 = in_t == OWL_AMD_BF16 || in_t == OWL_AMD_F16 || in_t == OWL_AMD_F32;
-  const int bm = w16 ? wgrad_tile_m(p.M) : tile_m(in_t), bn = w16 ? 128 : tile_m(in_t), bk = tile_k(in_t);
+  const int es = in_t == OWL_AMD_F32 ? 4 : 2;
+  const int bm = w16 ? wgrad_tile_m(p.M, es) : tile_m(in_t), bn = w16 ? 128 : tile_m(in_t), bk = tile_k(in_t);
   int64_t tiles = ((p.M + bm - 1) / bm) * (int64_t)((p.N + bn - 1) / bn);
   static const int env_target = env_int("OWL_AMD_SPLIT_TARGET", 0);  // workgroups to aim for (tuning)
   // three 128-row wgrad workgroups fit a CU, two of the 192-row ones; with
   // two wave groups per workgroup (each summing half a split) one 512-thread
   // workgroup per CU, 256 of them, whole ones only
-  const int kg = w16 ? wgrad_kgroups() : 1;
+  const int kg = w16 ? wgrad_kgroups(p, es) : 1;
   const int target = env_target > 0 ? env_target : (kg == 2 ? 256 : (w16 && bm == 128 ? 768 : 512));
   int64_t want = kg == 2 ? target / tiles : (target + tiles - 1) / tiles;
   int64_t max_split = (p.K + 16 * kg * bk - 1) / (16 * kg * bk);

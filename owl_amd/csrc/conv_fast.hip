@@ -1081,36 +1081,42 @@ bool wgrad_halo_fits(const ConvParams &p) {
     return !(e && atoi(e) == 0);
   }();
   return on && p.D == 1 && p.Do == 1 && p.kd == 1 && p.pd == 0 && p.kw == 3 && p.sw == 1 && p.rw == 1 &&
-         p.Ci % 64 == 0 && p.Ci >= 128 && p.Wo >= 32 && p.K % 32 == 0 && wgrad_tile_m(p.M) == 192 &&
+         p.Ci % 64 == 0 && p.Ci >= 128 && p.Wo >= 32 && p.K % 32 == 0 && wgrad_tile_m(p.M, 2) == 192 &&
          p.Co % 8 == 0 &&
          (p.split <= 1 || p.k_per_split % 64 == 0);
 }
 
-// Rows (tap, ci) per weight-gradient tile: 192 when that pads M no more than
-// 128 does (fewer LDS reads per MFMA), else 128.  OWL_AMD_WGRAD_BM = 128 or
-// 192 forces one (tuning).
-int wgrad_tile_m(int64_t M) {
+// Rows (tap, ci) per weight-gradient tile: 192 (fewer LDS reads per MFMA)
+// when that pads M no more than 128 does, for 16-bit storage up to 10% more
+// (conv3d 32 -> 64 channels, M = 864: 960 rows at 54.6 us against 896 at
+// 60.3 us), else 128.  OWL_AMD_WGRAD_BM = 128 or 192 forces one (tuning).
+int wgrad_tile_m(int64_t M, int es) {
   static const int forced = [] {
     const char *e = getenv("OWL_AMD_WGRAD_BM");
     return e ? atoi(e) : 0;
   }();
   if (forced == 128 || forced == 192) return forced;
-  return (M + 191) / 192 * 192 <= (M + 127) / 128 * 128 ? 192 : 128;
+  const int64_t p192 = (M + 191) / 192 * 192, p128 = (M + 127) / 128 * 128;
+  return (es == 2 ? p192 * 10 <= p128 * 11 : p192 <= p128) ? 192 : 128;
 }
 
-// Groups of four waves per weight-gradient workgroup: 2 (default), or 1 with
-// OWL_AMD_WGRAD_KG=1 (two 256-thread workgroups per CU, each summing a whole
-// split).  Read on every call; the split-K plan, and so the workspace size,
-// depends on it.
-int wgrad_kgroups() {
+// Groups of four waves per weight-gradient workgroup.  2: one 512-thread
+// workgroup per CU whose groups sum the two halves of a split (half the
+// partials).  1 for 16-bit tiles of 128 rows x 64 columns (N <= 64, about 120
+// VGPRs): three 256-thread workgroups per CU then hide more latency than the
+// halved partials save (conv3d 16 -> 32 channels: 30.5 + 5.4 us for the
+// reduce against 35.3 + 4.1).  OWL_AMD_WGRAD_KG = 1 or 2 forces one.  Read
+// on every call; the split-K plan, and so the workspace size, depends on it.
+int wgrad_kgroups(const ConvParams &p, int es) {
   const char *e = getenv("OWL_AMD_WGRAD_KG");
-  return e && atoi(e) == 1 ? 1 : 2;
+  if (e && (atoi(e) == 1 || atoi(e) == 2)) return atoi(e);
+  return es == 2 && p.N <= 64 && wgrad_tile_m(p.M, es) == 128 ? 1 : 2;
 }
 
 template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st) {
-  const int bm = wgrad_tile_m(p.M);
-  const int kg = p.split > 1 ? wgrad_kgroups() : 1;
+  const int bm = wgrad_tile_m(p.M, (int)sizeof(TI));
+  const int kg = p.split > 1 ? wgrad_kgroups(p, (int)sizeof(TI)) : 1;
   dim3 grid((unsigned)(((p.M + bm - 1) / bm) * ((p.N + 127) / 128) * p.split)), blk(256 * kg);
   if constexpr (sizeof(TI) == 2) {
     if (wgrad_halo_fits(p)) {

@@ -75,12 +75,15 @@ bool halo_fits_fp8(const ConvParams &p, int out_bytes);
 template <typename TO>
 bool run_halo_fp8(const ConvParams &p, const void *wp, hipStream_t st);
 void run_fp8_pack_krows(const unsigned char *w, unsigned char *wp, int64_t T, int Ci, int Co, hipStream_t st);
-// conv_fast.hip: backward kernel for 16-bit storage via transposing LDS reads
-// (tile 128 x 128 x 32 pixels; p.split / p.k_per_split as for run_igemm<2>).
-int wgrad_tile_m(int64_t M);
-// Groups of four waves per weight-gradient workgroup (2, or 1 with
-// OWL_AMD_WGRAD_KG=1): with 2 each group sums half of a split's pixels.
-int wgrad_kgroups();
+// conv_fast.hip: backward kernel for 16-bit and FP32 storage (es = bytes per
+// value) via LDS-staged [pixel][channel] rows (tiles of 128 or 192 rows
+// (tap, ci) x 128 or 64 output channels x one K-step of pixels; p.split /
+// p.k_per_split as for run_igemm<2>).
+int wgrad_tile_m(int64_t M, int es);
+// Groups of four waves per weight-gradient workgroup (2: each group sums
+// half of a split's pixels; 1 for light 16-bit tiles; OWL_AMD_WGRAD_KG = 1 or
+// 2 forces one).
+int wgrad_kgroups(const ConvParams &p, int es);
 template <typename TI, typename TO>
 void run_wgrad(const ConvParams &p, hipStream_t st);
 // True when run_wgrad takes the halo form (conv_wgrad_halo_kernel).
