@@ -108,7 +108,7 @@ enum {
   OWL_AMD_KT_FWD = 0,        /* main kernel of owl_amd_conv_fwd */
   OWL_AMD_KT_BWD_INPUT = 1,  /* main kernel of owl_amd_conv_bwd_input */
   OWL_AMD_KT_BWD_KERNEL = 2, /* main kernel of owl_amd_conv_bwd_kernel */
-  OWL_AMD_KT_AUX = 3,        /* split-K reduction */
+  OWL_AMD_KT_AUX = 3,        /* split-K reduction and other auxiliary kernels */
   OWL_AMD_KT_COUNT = 4
 };
 int owl_amd_kernel_timing(int enable);

@@ -111,6 +111,10 @@ void run_maxpool_relu_bwd(int64_t B, int H, int W, int C, const T *y, const T *p
 template <typename T>
 void run_upsample(bool backward, const T *src, T *dst, int64_t B, int H, int W, int C, int Ho, int Wo, int sh,
                   int sw, hipStream_t st);
+// resample.hip: dst[b, h, w, d] = src[b, h / sh, w / sw, d / sd] on the
+// strided grid, zero elsewhere (rows of row_bytes bytes, a multiple of 2).
+void run_stride_scatter(const void *src, void *dst, int64_t B, int Hs, int Ws, int Ds, int H, int W, int D, int sh,
+                        int sw, int sd, int64_t row_bytes, hipStream_t st);
 
 // complex.hip: complex kernels as real ones (see there).  pattern 0 expands
 // the forward's kernel, 1 the backward input's; fold turns the real weight

@@ -99,6 +99,76 @@ void run_upsample(bool backward, const T *src, T *dst, int64_t B, int H, int W, 
 
 template void run_upsample<float>(bool, const float *, float *, int64_t, int, int, int, int, int, int, int,
                                   hipStream_t);
+// Strided scatter of rows of row_bytes bytes: dst[b, h, w, d] = src[b, h / sh,
+// w / sw, d / sd] where h, w and d are multiples of the strides (and the
+// source cell exists), zero elsewhere.  This is the backward input of a 1x1
+// convolution with strides and no padding, whose dx is dy . w^T on the
+// strided grid and zero off it, built from the stride-1 backward input of
+// the subsampled grid.  A workgroup writes one dst row (b, h) of W x D
+// pixels, each lane whole U-sized units: with upr >= 64 units per pixel the
+// lanes of a wave stride one pixel, else each wave covers 64 / upr pixels
+// per step.  The row's grid test is made once; the (w, d) test once per
+// pixel and lane (a per-unit index decomposition by division cost more time
+// than the memory traffic).
+template <typename U>
+__global__ __launch_bounds__(256) void stride_scatter_kernel(const U *__restrict__ src, U *__restrict__ dst, int H,
+                                                             int W, int D, int Hs, int Ws, int Ds, int sh, int sw,
+                                                             int sd, int upr) {
+  const int bh = blockIdx.x;  // b * H + h
+  const int b = bh / H, h = bh - b * H;
+  const int hs = h / sh;
+  const bool hon = h == hs * sh && hs < Hs;
+  const int npix = W * D;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ppw = upr >= 64 ? 1 : 64 / upr;  // pixels per wave step
+  const int sp = upr >= 64 ? 0 : lane / upr;
+  const int u0 = upr >= 64 ? lane : lane - sp * upr;
+  const bool lane_ok = upr >= 64 || sp < ppw;
+  U *drow = dst + (int64_t)bh * npix * upr;
+  const U *srow = src + ((int64_t)b * Hs + (hon ? hs : 0)) * Ws * Ds * upr;
+  for (int p0 = wv * ppw; p0 < npix; p0 += 4 * ppw) {
+    const int pix = p0 + sp;
+    if (!lane_ok || pix >= npix) continue;
+    int w = pix, d = 0;
+    if (D != 1) {
+      w = pix / D;
+      d = pix - w * D;
+    }
+    const int ws = w / sw, ds = sd == 1 ? d : d / sd;
+    const bool on = hon && w == ws * sw && d == ds * sd && ws < Ws && ds < Ds;
+    for (int u = u0; u < upr; u += 64) {
+      U v;
+      __builtin_memset(&v, 0, sizeof(U));
+      if (on) v = srow[((int64_t)ws * Ds + ds) * upr + u];
+      drow[(int64_t)pix * upr + u] = v;
+    }
+  }
+}
+
+template <typename U>
+static void launch_scatter(const void *src, void *dst, int64_t B, int Hs, int Ws, int Ds, int H, int W, int D, int sh,
+                           int sw, int sd, int upr, hipStream_t st) {
+  if (B * H == 0) return;
+  launch_k((stride_scatter_kernel<U>), dim3((unsigned)(B * H)), dim3(256), st, (const U *)src, (U *)dst, H, W, D, Hs,
+           Ws, Ds, sh, sw, sd, upr);
+}
+
+void run_stride_scatter(const void *src, void *dst, int64_t B, int Hs, int Ws, int Ds, int H, int W, int D, int sh,
+                        int sw, int sd, int64_t row_bytes, hipStream_t st) {
+  const uintptr_t al = (uintptr_t)src | (uintptr_t)dst;
+  // the widest unit that divides a row and both addresses
+  for (int ub : {16, 8, 4}) {
+    if (row_bytes % ub == 0 && al % ub == 0) {
+      const int upr = (int)(row_bytes / ub);
+      if (ub == 16) launch_scatter<uint4>(src, dst, B, Hs, Ws, Ds, H, W, D, sh, sw, sd, upr, st);
+      else if (ub == 8) launch_scatter<uint2>(src, dst, B, Hs, Ws, Ds, H, W, D, sh, sw, sd, upr, st);
+      else launch_scatter<uint32_t>(src, dst, B, Hs, Ws, Ds, H, W, D, sh, sw, sd, upr, st);
+      return;
+    }
+  }
+  launch_scatter<uint16_t>(src, dst, B, Hs, Ws, Ds, H, W, D, sh, sw, sd, (int)(row_bytes / 2), st);
+}
+
 template void run_upsample<double>(bool, const double *, double *, int64_t, int, int, int, int, int, int, int,
                                    hipStream_t);
 template void run_upsample<bf16_t>(bool, const bf16_t *, bf16_t *, int64_t, int, int, int, int, int, int, int,
