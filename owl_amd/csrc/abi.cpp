@@ -631,6 +631,13 @@ namespace {
 // GEMM over every dx pixel (cross-checks, tuning); read on every call.
 bool strided_1x1_enabled() { return env_int("OWL_AMD_STRIDED_1X1", 1) != 0; }
 
+// The backward input takes the strided 1x1 path below, whose subsampled
+// grid lives in a per-device library buffer.
+bool strided_1x1_bwd_input(const owl_amd_conv_desc *d) {
+  return d->kh == 1 && d->kw == 1 && d->kd == 1 && (d->sh > 1 || d->sw > 1 || d->sd > 1) && d->ph == 0 &&
+         d->pw == 0 && d->pd == 0 && strided_1x1_enabled();
+}
+
 // Backward input of a 1x1 convolution with strides and no padding: dx is
 // dy . w^T on the strided grid (h = i sh, w = j sw, d = l sd) and zero off
 // it.  The stride-1 backward input of the subsampled problem writes the
@@ -683,8 +690,7 @@ extern "C" int owl_amd_conv_bwd_input(const owl_amd_conv_desc *d, int in_t, int 
   p.K = d->kh * d->kw * d->kd * d->Co;
   if (p.M == 0 || p.N == 0) return OWL_AMD_OK;
   if (!w || !dy || !dx) ARG_FAIL("null pointer");
-  if (d->kh == 1 && d->kw == 1 && d->kd == 1 && (d->sh > 1 || d->sw > 1 || d->sd > 1) && d->ph == 0 &&
-      d->pw == 0 && d->pd == 0 && d->Co > 0 && dtype_size(in_t) && dtype_size(out_t) && strided_1x1_enabled())
+  if (strided_1x1_bwd_input(d) && d->Co > 0 && dtype_size(in_t) && dtype_size(out_t))
     return bwd_input_strided_1x1(d, in_t, out_t, w, dy, dx, stream);
   p.a = dy; p.b = w; p.c = dx; p.ldc = p.Ci;
   const int v = vec_of(in_t);
@@ -1731,21 +1737,24 @@ int conv_dropin(ConvPass pass, T *input, T *kernel, T *output, int64_t B, int64_
   const size_t nw = (size_t)(kh * kw * kd * Ci * Co) * sizeof(T);
   const size_t ny = (size_t)(B * Ho * Wo * Do * Co) * sizeof(T);
   const int t = dt<T>();
-  const bool cplx = is_complex(t);  // expanded kernels in one per-device buffer: whole call locked
+  // complex kernels (expanded into one per-device buffer) and strided 1x1
+  // backward inputs (their subsampled grid in one per-device buffer): the
+  // whole call locked, not captured
+  const bool whole = is_complex(t) || (pass == BWD_IN && strided_1x1_bwd_input(&d));
   const std::string key = dropin_key('c', (int)pass, t, d);
-  const std::string *gk = cplx ? nullptr : &key;
+  const std::string *gk = whole ? nullptr : &key;
   if (pass == FWD) {
     // Owl memsets the output first (owl_ndarray_conv_impl.h:285): an empty
     // GEMM (K == 0 cannot occur, M == 0 writes nothing) leaves zeros.
     return run_dropin({{input, nx}, {kernel, nw}, {nullptr, ny}}, 2, output, ny,
                       [&](void **dv, hipStream_t st, DropinCtx &) {
                         return owl_amd_conv_fwd(&d, t, t, dv[0], dv[1], dv[2], st);
-                      }, -1, nullptr, 0, cplx, gk);
+                      }, -1, nullptr, 0, whole, gk);
   } else if (pass == BWD_IN) {
     return run_dropin({{kernel, nw}, {output, ny}, {nullptr, nx}}, 2, input, nx,
                       [&](void **dv, hipStream_t st, DropinCtx &) {
                         return owl_amd_conv_bwd_input(&d, t, t, dv[0], dv[1], dv[2], st);
-                      }, -1, nullptr, 0, cplx, gk);
+                      }, -1, nullptr, 0, whole, gk);
   } else {
     return run_dropin({{input, nx}, {output, ny}, {nullptr, nw}}, 2, kernel, nw,
                       [&](void **dv, hipStream_t st, DropinCtx &c) {
@@ -1755,7 +1764,7 @@ int conv_dropin(ConvPass pass, T *input, T *kernel, T *output, int64_t B, int64_
                         if (r) return r;
                         return owl_amd_conv_bwd_kernel(&d, t, t, dv[0], dv[1], dv[2], need ? c.ws.ptr : nullptr,
                                                        need, st);
-                      }, -1, nullptr, 0, cplx, gk);
+                      }, -1, nullptr, 0, whole, gk);
   }
 }
 

@@ -3,7 +3,8 @@ staging chunk (8 MiB) move through several chunks and give the same bits as
 the device path on the same data; calls from several host threads at once
 (ctypes releases the GIL) each get their own staging context and give the
 same bits as the same calls made one after another, with the drop-in graph
-cache off and on; repeated calls of one shape replay captured graphs with
+cache off and on (strided 1x1 backward inputs among them, which share a
+per-device buffer and so run under the library lock); repeated calls of one shape replay captured graphs with
 the bits of the device path."""
 import threading
 
@@ -44,6 +45,11 @@ def _work(seed):
     out = [y, nd.conv2d_backward_input(x, w, [1, 1], dy), nd.conv2d_backward_kernel(x, w, [1, 1], dy)]
     p = nd.max_pool2d(y, [3, 3], [2, 2], SAME)
     out.append(nd.max_pool2d_backward(SAME, y, [3, 3], [2, 2], rng.standard_normal(p.shape).astype(np.float32)))
+    # a strided 1x1 backward input: its subsampled grid goes through one
+    # per-device buffer, whose size differs between the threads' calls
+    w1 = (rng.standard_normal([1, 1, 16, 32 + 8 * (seed % 3)]) * 0.2).astype(np.float32)
+    y1 = nd.conv2d(x, w1, [2, 2], SAME)
+    out.append(nd.conv2d_backward_input(x, w1, [2, 2], rng.standard_normal(y1.shape).astype(np.float32)))
     return out
 
 
