@@ -58,6 +58,7 @@ CASES = [  # (x shape, kernel shape, stride, padding, rate)
     ([2, 11, 10, 4], [3, 2, 4, 6], [2, 1], VALID, None),
     ([1, 12, 12, 8], [3, 3, 8, 8], [1, 1], SAME, [2, 2]),
     ([2, 5, 6, 4, 3], [2, 3, 2, 3, 4], [1, 2, 1], SAME, None),
+    ([2, 9, 10, 6], [1, 1, 6, 4], [2, 3], SAME, None),   # 1x1 with strides: subsampled grid + scatter
 ]
 
 
