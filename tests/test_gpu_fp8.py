@@ -54,6 +54,32 @@ def quantize(t):
     return q.view(t.shape), sc
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("dt,n,offset", [(torch.bfloat16, 1000003, 0), (torch.float32, 1000001, 0),
+                                         (torch.bfloat16, 4099, 1), (torch.float32, 77, 3)])
+def test_quantizer_matches_torch(dt, n, offset):
+    """The per-tensor quantiser: scale = amax / 448 and q = e4m3(x * (448 /
+    amax)), byte for byte against torch's e4m3 cast of the same FP32
+    products; lengths that leave a tail past the last 16-byte chunk, and
+    sources offset from 16-byte alignment (the element-by-element path)."""
+    g = torch.Generator(device=DEV)
+    g.manual_seed(n)
+    base = (torch.randn(n + offset, device=DEV, generator=g) * 3).to(dt)
+    t = base[offset:]
+    q = torch.empty(n, dtype=torch.uint8, device=DEV)
+    sc = torch.empty(1, dtype=torch.float32, device=DEV)
+    code = _lib.F32 if dt == torch.float32 else _lib.BF16
+    _lib.check(_lib.lib().owl_amd_fp8_quantize(code, ctypes.c_void_p(t.data_ptr()), n,
+                                               ctypes.c_void_p(q.data_ptr()), ctypes.c_void_p(sc.data_ptr()), _s()))
+    torch.cuda.synchronize()
+    v = t.float()
+    amax = v.abs().max()
+    inv = torch.tensor(448.0, device=DEV) / amax
+    assert sc.item() == (amax / torch.tensor(448.0, device=DEV)).item()
+    ref = (v * inv).to(torch.float8_e4m3fn).view(torch.uint8)
+    assert torch.equal(q, ref)
+
+
 def dequant(q, sc):
     f = q.view(torch.float8_e4m3fn).float()
     return (f.double() * float(sc.item())).cpu().numpy()
