@@ -115,6 +115,28 @@ void run_upsample(bool backward, const T *src, T *dst, int64_t B, int H, int W, 
 // strided grid, zero elsewhere (rows of row_bytes bytes, a multiple of 2).
 void run_stride_scatter(const void *src, void *dst, int64_t B, int Hs, int Ws, int Ds, int H, int W, int D, int sh,
                         int sw, int sd, int64_t row_bytes, hipStream_t st);
+// resample.hip: the phase grids of a strided backward input (phase q =
+// (ch, cw, cd) holds the dx pixels (sh m + ch, sw n + cw, sd l + cd) as a
+// dense [B][mh][mw][md] grid at byte offset off[q] of src, or zeros when
+// off[q] < 0) interleaved into dx; first[q] = the workgroups of the phases
+// before q (B mh rows each), first[n] their total.
+constexpr int MAX_PHASES = 64;
+struct PhaseMap {
+  int n;
+  int ch[MAX_PHASES], cw[MAX_PHASES], cd[MAX_PHASES];
+  int mh[MAX_PHASES], mw[MAX_PHASES], md[MAX_PHASES];
+  int64_t off[MAX_PHASES];
+  int64_t first[MAX_PHASES + 1];
+};
+void run_phase_interleave(const void *src, void *dst, int H, int W, int D, int sh, int sw, int sd, int64_t row_bytes,
+                          const PhaseMap &pm, hipStream_t st);
+// resample.hip: dst block j (block_bytes each) = src block src[j], j < n.
+constexpr int MAX_TAPS = 64;
+struct TapMap {
+  int n;
+  int src[MAX_TAPS];
+};
+void run_tap_gather(const void *src, void *dst, int64_t block_bytes, const TapMap &tm, hipStream_t st);
 
 // complex.hip: complex kernels as real ones (see there).  pattern 0 expands
 // the forward's kernel, 1 the backward input's; fold turns the real weight

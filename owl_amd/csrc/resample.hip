@@ -153,6 +153,102 @@ static void launch_scatter(const void *src, void *dst, int64_t B, int Hs, int Ws
            Ws, Ds, sh, sw, sd, upr);
 }
 
+// Phase interleave for the backward input of a strided convolution: dx is
+// the union of s_h s_w s_d phase grids, phase (c_h, c_w, c_d) holding the
+// pixels (s_h m + c_h, s_w n + c_w, s_d l + c_d) as a dense [B][Mh][Mw][Md]
+// grid of rows of row_bytes bytes (at byte offset off in src; off < 0: the
+// phase has no kernel taps and its pixels are zero).  A workgroup copies one
+// source row (phase, b, m); the phase is found from the workgroup index by a
+// uniform scan of the phases' first workgroups, so every lookup is scalar.
+template <typename U>
+__global__ __launch_bounds__(256) void phase_interleave_kernel(const char *__restrict__ src, U *__restrict__ dst, int H,
+                                                               int W, int D, int sh, int sw, int sd, int upr,
+                                                               PhaseMap pm) {
+  const int64_t bid = blockIdx.x;
+  int p = 0;
+  while (p + 1 < pm.n && bid >= pm.first[p + 1]) ++p;
+  const int Mh = pm.mh[p], Mw = pm.mw[p], Md = pm.md[p];
+  const int64_t r = bid - pm.first[p];  // b * Mh + m
+  const int64_t b = r / Mh;
+  const int m = (int)(r - b * Mh);
+  const int h = sh * m + pm.ch[p];
+  const int npix = Mw * Md;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ppw = upr >= 64 ? 1 : 64 / upr;
+  const int sp = upr >= 64 ? 0 : lane / upr;
+  const int u0 = upr >= 64 ? lane : lane - sp * upr;
+  const bool lane_ok = upr >= 64 || sp < ppw;
+  const bool zero = pm.off[p] < 0;
+  const U *srow = (const U *)(src + (zero ? 0 : pm.off[p])) + r * npix * upr;
+  U *drow = dst + ((int64_t)b * H + h) * W * D * upr;
+  for (int p0 = wv * ppw; p0 < npix; p0 += 4 * ppw) {
+    const int pix = p0 + sp;
+    if (!lane_ok || pix >= npix) continue;
+    int n = pix, l = 0;
+    if (Md != 1) {
+      n = pix / Md;
+      l = pix - n * Md;
+    }
+    const int64_t dpix = (int64_t)(sw * n + pm.cw[p]) * D + sd * l + pm.cd[p];
+    for (int u = u0; u < upr; u += 64) {
+      U v;
+      __builtin_memset(&v, 0, sizeof(U));
+      if (!zero) v = srow[(int64_t)pix * upr + u];
+      drow[dpix * upr + u] = v;
+    }
+  }
+}
+
+template <typename U>
+static void launch_interleave(const void *src, void *dst, int H, int W, int D, int sh, int sw, int sd, int upr,
+                              const PhaseMap &pm, hipStream_t st) {
+  const int64_t blocks = pm.first[pm.n];
+  if (blocks <= 0) return;
+  launch_k((phase_interleave_kernel<U>), dim3((unsigned)blocks), dim3(256), st, (const char *)src, (U *)dst, H, W, D,
+           sh, sw, sd, upr, pm);
+}
+
+void run_phase_interleave(const void *src, void *dst, int H, int W, int D, int sh, int sw, int sd, int64_t row_bytes,
+                          const PhaseMap &pm, hipStream_t st) {
+  uintptr_t al = (uintptr_t)src | (uintptr_t)dst;
+  for (int q = 0; q < pm.n; ++q)
+    if (pm.off[q] > 0) al |= (uintptr_t)pm.off[q];
+  for (int ub : {16, 8, 4}) {
+    if (row_bytes % ub == 0 && al % ub == 0) {
+      const int upr = (int)(row_bytes / ub);
+      if (ub == 16) launch_interleave<uint4>(src, dst, H, W, D, sh, sw, sd, upr, pm, st);
+      else if (ub == 8) launch_interleave<uint2>(src, dst, H, W, D, sh, sw, sd, upr, pm, st);
+      else launch_interleave<uint32_t>(src, dst, H, W, D, sh, sw, sd, upr, pm, st);
+      return;
+    }
+  }
+  launch_interleave<uint16_t>(src, dst, H, W, D, sh, sw, sd, (int)(row_bytes / 2), pm, st);
+}
+
+// Tap-block gather: block j of dst (block_bytes bytes) = block src_tap[j] of
+// src; a workgroup copies 256 16-byte chunks (or 4-byte words) of one block.
+template <typename U>
+__global__ __launch_bounds__(256) void tap_gather_kernel(const U *__restrict__ src, U *__restrict__ dst, int upb,
+                                                         int wg_per_block, TapMap tm) {
+  const int j = blockIdx.x / wg_per_block;
+  const int64_t u = (int64_t)(blockIdx.x - j * wg_per_block) * 256 + threadIdx.x;
+  if (u < upb) dst[(int64_t)j * upb + u] = src[(int64_t)tm.src[j] * upb + u];
+}
+
+void run_tap_gather(const void *src, void *dst, int64_t block_bytes, const TapMap &tm, hipStream_t st) {
+  const bool v16 = block_bytes % 16 == 0 && (((uintptr_t)src | (uintptr_t)dst) & 15) == 0;
+  const int ub = v16 ? 16 : 2;
+  const int upb = (int)(block_bytes / ub);
+  const int wpb = (upb + 255) / 256;
+  if (tm.n <= 0 || wpb <= 0) return;
+  if (v16)
+    launch_k((tap_gather_kernel<uint4>), dim3((unsigned)(tm.n * wpb)), dim3(256), st, (const uint4 *)src, (uint4 *)dst,
+             upb, wpb, tm);
+  else
+    launch_k((tap_gather_kernel<uint16_t>), dim3((unsigned)(tm.n * wpb)), dim3(256), st, (const uint16_t *)src,
+             (uint16_t *)dst, upb, wpb, tm);
+}
+
 void run_stride_scatter(const void *src, void *dst, int64_t B, int Hs, int Ws, int Ds, int H, int W, int D, int sh,
                         int sw, int sd, int64_t row_bytes, hipStream_t st) {
   const uintptr_t al = (uintptr_t)src | (uintptr_t)dst;
