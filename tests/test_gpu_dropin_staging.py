@@ -3,8 +3,8 @@ staging chunk (8 MiB) move through several chunks and give the same bits as
 the device path on the same data; calls from several host threads at once
 (ctypes releases the GIL) each get their own staging context and give the
 same bits as the same calls made one after another, with the drop-in graph
-cache off and on (strided 1x1 backward inputs among them, which share a
-per-device buffer and so run under the library lock); repeated calls of one shape replay captured graphs with
+cache off and on (strided backward inputs among them, which share
+per-device buffers and so run under the library lock); repeated calls of one shape replay captured graphs with
 the bits of the device path."""
 import threading
 
@@ -50,6 +50,10 @@ def _work(seed):
     w1 = (rng.standard_normal([1, 1, 16, 32 + 8 * (seed % 3)]) * 0.2).astype(np.float32)
     y1 = nd.conv2d(x, w1, [2, 2], SAME)
     out.append(nd.conv2d_backward_input(x, w1, [2, 2], rng.standard_normal(y1.shape).astype(np.float32)))
+    # and a strided 3x3 one (stride phases: phase grids and grouped taps in
+    # per-device buffers)
+    y3 = nd.conv2d(x, w, [2, 2], SAME)
+    out.append(nd.conv2d_backward_input(x, w, [2, 2], rng.standard_normal(y3.shape).astype(np.float32)))
     return out
 
 
