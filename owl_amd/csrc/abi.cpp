@@ -193,6 +193,7 @@ struct DeviceState {
   DevBuf ytmp;                   // fused conv + pool without y, when the pooling runs apart
   DevBuf dsub;                   // strided backward input: dx's phase grids
   DevBuf wsub;                   // strided backward input: the kernel's taps grouped by phase
+  DevBuf xsub;                   // strided forward with dilation a multiple of the stride: x's one phase
 };
 
 // Recursive: the drop-ins hold it while calling the device API, which takes it
@@ -539,6 +540,69 @@ int complex_conv(int pass, const owl_amd_conv_desc *d, int in_t, int out_t, cons
 }  // namespace
 
 // ------------------------------------------------------------------ device API: conv
+namespace {
+// A strided forward whose dilation is a multiple of the stride along every
+// strided dimension (OWL_AMD_STRIDED_FWD_PHASE=0 turns the path off; read on
+// every call): x[s i + r a - p] always has residue c = (-p) mod s, so the
+// convolution reads only x's phase c, x_c[m] = x[s m + c], and equals the
+// stride-1 convolution of x_c (M = ceil((H - c) / s) pixels) with rate r / s
+// and pad (p + c) / s.  That phase is gathered into a per-device library
+// buffer and the stride-1 forward (the halo kernel where it fits) runs on it;
+// every y value is the same sum over (tap, ci), in that kernel's order.  A
+// 24x112x112x512 -> 128 5x5 layer with stride 2 and rate 2 (BF16): 510.7 us on
+// the direct-A kernel.
+bool strided_fwd_one_phase(const owl_amd_conv_desc *d) {
+  const int64_t s3[3] = {d->sh, d->sw, d->sd}, r3[3] = {d->rh, d->rw, d->rd}, p3[3] = {d->ph, d->pw, d->pd},
+                n3[3] = {d->H, d->W, d->D};
+  if (d->sh == 1 && d->sw == 1 && d->sd == 1) return false;
+  for (int a = 0; a < 3; ++a) {
+    if (r3[a] % s3[a] != 0) return false;
+    if ((s3[a] - p3[a] % s3[a]) % s3[a] >= n3[a]) return false;  // the phase has no pixels
+  }
+  return env_int("OWL_AMD_STRIDED_FWD_PHASE", 1) != 0;
+}
+
+int conv_fwd_one_phase(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x, const void *w,
+                       const void *bias, int act, void *y, void *pre, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t s3[3] = {d->sh, d->sw, d->sd}, p3[3] = {d->ph, d->pw, d->pd}, n3[3] = {d->H, d->W, d->D};
+  int64_t c[3], M[3];
+  for (int a = 0; a < 3; ++a) {
+    c[a] = (s3[a] - p3[a] % s3[a]) % s3[a];
+    M[a] = (n3[a] - c[a] + s3[a] - 1) / s3[a];
+  }
+  const int64_t row_bytes = d->Ci * (int64_t)dtype_size(in_t);
+  void *xs = nullptr;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DeviceState *s;
+    int rc = ensure_state(&s);
+    if (rc) return rc;
+    rc = ensure_buf(s->xsub, (size_t)(d->B * M[0] * M[1] * M[2] * row_bytes), st);
+    if (rc) return rc;
+    xs = s->xsub.ptr;
+  }
+  {
+    KTimer kt(OWL_AMD_KT_AUX, "stride_gather_kernel", st);
+    run_stride_gather(x, xs, d->B, (int)d->H, (int)d->W, (int)d->D, (int)M[0], (int)M[1], (int)M[2], (int)d->sh,
+                      (int)d->sw, (int)d->sd, (int)c[0], (int)c[1], (int)c[2], row_bytes, st);
+  }
+  OWL_CHECK_HIP(hipGetLastError());
+  owl_amd_conv_desc q = *d;
+  q.H = M[0];
+  q.W = M[1];
+  q.D = M[2];
+  q.rh = d->rh / d->sh;
+  q.rw = d->rw / d->sw;
+  q.rd = d->rd / d->sd;
+  q.ph = (d->ph + c[0]) / d->sh;
+  q.pw = (d->pw + c[1]) / d->sw;
+  q.pd = (d->pd + c[2]) / d->sd;
+  q.sh = q.sw = q.sd = 1;
+  return owl_amd_conv_fwd_bias_act(&q, in_t, out_t, xs, w, bias, act, y, pre, stream);
+}
+}  // namespace
+
 extern "C" int owl_amd_conv_fwd(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x,
                                 const void *w, void *y, void *stream) {
   if (is_complex(in_t) || is_complex(out_t)) return complex_conv(0, d, in_t, out_t, x, w, y, nullptr, 0, stream);
@@ -562,6 +626,8 @@ extern "C" int owl_amd_conv_fwd_bias_act(const owl_amd_conv_desc *d, int in_t, i
   p.K = d->kh * d->kw * d->kd * d->Ci;
   if (p.M == 0 || p.N == 0) return OWL_AMD_OK;
   if (!x || !w || !y) ARG_FAIL("null pointer");
+  if (dtype_size(in_t) && dtype_size(out_t) && strided_fwd_one_phase(d))
+    return conv_fwd_one_phase(d, in_t, out_t, x, w, bias, act, y, pre, stream);
   p.a = x; p.b = w; p.c = y; p.ldc = p.Co;
   p.bias = bias; p.pre = pre; p.act = act;
   const int v = vec_of(in_t);
@@ -1757,7 +1823,7 @@ extern "C" int owl_amd_cache_stats(int64_t *n, int64_t *bytes) {
   int64_t cnt = 0, tot = 0;
   for (auto &s : g_dev) {
     if (!s.ready) continue;
-    for (const DevBuf *b : {&s.ws, &s.red, &s.cplx, &s.f8w, &s.ytmp, &s.dsub, &s.wsub})
+    for (const DevBuf *b : {&s.ws, &s.red, &s.cplx, &s.f8w, &s.ytmp, &s.dsub, &s.wsub, &s.xsub})
       if (b->ptr) { cnt++; tot += (int64_t)b->bytes; }
   }
   for (DropinCtx *c : g_ctx)
@@ -1793,7 +1859,7 @@ extern "C" void owl_amd_cache_release(void) {
     if (hipSetDevice(dev) != hipSuccess) continue;
     (void)hipStreamSynchronize(s.stream);
     (void)hipDeviceSynchronize();
-    for (DevBuf *b : {&s.ws, &s.red, &s.cplx, &s.f8w, &s.ytmp, &s.dsub, &s.wsub}) {
+    for (DevBuf *b : {&s.ws, &s.red, &s.cplx, &s.f8w, &s.ytmp, &s.dsub, &s.wsub, &s.xsub}) {
       if (b->ptr) (void)hipFree(b->ptr);
       *b = DevBuf();
     }
@@ -1861,10 +1927,12 @@ int conv_dropin(ConvPass pass, T *input, T *kernel, T *output, int64_t B, int64_
   const size_t nw = (size_t)(kh * kw * kd * Ci * Co) * sizeof(T);
   const size_t ny = (size_t)(B * Ho * Wo * Do * Co) * sizeof(T);
   const int t = dt<T>();
-  // complex kernels (expanded into one per-device buffer) and strided
-  // backward inputs (their phase grids in per-device buffers): the whole call
-  // locked, not captured
-  const bool whole = is_complex(t) || (pass == BWD_IN && (strided_1x1_bwd_input(&d) || strided_phase_bwd_input(&d)));
+  // complex kernels (expanded into one per-device buffer), strided backward
+  // inputs (their phase grids in per-device buffers) and strided forwards
+  // that read one phase of x (gathered into a per-device buffer): the whole
+  // call locked, not captured
+  const bool whole = is_complex(t) || (pass == BWD_IN && (strided_1x1_bwd_input(&d) || strided_phase_bwd_input(&d))) ||
+                     (pass == FWD && strided_fwd_one_phase(&d));
   const std::string key = dropin_key('c', (int)pass, t, d);
   const std::string *gk = whole ? nullptr : &key;
   if (pass == FWD) {

@@ -137,6 +137,10 @@ struct TapMap {
   int src[MAX_TAPS];
 };
 void run_tap_gather(const void *src, void *dst, int64_t block_bytes, const TapMap &tm, hipStream_t st);
+// resample.hip: dst[b, m, n, l] = src[b, sh m + ch, sw n + cw, sd l + cd], dst
+// the dense [B][Mh][Mw][Md] grid of one stride phase (rows of row_bytes bytes).
+void run_stride_gather(const void *src, void *dst, int64_t B, int H, int W, int D, int Mh, int Mw, int Md, int sh,
+                       int sw, int sd, int ch, int cw, int cd, int64_t row_bytes, hipStream_t st);
 
 // complex.hip: complex kernels as real ones (see there).  pattern 0 expands
 // the forward's kernel, 1 the backward input's; fold turns the real weight

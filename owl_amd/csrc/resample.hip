@@ -225,6 +225,62 @@ void run_phase_interleave(const void *src, void *dst, int H, int W, int D, int s
   launch_interleave<uint16_t>(src, dst, H, W, D, sh, sw, sd, (int)(row_bytes / 2), pm, st);
 }
 
+// Stride-phase gather: dst[b, m, n, l] = src[b, sh m + ch, sw n + cw, sd l + cd]
+// (rows of row_bytes bytes; dst is the dense [B][Mh][Mw][Md] grid of one
+// phase).  A workgroup copies one dst row (b, m), lanes over U-sized units as
+// in the scatter above.
+template <typename U>
+__global__ __launch_bounds__(256) void stride_gather_kernel(const U *__restrict__ src, U *__restrict__ dst, int H,
+                                                            int W, int D, int Mh, int Mw, int Md, int sh, int sw,
+                                                            int sd, int ch, int cw, int cd, int upr) {
+  const int64_t bm = blockIdx.x;  // b * Mh + m
+  const int64_t b = bm / Mh;
+  const int m = (int)(bm - b * Mh);
+  const int h = sh * m + ch;
+  const int npix = Mw * Md;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ppw = upr >= 64 ? 1 : 64 / upr;
+  const int sp = upr >= 64 ? 0 : lane / upr;
+  const int u0 = upr >= 64 ? lane : lane - sp * upr;
+  const bool lane_ok = upr >= 64 || sp < ppw;
+  const U *srow = src + ((int64_t)b * H + h) * W * D * upr;
+  U *drow = dst + bm * npix * upr;
+  for (int p0 = wv * ppw; p0 < npix; p0 += 4 * ppw) {
+    const int pix = p0 + sp;
+    if (!lane_ok || pix >= npix) continue;
+    int n = pix, l = 0;
+    if (Md != 1) {
+      n = pix / Md;
+      l = pix - n * Md;
+    }
+    const int64_t spix = (int64_t)(sw * n + cw) * D + sd * l + cd;
+    for (int u = u0; u < upr; u += 64) drow[(int64_t)pix * upr + u] = srow[spix * upr + u];
+  }
+}
+
+template <typename U>
+static void launch_gather(const void *src, void *dst, int64_t B, int H, int W, int D, int Mh, int Mw, int Md, int sh,
+                          int sw, int sd, int ch, int cw, int cd, int upr, hipStream_t st) {
+  if (B * Mh == 0) return;
+  launch_k((stride_gather_kernel<U>), dim3((unsigned)(B * Mh)), dim3(256), st, (const U *)src, (U *)dst, H, W, D, Mh,
+           Mw, Md, sh, sw, sd, ch, cw, cd, upr);
+}
+
+void run_stride_gather(const void *src, void *dst, int64_t B, int H, int W, int D, int Mh, int Mw, int Md, int sh,
+                       int sw, int sd, int ch, int cw, int cd, int64_t row_bytes, hipStream_t st) {
+  const uintptr_t al = (uintptr_t)src | (uintptr_t)dst;
+  for (int ub : {16, 8, 4}) {
+    if (row_bytes % ub == 0 && al % ub == 0) {
+      const int upr = (int)(row_bytes / ub);
+      if (ub == 16) launch_gather<uint4>(src, dst, B, H, W, D, Mh, Mw, Md, sh, sw, sd, ch, cw, cd, upr, st);
+      else if (ub == 8) launch_gather<uint2>(src, dst, B, H, W, D, Mh, Mw, Md, sh, sw, sd, ch, cw, cd, upr, st);
+      else launch_gather<uint32_t>(src, dst, B, H, W, D, Mh, Mw, Md, sh, sw, sd, ch, cw, cd, upr, st);
+      return;
+    }
+  }
+  launch_gather<uint16_t>(src, dst, B, H, W, D, Mh, Mw, Md, sh, sw, sd, ch, cw, cd, (int)(row_bytes / 2), st);
+}
+
 // Tap-block gather: block j of dst (block_bytes bytes) = block src_tap[j] of
 // src; a workgroup copies 256 16-byte chunks (or 4-byte words) of one block.
 template <typename U>

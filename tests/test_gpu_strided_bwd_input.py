@@ -8,7 +8,9 @@ the FP64 oracle on the same rounded inputs (FP32 / FP64 outputs rel-L2 <
 pixel (OWL_AMD_STRIDED_PHASES=0).  Cases: kernels 2, 3 and 5 with strides 2
 and 3 (some phases without taps when the kernel is shorter than the
 stride), SAME and VALID, uneven sizes, mixed strides, 3-D, every real
-dtype."""
+dtype.  Also the forward of strided convolutions whose dilation is a
+multiple of the stride, which read one stride phase of x: against the
+oracle and the direct path (OWL_AMD_STRIDED_FWD_PHASE=0)."""
 import os
 
 import numpy as np
@@ -87,3 +89,57 @@ def test_strided_bwd_input_by_phases(case):
     assert rel(dx, dg) < TOL[dt], rel(dx, dg)
     # the same call twice gives the same bits
     assert np.array_equal(dx, _bwd_input(xt, wt, stride, gt, generic=False))
+
+
+# Forward of strided convolutions whose dilation is a multiple of the stride:
+# they read one stride phase of x, gathered, then a stride-1 convolution with
+# rate r / s.  (x shape, kernel shape, strides, rates, padding, dtype)
+FWD_CASES = [
+    ([4, 28, 28, 32], [3, 3, 32, 48], [2, 2], [2, 2], SAME, "bf16"),
+    ([2, 29, 31, 16], [5, 5, 16, 24], [2, 2], [2, 2], VALID, "f32"),
+    ([2, 16, 17, 8], [3, 3, 8, 16], [3, 3], [3, 3], SAME, "f16"),
+    ([2, 9, 10, 16], [3, 3, 16, 8], [2, 1], [2, 1], SAME, "f64"),     # stride 1 along w
+    ([2, 14, 12, 16], [3, 3, 16, 16], [2, 2], [4, 2], SAME, "bf16"),  # rate twice the stride
+    ([2, 8, 9, 10, 16], [3, 3, 3, 16, 8], [2, 2, 2], [2, 2, 2], SAME, "bf16"),  # 3-D
+]
+
+
+def _fwd(x, w, stride, rate, pad, generic):
+    old = os.environ.get("OWL_AMD_STRIDED_FWD_PHASE")
+    if generic:
+        os.environ["OWL_AMD_STRIDED_FWD_PHASE"] = "0"
+    try:
+        f = nd.dilated_conv2d if x.dim() == 4 else nd.dilated_conv3d
+        y = f(x, w, stride, rate, pad)
+        torch.cuda.synchronize()
+    finally:
+        if old is None:
+            os.environ.pop("OWL_AMD_STRIDED_FWD_PHASE", None)
+        else:
+            os.environ["OWL_AMD_STRIDED_FWD_PHASE"] = old
+    return y.double().cpu().numpy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(FWD_CASES)))
+def test_strided_dilated_fwd_on_one_phase(case):
+    xs, ks, stride, rate, pad, dt = FWD_CASES[case]
+    print(f"one-phase forward case {case} x{xs} w{ks} s{stride} r{rate} {dt}", flush=True)
+    td = TD[dt]
+    rng = np.random.default_rng(950 + case)
+    x = rng.standard_normal(xs)
+    w = rng.standard_normal(ks) / np.sqrt(np.prod(ks[:-1]))
+    rnd = lambda a: torch.from_numpy(a).to(td).to(torch.float64).numpy()
+    ref = on.dilated_conv2d(rnd(x), rnd(w), stride, rate, pad)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV).to(td)
+    xt, wt = dev(x), dev(w)
+    _lib.kernel_timing(True)
+    y = _fwd(xt, wt, stride, rate, pad, generic=False)
+    aux = _lib.kernel_time(_lib.KT_AUX)[2]
+    _lib.kernel_timing(False)
+    assert aux == "stride_gather_kernel", aux
+    assert y.shape == ref.shape
+    assert rel(y, ref) < TOL[dt], rel(y, ref)
+    yg = _fwd(xt, wt, stride, rate, pad, generic=True)
+    assert rel(y, yg) < TOL[dt], rel(y, yg)
+    assert np.array_equal(y, _fwd(xt, wt, stride, rate, pad, generic=False))
