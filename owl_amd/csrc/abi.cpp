@@ -601,6 +601,106 @@ int conv_fwd_one_phase(const owl_amd_conv_desc *d, int in_t, int out_t, const vo
   q.sh = q.sw = q.sd = 1;
   return owl_amd_conv_fwd_bias_act(&q, in_t, out_t, xs, w, bias, act, y, pre, stream);
 }
+
+// Space-to-depth forward of a strided 2-D convolution with unit dilation
+// (16-bit storage; OWL_AMD_S2D=0 turns it off, 2 takes it wherever it applies
+// regardless of size, for tests; read on every call).  With
+// a - p = s u + c, y[i] = sum over phases c and u of x_c[i + u] w[s u + c + p],
+// x_c[m] = x[s m + c]: a stride-1 convolution of X = x's phases stacked as
+// channels ([B][ceil(H / sh)][ceil(W / sw)][sh sw Ci]) with U = u_max - u_min
+// + 1 taps per dimension (u_min = -ceil(p / s), u_max = floor((k - 1 - p) /
+// s)), pad -u_min, and the kernel's taps moved to (u - u_min, phase) blocks,
+// zero where s u + c + p falls outside the kernel.  The zero taps cost
+// U_h U_w sh sw / (kh kw) times the MACs, so the path is taken when that is at
+// most 1.5 (5x5 and larger kernels at stride 2) and the stride-1 problem fits
+// the halo kernel: the ragged batch's 11x112x112x256 -> 256 5x5 stride-2 layer
+// ran 392 us on the direct-A kernel.
+struct S2dPlan {
+  int64_t umin[2], U[2], M[2];
+};
+bool s2d_plan(const owl_amd_conv_desc *d, int in_t, int out_t, S2dPlan *pl, owl_amd_conv_desc *q) {
+  if (in_t != OWL_AMD_BF16 && in_t != OWL_AMD_F16) return false;
+  if (d->sh == 1 && d->sw == 1) return false;
+  if (d->D != 1 || d->kd != 1 || d->sd != 1 || d->Do != 1 || d->pd != 0) return false;
+  if (d->rh != 1 || d->rw != 1) return false;
+  const int mode = env_int("OWL_AMD_S2D", 1);  // 0 off, 1 where it pays, 2 wherever it applies (tests)
+  if (mode == 0) return false;
+  const int64_t s2[2] = {d->sh, d->sw}, k2[2] = {d->kh, d->kw}, p2[2] = {d->ph, d->pw}, n2[2] = {d->H, d->W};
+  for (int a = 0; a < 2; ++a) {
+    if (p2[a] > k2[a] - 1) return false;
+    pl->umin[a] = -((p2[a] + s2[a] - 1) / s2[a]);
+    pl->U[a] = (k2[a] - 1 - p2[a]) / s2[a] - pl->umin[a] + 1;
+    pl->M[a] = (n2[a] + s2[a] - 1) / s2[a];
+  }
+  const int64_t nph = d->sh * d->sw;
+  if (pl->U[0] * pl->U[1] * nph * 2 > 3 * d->kh * d->kw) return false;  // more than 1.5x the MACs
+  // enough output tiles for the stride-1 kernel (a 24x7x7x256 5x5 layer ran
+  // 122 against 67 us), and enough work per byte of x that rewriting x costs
+  // little beside it (12x112x112x256 -> 128 5x5: 144 us plus ~30 us of
+  // rewriting against 157 us; 11x112x112x256 -> 256: 220 + ~30 against 389)
+  const double out_px = (double)d->B * d->Ho * d->Wo;
+  const double flop = 2.0 * out_px * d->Co * d->kh * d->kw * d->Ci, xbytes = (double)d->B * d->H * d->W * d->Ci * 2.0;
+  if (mode == 1 && (out_px < 16384.0 || flop < 1024.0 * xbytes)) return false;
+  if (pl->U[0] * pl->U[1] * nph > MAX_S2D_BLOCKS) return false;
+  *q = *d;
+  q->H = pl->M[0];
+  q->W = pl->M[1];
+  q->Ci = nph * d->Ci;
+  q->kh = pl->U[0];
+  q->kw = pl->U[1];
+  q->sh = q->sw = 1;
+  q->ph = -pl->umin[0];
+  q->pw = -pl->umin[1];
+  if (check_conv(q)) return false;
+  ConvParams cp = make_params(q);
+  return halo_fits(cp, 0, 2, (int)dtype_size(out_t));
+}
+
+// The forward takes the space-to-depth path (its phases and kernel in
+// per-device library buffers).
+bool s2d_uses_buffers(const owl_amd_conv_desc *d, int t) {
+  S2dPlan pl;
+  owl_amd_conv_desc q;
+  return s2d_plan(d, t, t, &pl, &q);
+}
+
+int conv_fwd_s2d(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x, const void *w, const void *bias,
+                 int act, void *y, void *pre, void *stream, const S2dPlan &pl, const owl_amd_conv_desc &q) {
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t es = 2, nph = d->sh * d->sw;
+  const int64_t xbytes = d->B * pl.M[0] * pl.M[1] * nph * d->Ci * es;
+  const int64_t block = d->Ci * d->Co * es;
+  S2dTaps tm;
+  memset(&tm, 0, sizeof(tm));
+  for (int64_t th = 0; th < pl.U[0]; ++th)
+    for (int64_t tw = 0; tw < pl.U[1]; ++tw)
+      for (int64_t ch = 0; ch < d->sh; ++ch)
+        for (int64_t cw = 0; cw < d->sw; ++cw) {
+          const int64_t ah = d->sh * (th + pl.umin[0]) + ch + d->ph, aw = d->sw * (tw + pl.umin[1]) + cw + d->pw;
+          tm.tap[tm.n++] = (ah >= 0 && ah < d->kh && aw >= 0 && aw < d->kw) ? (int)(ah * d->kw + aw) : -1;
+        }
+  void *xs = nullptr, *ws = nullptr;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DeviceState *s;
+    int rc = ensure_state(&s);
+    if (rc) return rc;
+    rc = ensure_buf(s->xsub, (size_t)xbytes, st);
+    if (rc) return rc;
+    rc = ensure_buf(s->wsub, (size_t)(tm.n * block), st);
+    if (rc) return rc;
+    xs = s->xsub.ptr;
+    ws = s->wsub.ptr;
+  }
+  {
+    KTimer kt(OWL_AMD_KT_AUX, "space_to_depth_kernel", st);
+    run_space_to_depth(x, xs, d->B, (int)d->H, (int)d->W, 1, (int)pl.M[0], (int)pl.M[1], 1, (int)d->sh, (int)d->sw,
+                       1, d->Ci * es, st);
+    run_s2d_kernel(w, ws, block, tm, st);
+  }
+  OWL_CHECK_HIP(hipGetLastError());
+  return owl_amd_conv_fwd_bias_act(&q, in_t, out_t, xs, ws, bias, act, y, pre, stream);
+}
 }  // namespace
 
 extern "C" int owl_amd_conv_fwd(const owl_amd_conv_desc *d, int in_t, int out_t, const void *x,
@@ -628,6 +728,12 @@ extern "C" int owl_amd_conv_fwd_bias_act(const owl_amd_conv_desc *d, int in_t, i
   if (!x || !w || !y) ARG_FAIL("null pointer");
   if (dtype_size(in_t) && dtype_size(out_t) && strided_fwd_one_phase(d))
     return conv_fwd_one_phase(d, in_t, out_t, x, w, bias, act, y, pre, stream);
+  {
+    S2dPlan pl;
+    owl_amd_conv_desc q;
+    if (dtype_size(out_t) && s2d_plan(d, in_t, out_t, &pl, &q))
+      return conv_fwd_s2d(d, in_t, out_t, x, w, bias, act, y, pre, stream, pl, q);
+  }
   p.a = x; p.b = w; p.c = y; p.ldc = p.Co;
   p.bias = bias; p.pre = pre; p.act = act;
   const int v = vec_of(in_t);
@@ -1932,7 +2038,7 @@ int conv_dropin(ConvPass pass, T *input, T *kernel, T *output, int64_t B, int64_
   // that read one phase of x (gathered into a per-device buffer): the whole
   // call locked, not captured
   const bool whole = is_complex(t) || (pass == BWD_IN && (strided_1x1_bwd_input(&d) || strided_phase_bwd_input(&d))) ||
-                     (pass == FWD && strided_fwd_one_phase(&d));
+                     (pass == FWD && (strided_fwd_one_phase(&d) || s2d_uses_buffers(&d, t)));
   const std::string key = dropin_key('c', (int)pass, t, d);
   const std::string *gk = whole ? nullptr : &key;
   if (pass == FWD) {

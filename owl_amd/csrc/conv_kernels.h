@@ -141,6 +141,17 @@ void run_tap_gather(const void *src, void *dst, int64_t block_bytes, const TapMa
 // the dense [B][Mh][Mw][Md] grid of one stride phase (rows of row_bytes bytes).
 void run_stride_gather(const void *src, void *dst, int64_t B, int H, int W, int D, int Mh, int Mw, int Md, int sh,
                        int sw, int sd, int ch, int cw, int cd, int64_t row_bytes, hipStream_t st);
+// resample.hip: space-to-depth of x ([B][Mh][Mw][Md][sh sw sd C], phase-major
+// channel blocks of row_bytes, zeros past the image), and the matching kernel
+// relayout: block j (block_bytes) of dst = tap tap[j] of src, zeros for -1.
+void run_space_to_depth(const void *src, void *dst, int64_t B, int H, int W, int D, int Mh, int Mw, int Md, int sh,
+                        int sw, int sd, int64_t row_bytes, hipStream_t st);
+constexpr int MAX_S2D_BLOCKS = 256;
+struct S2dTaps {
+  int n;
+  int tap[MAX_S2D_BLOCKS];
+};
+void run_s2d_kernel(const void *src, void *dst, int64_t block_bytes, const S2dTaps &tm, hipStream_t st);
 
 // complex.hip: complex kernels as real ones (see there).  pattern 0 expands
 // the forward's kernel, 1 the backward input's; fold turns the real weight

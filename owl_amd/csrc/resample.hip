@@ -281,6 +281,96 @@ void run_stride_gather(const void *src, void *dst, int64_t B, int H, int W, int 
   launch_gather<uint16_t>(src, dst, B, H, W, D, Mh, Mw, Md, sh, sw, sd, ch, cw, cd, (int)(row_bytes / 2), st);
 }
 
+// Space-to-depth: dst[b, m, n, l, ((c_h sw + c_w) sd + c_d) C + c] =
+// src[b, sh m + c_h, sw n + c_w, sd l + c_d, c] (zero past the image), dst
+// the [B][Mh][Mw][Md][sh sw sd C] grid; rows of row_bytes = C element bytes
+// per phase.  A workgroup writes one dst row (b, m); each wave step covers
+// one (pixel, phase) chunk run.
+template <typename U>
+__global__ __launch_bounds__(256) void space_to_depth_kernel(const U *__restrict__ src, U *__restrict__ dst, int H,
+                                                             int W, int D, int Mh, int Mw, int Md, int sh, int sw,
+                                                             int sd, int upr) {
+  const int64_t bm = blockIdx.x;  // b * Mh + m
+  const int64_t b = bm / Mh;
+  const int m = (int)(bm - b * Mh);
+  const int nph = sh * sw * sd;
+  const int nitems = Mw * Md * nph;  // (pixel, phase) pairs of the row
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ppw = upr >= 64 ? 1 : 64 / upr;
+  const int sp = upr >= 64 ? 0 : lane / upr;
+  const int u0 = upr >= 64 ? lane : lane - sp * upr;
+  const bool lane_ok = upr >= 64 || sp < ppw;
+  U *drow = dst + bm * (int64_t)nitems * upr;
+  for (int p0 = wv * ppw; p0 < nitems; p0 += 4 * ppw) {
+    const int it = p0 + sp;
+    if (!lane_ok || it >= nitems) continue;
+    const int pix = it / nph, ph = it - pix * nph;
+    int n = pix, l = 0;
+    if (Md != 1) {
+      n = pix / Md;
+      l = pix - n * Md;
+    }
+    int ch = ph, cw = 0, cd = 0;
+    if (nph != sh) {
+      cd = ph % sd;
+      const int r = ph / sd;
+      cw = r % sw;
+      ch = r / sw;
+    }
+    const int h = sh * m + ch, w = sw * n + cw, d = sd * l + cd;
+    const bool in = h < H && w < W && d < D;
+    const U *srow = src + (((int64_t)b * H + h) * W + w) * (int64_t)D * upr + (int64_t)d * upr;
+    for (int u = u0; u < upr; u += 64) {
+      U v;
+      __builtin_memset(&v, 0, sizeof(U));
+      if (in) v = srow[u];
+      drow[(int64_t)it * upr + u] = v;
+    }
+  }
+}
+
+template <typename U>
+static void launch_s2d(const void *src, void *dst, int64_t B, int H, int W, int D, int Mh, int Mw, int Md, int sh,
+                       int sw, int sd, int upr, hipStream_t st) {
+  if (B * Mh == 0) return;
+  launch_k((space_to_depth_kernel<U>), dim3((unsigned)(B * Mh)), dim3(256), st, (const U *)src, (U *)dst, H, W, D, Mh,
+           Mw, Md, sh, sw, sd, upr);
+}
+
+void run_space_to_depth(const void *src, void *dst, int64_t B, int H, int W, int D, int Mh, int Mw, int Md, int sh,
+                        int sw, int sd, int64_t row_bytes, hipStream_t st) {
+  const uintptr_t al = (uintptr_t)src | (uintptr_t)dst;
+  for (int ub : {16, 8, 4}) {
+    if (row_bytes % ub == 0 && al % ub == 0) {
+      const int upr = (int)(row_bytes / ub);
+      if (ub == 16) launch_s2d<uint4>(src, dst, B, H, W, D, Mh, Mw, Md, sh, sw, sd, upr, st);
+      else if (ub == 8) launch_s2d<uint2>(src, dst, B, H, W, D, Mh, Mw, Md, sh, sw, sd, upr, st);
+      else launch_s2d<uint32_t>(src, dst, B, H, W, D, Mh, Mw, Md, sh, sw, sd, upr, st);
+      return;
+    }
+  }
+  launch_s2d<uint16_t>(src, dst, B, H, W, D, Mh, Mw, Md, sh, sw, sd, (int)(row_bytes / 2), st);
+}
+
+// Kernel for the space-to-depth convolution: dst[t][ph][ci][co] = src tap
+// tap_of[t * nph + ph] ([ci][co] block), or zeros where it is -1.
+__global__ __launch_bounds__(256) void s2d_kernel_relayout(const uint16_t *__restrict__ src, uint16_t *__restrict__ dst,
+                                                           int upb, int wg_per_block, S2dTaps tm) {
+  const int j = blockIdx.x / wg_per_block;
+  const int64_t u = (int64_t)(blockIdx.x - j * wg_per_block) * 256 + threadIdx.x;
+  if (u >= upb) return;
+  const int t = tm.tap[j];
+  dst[(int64_t)j * upb + u] = t >= 0 ? src[(int64_t)t * upb + u] : (uint16_t)0;
+}
+
+void run_s2d_kernel(const void *src, void *dst, int64_t block_bytes, const S2dTaps &tm, hipStream_t st) {
+  const int upb = (int)(block_bytes / 2);
+  const int wpb = (upb + 255) / 256;
+  if (tm.n <= 0 || wpb <= 0) return;
+  launch_k(s2d_kernel_relayout, dim3((unsigned)(tm.n * wpb)), dim3(256), st, (const uint16_t *)src, (uint16_t *)dst,
+           upb, wpb, tm);
+}
+
 // Tap-block gather: block j of dst (block_bytes bytes) = block src_tap[j] of
 // src; a workgroup copies 256 16-byte chunks (or 4-byte words) of one block.
 template <typename U>

@@ -9,8 +9,10 @@ pixel (OWL_AMD_STRIDED_PHASES=0).  Cases: kernels 2, 3 and 5 with strides 2
 and 3 (some phases without taps when the kernel is shorter than the
 stride), SAME and VALID, uneven sizes, mixed strides, 3-D, every real
 dtype.  Also the forward of strided convolutions whose dilation is a
-multiple of the stride, which read one stride phase of x: against the
-oracle and the direct path (OWL_AMD_STRIDED_FWD_PHASE=0)."""
+multiple of the stride, which read one stride phase of x, and strided 2-D
+forwards as a stride-1 convolution of x's phases stacked as channels
+(space-to-depth): against the oracle and the direct paths
+(OWL_AMD_STRIDED_FWD_PHASE=0, OWL_AMD_S2D=0)."""
 import os
 
 import numpy as np
@@ -143,3 +145,65 @@ def test_strided_dilated_fwd_on_one_phase(case):
     yg = _fwd(xt, wt, stride, rate, pad, generic=True)
     assert rel(y, yg) < TOL[dt], rel(y, yg)
     assert np.array_equal(y, _fwd(xt, wt, stride, rate, pad, generic=False))
+
+
+# Forward of strided 2-D convolutions (unit dilation, 16-bit) as a stride-1
+# convolution of x's stride phases stacked as channels (space-to-depth), for
+# kernels whose zero-padded phase taps cost at most 1.5x the MACs.
+S2D_CASES = [
+    ([4, 28, 28, 32], [5, 5, 32, 48], [2, 2], SAME, "bf16"),
+    ([2, 29, 31, 16], [5, 5, 16, 24], [2, 2], VALID, "f16"),
+    ([2, 30, 30, 8], [7, 7, 8, 16], [2, 2], SAME, "bf16"),
+    ([2, 31, 34, 16], [5, 5, 16, 32], [3, 3], SAME, "bf16"),   # stride 3 (pads 2: 2 x 2 phase taps)
+    ([2, 20, 22, 16], [5, 3, 16, 16], [2, 1], SAME, "bf16"),   # strided along h only
+]
+
+
+def _fwd_s2d(x, w, stride, pad, generic, bias=None):
+    # these shapes are below the sizes where the library takes the path by
+    # itself: OWL_AMD_S2D=2 takes it wherever it applies
+    old = os.environ.get("OWL_AMD_S2D")
+    os.environ["OWL_AMD_S2D"] = "0" if generic else "2"
+    try:
+        if bias is None:
+            y = nd.conv2d(x, w, stride, pad)
+        else:
+            y = nd.conv2d_bias_act(x, w, bias, stride, pad)
+        torch.cuda.synchronize()
+    finally:
+        if old is None:
+            os.environ.pop("OWL_AMD_S2D", None)
+        else:
+            os.environ["OWL_AMD_S2D"] = old
+    return y.double().cpu().numpy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(S2D_CASES)))
+def test_strided_fwd_space_to_depth(case):
+    xs, ks, stride, pad, dt = S2D_CASES[case]
+    print(f"space-to-depth forward case {case} x{xs} w{ks} s{stride} {dt}", flush=True)
+    td = TD[dt]
+    rng = np.random.default_rng(980 + case)
+    x = rng.standard_normal(xs)
+    w = rng.standard_normal(ks) / np.sqrt(np.prod(ks[:-1]))
+    rnd = lambda a: torch.from_numpy(a).to(td).to(torch.float64).numpy()
+    ref = on.conv2d(rnd(x), rnd(w), stride, pad)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV).to(td)
+    xt, wt = dev(x), dev(w)
+    _lib.kernel_timing(True)
+    y = _fwd_s2d(xt, wt, stride, pad, generic=False)
+    aux = _lib.kernel_time(_lib.KT_AUX)[2]
+    fwd = _lib.kernel_time(_lib.KT_FWD)[2]
+    _lib.kernel_timing(False)
+    assert aux == "space_to_depth_kernel" and fwd == "conv_halo_kernel", (aux, fwd)
+    assert y.shape == ref.shape
+    assert rel(y, ref) < TOL[dt], rel(y, ref)
+    yg = _fwd_s2d(xt, wt, stride, pad, generic=True)
+    assert rel(y, yg) < TOL[dt], rel(y, yg)
+    assert np.array_equal(y, _fwd_s2d(xt, wt, stride, pad, generic=False))
+    # bias and relu pass through to the stride-1 convolution's epilogue
+    b = torch.from_numpy(rng.standard_normal(ks[-1]) * 0.1).float().to(DEV)
+    yb = _fwd_s2d(xt, wt, stride, pad, generic=False, bias=b)
+    ref_b = np.maximum(ref + b.double().cpu().numpy(), 0.0)
+    assert rel(yb, ref_b) < TOL[dt], rel(yb, ref_b)
