@@ -541,6 +541,71 @@ int complex_conv(int pass, const owl_amd_conv_desc *d, int in_t, int out_t, cons
 
 // ------------------------------------------------------------------ device API: conv
 namespace {
+// A forward with few output tiles over a long K (small late-stage layers,
+// single images) runs as split-K on the implicit GEMM: each split sums one
+// range of whole K-steps of (tap, channel) into FP32 (FP64) partials in the
+// library's split-K buffer, and the deterministic split-K reduction sums them
+// in split order into y.  Plain forwards only (no fused epilogue);
+// OWL_AMD_FWD_SPLITK=0 turns it off (read on every call).  Returns -1 when the
+// problem does not take it.  5x7x7x512 -> 256 3x3 stride 2 rate 2 took 124 us
+// and 16x14x14x512 -> 32 5x5 rate 2 150 us as one launch of 1-10 workgroups.
+// The split count for a forward of M x N outputs over K, or 0 when it stays
+// unsplit (more than 8 output tiles of the implicit GEMM -- larger problems
+// fill the GPU on the halo and direct-A kernels --, fewer than 64 K-steps, or
+// off).
+int64_t fwd_split_count(int64_t M, int64_t N, int64_t K, int in_t) {
+  if (env_int("OWL_AMD_FWD_SPLITK", 1) == 0) return 0;
+  const int bm = tile_m(in_t), bk = tile_k(in_t);
+  const int64_t tiles = ((M + bm - 1) / bm) * ((N + bm - 1) / bm);
+  const int64_t ksteps = (K + bk - 1) / bk;
+  if (tiles > 8 || ksteps < 64) return 0;
+  int64_t split = 256 / tiles;
+  if (split > ksteps / 8) split = ksteps / 8;
+  return split < 2 ? 0 : split;
+}
+
+// The plain forward of a descriptor takes the split-K path (its partials in
+// the per-device split-K buffer).
+bool fwd_splitk_applies(const owl_amd_conv_desc *d, int t) {
+  return dtype_size(t) && fwd_split_count(d->B * d->Ho * d->Wo * d->Do, d->Co, d->kh * d->kw * d->kd * d->Ci, t) > 0;
+}
+
+int fwd_splitk(ConvParams p, int in_t, int out_t, bool aligned, hipStream_t st) {
+  if (!dtype_size(in_t) || !dtype_size(out_t)) return -1;
+  const int bk = tile_k(in_t);
+  const int64_t ksteps = (p.K + bk - 1) / bk;
+  int64_t split = fwd_split_count(p.M, p.N, p.K, in_t);
+  if (split == 0) return -1;
+  const int64_t kps = (ksteps + split - 1) / split * bk;
+  split = (p.K + kps - 1) / kps;
+  const int64_t es = in_t == OWL_AMD_F64 ? 8 : 4;
+  const size_t bytes = (size_t)(split * p.M * p.N * es);
+  void *ws = nullptr;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DeviceState *s;
+    int rc = ensure_state(&s);
+    if (rc) return rc;
+    rc = ensure_buf(s->ws, bytes, st);
+    if (rc) return rc;
+    ws = s->ws.ptr;
+  }
+  void *y = p.c;
+  p.split = (int)split;
+  p.k_per_split = kps;
+  p.c = ws;
+  int rc = dispatch_igemm<0>(p, in_t, out_t, aligned, st);
+  if (rc) return rc;
+  const int64_t n = p.M * (int64_t)p.N;
+  KTimer kt(OWL_AMD_KT_AUX, "splitk_reduce_kernel", st);
+  if (in_t == OWL_AMD_F64) run_splitk_reduce<double, double>((const double *)ws, (double *)y, n, p.split, st);
+  else if (out_t == OWL_AMD_F32) run_splitk_reduce<float, float>((const float *)ws, (float *)y, n, p.split, st);
+  else if (out_t == OWL_AMD_BF16) run_splitk_reduce<float, bf16_t>((const float *)ws, (bf16_t *)y, n, p.split, st);
+  else run_splitk_reduce<float, f16_t>((const float *)ws, (f16_t *)y, n, p.split, st);
+  OWL_CHECK_HIP(hipGetLastError());
+  return OWL_AMD_OK;
+}
+
 // A strided forward whose dilation is a multiple of the stride along every
 // strided dimension (OWL_AMD_STRIDED_FWD_PHASE=0 turns the path off; read on
 // every call): x[s i + r a - p] always has residue c = (-p) mod s, so the
@@ -739,6 +804,10 @@ extern "C" int owl_amd_conv_fwd_bias_act(const owl_amd_conv_desc *d, int in_t, i
   const int v = vec_of(in_t);
   bool aligned = dtype_size(in_t) && p.Ci % v == 0 && p.Co % v == 0 && aligned16(x, w, y) &&
                  aligned16(pre, nullptr, nullptr);
+  if (!bias && act == OWL_AMD_ACT_NONE && !pre) {
+    const int rc2 = fwd_splitk(p, in_t, out_t, aligned, (hipStream_t)stream);
+    if (rc2 != -1) return rc2;
+  }
   if (try_fast<0>(p, in_t, out_t, aligned, (hipStream_t)stream)) {
     OWL_CHECK_HIP(hipGetLastError());
     return OWL_AMD_OK;
@@ -2034,11 +2103,11 @@ int conv_dropin(ConvPass pass, T *input, T *kernel, T *output, int64_t B, int64_
   const size_t ny = (size_t)(B * Ho * Wo * Do * Co) * sizeof(T);
   const int t = dt<T>();
   // complex kernels (expanded into one per-device buffer), strided backward
-  // inputs (their phase grids in per-device buffers) and strided forwards
-  // that read one phase of x (gathered into a per-device buffer): the whole
-  // call locked, not captured
+  // inputs (their phase grids in per-device buffers), strided forwards that
+  // read one phase of x or go through space-to-depth, and split-K forwards
+  // (per-device buffers all): the whole call locked, not captured
   const bool whole = is_complex(t) || (pass == BWD_IN && (strided_1x1_bwd_input(&d) || strided_phase_bwd_input(&d))) ||
-                     (pass == FWD && (strided_fwd_one_phase(&d) || s2d_uses_buffers(&d, t)));
+                     (pass == FWD && (strided_fwd_one_phase(&d) || s2d_uses_buffers(&d, t) || fwd_splitk_applies(&d, t)));
   const std::string key = dropin_key('c', (int)pass, t, d);
   const std::string *gk = whole ? nullptr : &key;
   if (pass == FWD) {

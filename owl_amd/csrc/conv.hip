@@ -718,7 +718,7 @@ template <int PASS, typename TI, typename TO>
 static void launch_igemm(const ConvParams &p, bool aligned, hipStream_t st) {
   typedef Tile<TI> TL;
   int64_t tiles = ((p.M + TL::BM - 1) / TL::BM) * (int64_t)((p.N + TL::BN - 1) / TL::BN);
-  const int64_t blocks = tiles * (PASS == 2 ? p.split : 1);
+  const int64_t blocks = tiles * (PASS == 2 || (PASS == 0 && p.split > 1) ? p.split : 1);
   dim3 grid((unsigned)blocks);
   if (aligned)
     launch_k((conv_igemm_kernel<PASS, TI, TO, true>), grid, dim3(NT), st, p, (int)tiles);

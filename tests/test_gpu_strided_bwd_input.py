@@ -207,3 +207,54 @@ def test_strided_fwd_space_to_depth(case):
     yb = _fwd_s2d(xt, wt, stride, pad, generic=False, bias=b)
     ref_b = np.maximum(ref + b.double().cpu().numpy(), 0.0)
     assert rel(yb, ref_b) < TOL[dt], rel(yb, ref_b)
+
+
+# Forwards with few output tiles over a long K run as split-K on the implicit
+# GEMM (FP32 / FP64 partials, the deterministic split-K reduction).
+SPLITK_CASES = [
+    ([16, 14, 14, 512], [5, 5, 512, 32], [1, 1], [2, 2], VALID, "bf16"),   # 576 outputs, K 12800
+    ([5, 7, 7, 512], [3, 3, 512, 256], [2, 2], [2, 2], VALID, "bf16"),     # through the one-phase forward
+    ([2, 9, 9, 256], [3, 3, 256, 64], [1, 1], [1, 1], SAME, "f32"),
+    ([3, 6, 6, 512], [3, 3, 512, 48], [1, 1], [1, 1], SAME, "f64"),
+]
+
+
+def _fwd_split(x, w, stride, rate, pad, generic):
+    old = os.environ.get("OWL_AMD_FWD_SPLITK")
+    if generic:
+        os.environ["OWL_AMD_FWD_SPLITK"] = "0"
+    try:
+        y = nd.dilated_conv2d(x, w, stride, rate, pad)
+        torch.cuda.synchronize()
+    finally:
+        if old is None:
+            os.environ.pop("OWL_AMD_FWD_SPLITK", None)
+        else:
+            os.environ["OWL_AMD_FWD_SPLITK"] = old
+    return y.double().cpu().numpy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(SPLITK_CASES)))
+def test_forward_split_k_for_few_tiles(case):
+    xs, ks, stride, rate, pad, dt = SPLITK_CASES[case]
+    print(f"split-K forward case {case} x{xs} w{ks} s{stride} r{rate} {dt}", flush=True)
+    td = TD[dt]
+    rng = np.random.default_rng(990 + case)
+    x = rng.standard_normal(xs)
+    w = rng.standard_normal(ks) / np.sqrt(np.prod(ks[:-1]))
+    rnd = lambda a: torch.from_numpy(a).to(td).to(torch.float64).numpy()
+    ref = on.dilated_conv2d(rnd(x), rnd(w), stride, rate, pad)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV).to(td)
+    xt, wt = dev(x), dev(w)
+    _lib.kernel_timing(True)
+    y = _fwd_split(xt, wt, stride, rate, pad, generic=False)
+    fwd = _lib.kernel_time(_lib.KT_FWD)[2]
+    aux = _lib.kernel_time(_lib.KT_AUX)[2]
+    _lib.kernel_timing(False)
+    assert fwd == "conv_igemm_kernel" and aux == "splitk_reduce_kernel", (fwd, aux)
+    assert y.shape == ref.shape
+    assert rel(y, ref) < TOL[dt], rel(y, ref)
+    yg = _fwd_split(xt, wt, stride, rate, pad, generic=True)
+    assert rel(y, yg) < TOL[dt], rel(y, yg)
+    assert np.array_equal(y, _fwd_split(xt, wt, stride, rate, pad, generic=False))
