@@ -2004,6 +2004,17 @@ bool halo_fits(const ConvParams &p, int pass, int in_bytes, int out_bytes) {
   return pass == 0 ? halo_geom<0>(p, in_bytes, out_bytes, g) : halo_geom<1>(p, in_bytes, out_bytes, g);
 }
 
+// Share of a halo launch's MFMA work on real output pixels and channels (0
+// when the problem does not fit the halo kernel): tiles hold whole rows of one
+// image, so small images leave most of a tile empty (7x7: 49 of 224 pixels).
+double halo_fill(const ConvParams &p, int pass, int in_bytes, int out_bytes) {
+  HaloGeom g;
+  const bool ok = pass == 0 ? halo_geom<0>(p, in_bytes, out_bytes, g) : halo_geom<1>(p, in_bytes, out_bytes, g);
+  if (!ok) return 0.0;
+  const double ch = (double)g.N / ((double)((g.N + g.bn - 1) / g.bn) * g.bn);
+  return ch * (double)g.Ho * g.Wo / ((double)((g.Ho + g.TR - 1) / g.TR) * halo_cap(g.bn, g.ms));
+}
+
 // Runs the halo kernel if the problem fits it; returns false otherwise.
 // The forward reads the weights in place ([tap][ci][co], transposing LDS
 // reads); the backward input reads them as [tap][n = ci][k = co] rows.

@@ -67,6 +67,8 @@ int fast_bk(int elem_bytes);
 template <int PASS, typename TI, typename TO>
 bool run_halo(const ConvParams &p, hipStream_t st);
 bool halo_fits(const ConvParams &p, int pass, int in_bytes, int out_bytes);
+// Share of a halo launch's MFMA work on real outputs (0 if it does not fit).
+double halo_fill(const ConvParams &p, int pass, int in_bytes, int out_bytes);
 // FP8 e4m3 forward on the halo kernel (conv_fast.hip): x as [..][Ci] bytes,
 // wp the kernel repacked to [tap][co][ci] (conv_fp8.hip), per-tensor scales
 // on the device; false when the layer does not fit (2-D, stride 1,

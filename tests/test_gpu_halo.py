@@ -169,3 +169,32 @@ def test_halo3d_vs_oracle(case, dtype, halo_edge):
     assert rel(y, ry) < 1e-2 and rel(dx, rdx) < 1e-2, (rel(y, ry), rel(dx, rdx))
     y2, dx2, _ = _run3(x, w, g, rate, pad, td, None)
     assert np.array_equal(y, y2) and np.array_equal(dx, dx2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["bf16", "f16"])
+def test_small_image_backward_input_leaves_the_halo_kernel(dtype):
+    """7x7 images fill 49 of a halo tile's 224 pixels: the 2-D backward input
+    runs on the direct-A kernel (tiles of pixels across images) instead, and
+    matches the oracle; the forward stays on the halo kernel."""
+    from owl_amd import _lib
+    td = torch.bfloat16 if dtype == "bf16" else torch.float16
+    rng = np.random.default_rng(77)
+    xs, ks = [6, 7, 7, 64], [3, 3, 64, 96]
+    x = rng.standard_normal(xs)
+    w = rng.standard_normal(ks) / np.sqrt(9 * 64)
+    g = rng.standard_normal([6, 7, 7, 96])
+    rnd = lambda a: torch.from_numpy(a).to(td).to(torch.float64).numpy()
+    ry = on.conv2d(rnd(x), rnd(w), [1, 1], SAME)
+    rdx = on.conv2d_backward_input(rnd(x), rnd(w), [1, 1], rnd(g))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV).to(td)
+    xt, wt, gt = dev(x), dev(w), dev(g)
+    _lib.kernel_timing(True)
+    y = nd.conv2d(xt, wt, [1, 1], SAME, out_dtype=torch.float32)
+    dx = nd.conv2d_backward_input(xt, wt, [1, 1], gt, out_dtype=torch.float32)
+    torch.cuda.synchronize()
+    names = (_lib.kernel_time(_lib.KT_FWD)[2], _lib.kernel_time(_lib.KT_BWD_INPUT)[2])
+    _lib.kernel_timing(False)
+    assert names == ("conv_halo_kernel", "conv_fast_kernel"), names
+    assert rel(y.double().cpu().numpy(), ry) < 1e-5
+    assert rel(dx.double().cpu().numpy(), rdx) < 1e-5
